@@ -1,0 +1,523 @@
+// C-ABI entry points: context, fields, conversions, fft2d.
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "ctx.hpp"
+
+namespace doe {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// exp(-2*pi*i*m/N) for m in [0, N): evaluated per octant in long double so
+// that symmetric entries are exact negations/swaps of each other.
+static void twiddle_host(int n, std::vector<long double>& c, std::vector<long double>& s) {
+  c.assign(n, 0.0L);
+  s.assign(n, 0.0L);
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int m = 0; m < n; ++m) {
+    // angle = 2*pi*m/n; reduce to the first octant of [0, 2*pi)
+    int q = (int)((8L * m) / n);  // octant 0..7
+    long double a;                // angle within the octant
+    long double cc, ss;
+    // r = m/n in [0,1); work with exact multiples of 1/(8n)
+    const long num = 8L * m - (long)q * n;  // in [0, n)
+    a = (pi / 4.0L) * (long double)num / (long double)n;
+    const long double ca = cosl(a), sa = sinl(a);
+    const long double h = 0.70710678118654752440084436210484903928L;
+    switch (q) {
+      case 0: cc = ca; ss = sa; break;
+      case 1: cc = h * (ca - sa); ss = h * (ca + sa); break;
+      case 2: cc = -sa; ss = ca; break;
+      case 3: cc = -h * (ca + sa); ss = h * (ca - sa); break;
+      case 4: cc = -ca; ss = -sa; break;
+      case 5: cc = -h * (ca - sa); ss = -h * (ca + sa); break;
+      case 6: cc = sa; ss = -ca; break;
+      default: cc = h * (ca + sa); ss = -h * (ca - sa); break;
+    }
+    c[m] = cc;
+    s[m] = -ss;  // forward kernel exp(-i*angle)
+  }
+  // exact values at the quarter points
+  c[0] = 1.0L; s[0] = 0.0L;
+  if (n % 4 == 0) { c[n / 4] = 0.0L; s[n / 4] = -1.0L; c[3 * n / 4] = 0.0L; s[3 * n / 4] = 1.0L; }
+  if (n % 2 == 0 && n >= 2) { c[n / 2] = -1.0L; s[n / 2] = 0.0L; }
+}
+
+int get_twiddle(doe_ctx* ctx, int n, int tsize, Twiddle* out) {
+  auto key = std::make_pair(n, tsize);
+  auto it = ctx->tw.find(key);
+  if (it != ctx->tw.end()) {
+    *out = it->second;
+    return DOE_OK;
+  }
+  std::vector<long double> c, s;
+  twiddle_host(n, c, s);
+  Twiddle t;
+  const size_t bytes = size_t(n) * tsize;
+  DOE_HIP(hipMalloc(&t.re, bytes));
+  DOE_HIP(hipMalloc(&t.im, bytes));
+  if (tsize == 8) {
+    std::vector<double> a(n), b(n);
+    for (int m = 0; m < n; ++m) { a[m] = (double)c[m]; b[m] = (double)s[m]; }
+    DOE_HIP(hipMemcpy(t.re, a.data(), bytes, hipMemcpyHostToDevice));
+    DOE_HIP(hipMemcpy(t.im, b.data(), bytes, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> a(n), b(n);
+    for (int m = 0; m < n; ++m) { a[m] = (float)c[m]; b[m] = (float)s[m]; }
+    DOE_HIP(hipMemcpy(t.re, a.data(), bytes, hipMemcpyHostToDevice));
+    DOE_HIP(hipMemcpy(t.im, b.data(), bytes, hipMemcpyHostToDevice));
+  }
+  ctx->tw[key] = t;
+  *out = t;
+  return DOE_OK;
+}
+
+hipEvent_t prof_event(doe_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev) {
+  // Identical parameter blocks (repeated calls with the same geometry) reuse
+  // the device copy without a transfer or a stream synchronisation.
+  if (ctx->dparams && bytes == ctx->params_host.size() &&
+      std::memcmp(ctx->params_host.data(), host, bytes) == 0) {
+    *dev = ctx->dparams;
+    return DOE_OK;
+  }
+  if (bytes > ctx->dparams_cap) {
+    if (ctx->dparams) {
+      DOE_HIP(hipStreamSynchronize(ctx->stream));
+      DOE_HIP(hipFree(ctx->dparams));
+    }
+    size_t cap = 4096;
+    while (cap < bytes) cap <<= 1;
+    DOE_HIP(hipMalloc(&ctx->dparams, cap));
+    ctx->dparams_cap = cap;
+  }
+  // Ordered on the context stream; the host buffer is consumed before return.
+  DOE_HIP(hipStreamSynchronize(ctx->stream));  // previous users of the block are done
+  DOE_HIP(hipMemcpyAsync(ctx->dparams, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->params_host.assign((const char*)host, (const char*)host + bytes);
+  *dev = ctx->dparams;
+  return DOE_OK;
+}
+
+int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf) {
+  if (n > ctx->partials_cap) {
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->partials) DOE_HIP(hipFree(ctx->partials));
+    size_t cap = 1024;
+    while (cap < n) cap <<= 1;
+    DOE_HIP(hipMalloc((void**)&ctx->partials, cap * sizeof(double)));
+    ctx->partials_cap = cap;
+  }
+  if (nf > ctx->counters_cap) {
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->counters) DOE_HIP(hipFree(ctx->counters));
+    size_t cap = 64;
+    while (cap < nf) cap <<= 1;
+    DOE_HIP(hipMalloc((void**)&ctx->counters, cap * sizeof(unsigned)));
+    DOE_HIP(hipMemset(ctx->counters, 0, cap * sizeof(unsigned)));
+    ctx->counters_cap = cap;
+  }
+  return DOE_OK;
+}
+
+int ensure_stage(doe_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->stage_cap) return DOE_OK;
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->stage) DOE_HIP(hipFree(ctx->stage));
+  size_t cap = 1 << 20;
+  while (cap < bytes) cap <<= 1;
+  DOE_HIP(hipMalloc(&ctx->stage, cap));
+  ctx->stage_cap = cap;
+  return DOE_OK;
+}
+
+template <typename S>
+__global__ void k_from_f64(const double* __restrict__ a, S* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) st<double>(b + i, a[i]);
+}
+template <typename S>
+__global__ void k_to_f64(const S* __restrict__ a, double* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = ld<double>(a + i);
+}
+template <typename S>
+__global__ void k_copy(const S* __restrict__ a, S* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+template <class F>
+static int with_dtype(int dt, F&& f) {
+  switch (dt) {
+    case DOE_F32: return f(float{});
+    case DOE_F64: return f(double{});
+    case DOE_F16: return f(__half{});
+    case DOE_BF16: return f(__hip_bfloat16{});
+    default: set_error("unknown dtype"); return DOE_ERR_INVALID;
+  }
+}
+
+static int convert_in(doe_ctx* ctx, const double* host, void* dst, long n, int dt) {
+  const size_t bytes = size_t(n) * sizeof(double);
+  if (dt == DOE_F64) {
+    DOE_HIP(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+    return DOE_OK;
+  }
+  int rc = ensure_stage(ctx, bytes);
+  if (rc) return rc;
+  DOE_HIP(hipMemcpyAsync(ctx->stage, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned grid = unsigned((n + 255) / 256);
+  return with_dtype(dt, [&](auto tag) -> int {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_from_f64<S>, dim3(grid), dim3(256), 0, ctx->stream, (const double*)ctx->stage,
+                       (S*)dst, n);
+    DOE_HIP(hipGetLastError());
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+    return DOE_OK;
+  });
+}
+
+static int convert_out(doe_ctx* ctx, const void* src, double* host, long n, int dt) {
+  const size_t bytes = size_t(n) * sizeof(double);
+  if (dt == DOE_F64) {
+    DOE_HIP(hipMemcpyAsync(host, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+    return DOE_OK;
+  }
+  int rc = ensure_stage(ctx, bytes);
+  if (rc) return rc;
+  const unsigned grid = unsigned((n + 255) / 256);
+  rc = with_dtype(dt, [&](auto tag) -> int {
+    using S = decltype(tag);
+    hipLaunchKernelGGL(k_to_f64<S>, dim3(grid), dim3(256), 0, ctx->stream, (const S*)src,
+                       (double*)ctx->stage, n);
+    DOE_HIP(hipGetLastError());
+    return DOE_OK;
+  });
+  if (rc) return rc;
+  DOE_HIP(hipMemcpyAsync(host, ctx->stage, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+int check_field(const doe_ctx* ctx, const doe_field* f) {
+  DOE_REQUIRE(ctx != nullptr, DOE_ERR_INVALID, "null context");
+  DOE_REQUIRE(f != nullptr, DOE_ERR_INVALID, "null field");
+  DOE_REQUIRE(is_pow2(f->nx) && is_pow2(f->ny) && f->nx >= 2 && f->ny >= 2 && f->nx <= kMaxN &&
+                  f->ny <= kMaxN,
+              DOE_ERR_UNSUPPORTED, "grid sides must be powers of two in [2, 8192]");
+  return DOE_OK;
+}
+
+int dispatch_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
+  switch (f->dtype) {
+    case DOE_F32: return run_fft2d<float, float>(ctx, f, inverse);
+    case DOE_F64: return run_fft2d<double, double>(ctx, f, inverse);
+    case DOE_F16: return run_fft2d<float, __half>(ctx, f, inverse);
+    case DOE_BF16: return run_fft2d<float, __hip_bfloat16>(ctx, f, inverse);
+  }
+  set_error("unknown dtype");
+  return DOE_ERR_INVALID;
+}
+
+}  // namespace doe
+
+using namespace doe;
+
+extern "C" {
+
+int doe_version(void) { return 100; }
+
+const char* doe_last_error(void) { return g_last_error.c_str(); }
+
+const char* doe_status_string(int s) {
+  switch (s) {
+    case DOE_OK: return "ok";
+    case DOE_ERR_INVALID: return "invalid argument";
+    case DOE_ERR_HIP: return "HIP runtime error";
+    case DOE_ERR_OOM: return "out of device memory";
+    case DOE_ERR_UNSUPPORTED: return "unsupported size or dtype";
+    case DOE_ERR_SHAPE: return "plane length does not match grid";
+    case DOE_ERR_STATE: return "invalid call order";
+    default: return "unknown status";
+  }
+}
+
+int doe_device_count(int* count) {
+  DOE_REQUIRE(count, DOE_ERR_INVALID, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return DOE_OK;
+}
+
+int doe_ctx_create(int device, void* stream, doe_ctx** out) {
+  DOE_REQUIRE(out, DOE_ERR_INVALID, "null out");
+  *out = nullptr;
+  int c = 0;
+  DOE_HIP(hipGetDeviceCount(&c));
+  DOE_REQUIRE(device >= 0 && device < c, DOE_ERR_INVALID, "no such HIP device");
+  DOE_HIP(hipSetDevice(device));
+  doe_ctx* ctx = new doe_ctx();
+  ctx->device = device;
+  if (stream) {
+    ctx->stream = (hipStream_t)stream;
+    ctx->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete ctx;
+      set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+      return DOE_ERR_HIP;
+    }
+    ctx->own_stream = true;
+  }
+  *out = ctx;
+  return DOE_OK;
+}
+
+int doe_ctx_destroy(doe_ctx* ctx) {
+  if (!ctx) return DOE_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->tw) {
+    hipFree(kv.second.re);
+    hipFree(kv.second.im);
+  }
+  if (ctx->dparams) hipFree(ctx->dparams);
+  if (ctx->stage) hipFree(ctx->stage);
+  if (ctx->partials) hipFree(ctx->partials);
+  if (ctx->counters) hipFree(ctx->counters);
+  for (auto& r : ctx->prof) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto e : ctx->ev_pool) hipEventDestroy(e);
+  if (ctx->t0) hipEventDestroy(ctx->t0);
+  if (ctx->t1) hipEventDestroy(ctx->t1);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DOE_OK;
+}
+
+int doe_ctx_sync(doe_ctx* ctx) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+int doe_ctx_stream(doe_ctx* ctx, void** stream) {
+  DOE_REQUIRE(ctx && stream, DOE_ERR_INVALID, "null argument");
+  *stream = (void*)ctx->stream;
+  return DOE_OK;
+}
+
+int doe_ctx_set_flags(doe_ctx* ctx, int flags) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->flags = flags;
+  return DOE_OK;
+}
+
+int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap) {
+  DOE_REQUIRE(ctx && buf && cap > 0, DOE_ERR_INVALID, "null argument");
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  std::map<std::string, std::pair<long, double>> agg;
+  std::vector<std::string> order;
+  for (auto& r : ctx->prof) {
+    float ms = 0.f;
+    DOE_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    auto it = agg.find(r.tag);
+    if (it == agg.end()) {
+      order.push_back(r.tag);
+      agg[r.tag] = {1, ms};
+    } else {
+      it->second.first += 1;
+      it->second.second += ms;
+    }
+    ctx->ev_pool.push_back(r.a);
+    ctx->ev_pool.push_back(r.b);
+  }
+  ctx->prof.clear();
+  std::string out;
+  for (auto& k : order) {
+    char line[256];
+    snprintf(line, sizeof line, "%s %ld %.6f\n", k.c_str(), agg[k].first, agg[k].second);
+    out += line;
+  }
+  DOE_REQUIRE((long)out.size() < cap, DOE_ERR_INVALID, "report buffer too small");
+  std::memcpy(buf, out.c_str(), out.size() + 1);
+  return DOE_OK;
+}
+
+int doe_timer_start(doe_ctx* ctx) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  if (!ctx->t0) DOE_HIP(hipEventCreate(&ctx->t0));
+  if (!ctx->t1) DOE_HIP(hipEventCreate(&ctx->t1));
+  DOE_HIP(hipEventRecord(ctx->t0, ctx->stream));
+  return DOE_OK;
+}
+
+int doe_timer_stop(doe_ctx* ctx, double* ms) {
+  DOE_REQUIRE(ctx && ms && ctx->t0 && ctx->t1, DOE_ERR_STATE, "doe_timer_stop without start");
+  DOE_HIP(hipEventRecord(ctx->t1, ctx->stream));
+  DOE_HIP(hipEventSynchronize(ctx->t1));
+  float f = 0.f;
+  DOE_HIP(hipEventElapsedTime(&f, ctx->t0, ctx->t1));
+  *ms = f;
+  return DOE_OK;
+}
+
+int doe_field_create(doe_ctx* ctx, int nx, int ny, int batch, int dtype, doe_field** out) {
+  DOE_REQUIRE(ctx && out, DOE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  DOE_REQUIRE(dtype_size(dtype) > 0, DOE_ERR_INVALID, "unknown dtype");
+  DOE_REQUIRE(batch >= 1, DOE_ERR_INVALID, "batch must be >= 1");
+  doe_field* f = new doe_field();
+  f->nx = nx;
+  f->ny = ny;
+  f->batch = batch;
+  f->dtype = dtype;
+  int rc = check_field(ctx, f);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  const size_t bytes = f->count() * dtype_size(dtype);
+  if (hipMalloc(&f->re, bytes) != hipSuccess || hipMalloc(&f->im, bytes) != hipSuccess) {
+    if (f->re) hipFree(f->re);
+    delete f;
+    (void)hipGetLastError();
+    set_error("doe_field_create: device allocation failed");
+    return DOE_ERR_OOM;
+  }
+  f->owned = true;
+  *out = f;
+  return DOE_OK;
+}
+
+int doe_field_wrap(doe_ctx* ctx, void* re, void* im, int nx, int ny, int batch, int dtype,
+                   doe_field** out) {
+  DOE_REQUIRE(ctx && out && re && im, DOE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  DOE_REQUIRE(dtype_size(dtype) > 0, DOE_ERR_INVALID, "unknown dtype");
+  DOE_REQUIRE(batch >= 1, DOE_ERR_INVALID, "batch must be >= 1");
+  doe_field* f = new doe_field();
+  f->nx = nx;
+  f->ny = ny;
+  f->batch = batch;
+  f->dtype = dtype;
+  f->re = re;
+  f->im = im;
+  f->owned = false;
+  int rc = check_field(ctx, f);
+  if (rc) {
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return DOE_OK;
+}
+
+int doe_field_free(doe_field* f) {
+  if (!f) return DOE_OK;
+  if (f->owned) {
+    hipFree(f->re);
+    hipFree(f->im);
+  }
+  delete f;
+  return DOE_OK;
+}
+
+int doe_field_info(const doe_field* f, int* nx, int* ny, int* batch, int* dtype, void** re,
+                   void** im) {
+  DOE_REQUIRE(f, DOE_ERR_INVALID, "null field");
+  if (nx) *nx = f->nx;
+  if (ny) *ny = f->ny;
+  if (batch) *batch = f->batch;
+  if (dtype) *dtype = f->dtype;
+  if (re) *re = f->re;
+  if (im) *im = f->im;
+  return DOE_OK;
+}
+
+int doe_field_upload(doe_ctx* ctx, doe_field* f, const double* re, const double* im) {
+  int rc = check_field(ctx, f);
+  if (rc) return rc;
+  DOE_REQUIRE(re && im, DOE_ERR_INVALID, "null host plane");
+  DOE_HIP(hipSetDevice(ctx->device));
+  rc = convert_in(ctx, re, f->re, (long)f->count(), f->dtype);
+  if (rc) return rc;
+  return convert_in(ctx, im, f->im, (long)f->count(), f->dtype);
+}
+
+int doe_field_download(doe_ctx* ctx, doe_field* f, double* re, double* im) {
+  int rc = check_field(ctx, f);
+  if (rc) return rc;
+  DOE_REQUIRE(re && im, DOE_ERR_INVALID, "null host plane");
+  DOE_HIP(hipSetDevice(ctx->device));
+  rc = convert_out(ctx, f->re, re, (long)f->count(), f->dtype);
+  if (rc) return rc;
+  return convert_out(ctx, f->im, im, (long)f->count(), f->dtype);
+}
+
+int doe_field_copy(doe_ctx* ctx, const doe_field* src, doe_field* dst) {
+  int rc = check_field(ctx, src);
+  if (rc) return rc;
+  rc = check_field(ctx, dst);
+  if (rc) return rc;
+  DOE_REQUIRE(src->nx == dst->nx && src->ny == dst->ny && src->batch == dst->batch &&
+                  src->dtype == dst->dtype,
+              DOE_ERR_SHAPE, "doe_field_copy: shape mismatch");
+  const size_t bytes = src->count() * dtype_size(src->dtype);
+  DOE_HIP(hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  DOE_HIP(hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
+  int rc = check_field(ctx, f);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(ctx->device));
+  rc = dispatch_fft2d(ctx, f, inverse);
+  if (rc) return rc;
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_fft2d_host(doe_ctx* ctx, double* re, double* im, long len, int nx, int ny, int inverse,
+                   int dtype) {
+  DOE_REQUIRE(ctx && re && im, DOE_ERR_INVALID, "null argument");
+  if (len != (long)nx * (long)ny) {
+    set_error("fft2d: plane length " + std::to_string(len) + " does not match " +
+              std::to_string(nx) + "x" + std::to_string(ny));
+    return DOE_ERR_SHAPE;
+  }
+  doe_field* f = nullptr;
+  int rc = doe_field_create(ctx, nx, ny, 1, dtype, &f);
+  if (rc) return rc;
+  rc = doe_field_upload(ctx, f, re, im);
+  if (!rc) rc = doe_fft2d(ctx, f, inverse);
+  if (!rc) rc = doe_field_download(ctx, f, re, im);
+  doe_field_free(f);
+  return rc;
+}
+
+}  // extern "C"

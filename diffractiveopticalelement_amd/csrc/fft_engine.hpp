@@ -1,0 +1,251 @@
+// Register-resident, LDS-exchanged Stockham FFT engine for one line of a
+// power-of-two 2-D grid (a row or a column).
+//
+// This replaces the reference's radix-2 DIT fft1d (src/fft.js:86-139) with a
+// mixed-radix (16/8/4/2) Stockham auto-sort formulation: each line of N points
+// is held by T = N/P threads, P points per thread.  A stage of radix R
+// performs P/R radix-R butterflies per thread entirely in registers; between
+// stages the line is exchanged through LDS.  Conventions match the reference:
+// forward is the unscaled DFT with exp(-2*pi*i*n*k/N), the inverse conjugates
+// the kernel and scales by 1/N.
+//
+// Register convention (all positions are indices along the line):
+//   entry:  v[b*R0 + r] = x[t + b*T + r*(N/R0)]      (R0 = first radix)
+//   exit:   v[b*RL + q] = X[t + b*T + q*(N/RL)]      (RL = last radix)
+// Running the inverse with the stage order reversed (REV) makes its entry
+// positions equal the forward's exit positions, so a forward FFT, a pointwise
+// operation and an inverse FFT chain in registers without an extra exchange.
+#pragma once
+#include "common.hpp"
+
+namespace doe {
+
+// Multiply (xr + i xi) by exp(-/+ 2*pi*i*m/16): minus for the forward
+// transform, plus for the inverse.
+template <int M, bool INV, typename T>
+__device__ __forceinline__ void mul_w16(T& xr, T& xi) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) {
+    return;
+  } else if constexpr (m == 8) {
+    xr = -xr;
+    xi = -xi;
+  } else if constexpr (m == 4 || m == 12) {
+    // exp(-i*pi/2) = -i (forward, m=4); the other three cases follow.
+    constexpr bool times_minus_i = (m == 4) != INV;
+    const T r = xr;
+    if constexpr (times_minus_i) {
+      xr = xi;
+      xi = -r;
+    } else {
+      xr = -xi;
+      xi = r;
+    }
+  } else {
+    constexpr T c = T(c16(m));
+    constexpr T s = INV ? T(s16(m)) : T(-s16(m));
+    const T r = xr * c - xi * s;
+    xi = xr * s + xi * c;
+    xr = r;
+  }
+}
+
+// In-register radix-R DFT (R <= 16), natural order in and out.
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void dft(T* vr, T* vi) {
+  if constexpr (R > 1) {
+    constexpr int L = ilog2(R);
+    T ar[R], ai[R];
+    static_for<0, R>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      ar[bitrev(i, L)] = vr[i];
+      ai[bitrev(i, L)] = vi[i];
+    });
+    static_for<1, L + 1>([&](auto S) {
+      constexpr int len = 1 << decltype(S)::value;
+      constexpr int half = len >> 1;
+      static_for<0, R / 2>([&](auto B) {
+        constexpr int b = decltype(B)::value;
+        constexpr int k = b % half;
+        constexpr int p = (b / half) * len + k;
+        constexpr int q = p + half;
+        T tr = ar[q], ti = ai[q];
+        mul_w16<k * (16 / len), INV>(tr, ti);
+        ar[q] = ar[p] - tr;
+        ai[q] = ai[p] - ti;
+        ar[p] = ar[p] + tr;
+        ai[p] = ai[p] + ti;
+      });
+    });
+    static_for<0, R>([&](auto I) {
+      constexpr int i = decltype(I)::value;
+      vr[i] = ar[i];
+      vi[i] = ai[i];
+    });
+  }
+}
+
+// Stage plan for a line of N points with P points per thread: full stages of
+// radix P, then one remainder stage.  REV reverses the order.
+template <int N, int P>
+struct LinePlan {
+  static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
+  static_assert(P <= N && P <= 16 && (P & (P - 1)) == 0, "bad P");
+  static constexpr int T = N / P;
+  static constexpr int LOGN = ilog2(N);
+  static constexpr int LOGP = ilog2(P);
+  static constexpr int NFULL = LOGN / LOGP;
+  static constexpr int REM = 1 << (LOGN - NFULL * LOGP);
+  static constexpr int NST = NFULL + (REM > 1 ? 1 : 0);
+  static constexpr int radix_fwd(int s) { return s < NFULL ? P : REM; }
+  template <bool REV>
+  static constexpr int radix(int s) { return REV ? radix_fwd(NST - 1 - s) : radix_fwd(s); }
+  template <bool REV>
+  static constexpr int ns(int s) {  // product of the radices before stage s
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= radix<REV>(i);
+    return p;
+  }
+  template <bool REV>
+  static constexpr int first() { return radix<REV>(0); }
+  template <bool REV>
+  static constexpr int last() { return radix<REV>(NST - 1); }
+  // Position (along the line) held by register i of thread t at entry/exit.
+  template <bool REV>
+  __device__ __forceinline__ static int in_pos(int t, int i) {
+    constexpr int R = first<REV>();
+    return t + (i / R) * T + (i % R) * (N / R);
+  }
+  template <bool REV>
+  __device__ __forceinline__ static int out_pos(int t, int i) {
+    constexpr int R = last<REV>();
+    return t + (i / R) * T + (i % R) * (N / R);
+  }
+};
+
+// One full line FFT.  `lds_re/lds_im` point at this line's exchange area and
+// `lidx(pos)` maps a line position to an LDS index.  Every thread of the
+// workgroup must call this (it contains barriers), active or not.
+// Twiddle table: tw[m] = exp(-2*pi*i*m/N), m in [0, N), planar re/im.
+template <typename T, int N, int P, bool INV, bool REV, class LIdx>
+__device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_re, T* lds_im,
+                                         const LIdx& lidx, const T* __restrict__ twr,
+                                         const T* __restrict__ twi) {
+  using PL = LinePlan<N, P>;
+  constexpr int TT = PL::T;
+  static_for<0, PL::NST>([&](auto SI) {
+    constexpr int s = decltype(SI)::value;
+    constexpr int R = PL::template radix<REV>(s);
+    constexpr int NS = PL::template ns<REV>(s);
+    constexpr int NB = P / R;
+    if constexpr (s > 0) {
+      constexpr int RP = PL::template radix<REV>(s - 1);
+      constexpr int NSP = PL::template ns<REV>(s - 1);
+      constexpr int NBP = P / RP;
+      __syncthreads();
+      static_for<0, NBP>([&](auto BI) {
+        constexpr int b = decltype(BI)::value;
+        const int j = t + b * TT;
+        const int base = (j / NSP) * NSP * RP + (j % NSP);
+        static_for<0, RP>([&](auto QI) {
+          constexpr int q = decltype(QI)::value;
+          const int li = lidx(base + q * NSP);
+          lds_re[li] = vr[b * RP + q];
+          lds_im[li] = vi[b * RP + q];
+        });
+      });
+      __syncthreads();
+      static_for<0, NB>([&](auto BI) {
+        constexpr int b = decltype(BI)::value;
+        const int j = t + b * TT;
+        static_for<0, R>([&](auto RI) {
+          constexpr int r = decltype(RI)::value;
+          const int li = lidx(j + r * (N / R));
+          vr[b * R + r] = lds_re[li];
+          vi[b * R + r] = lds_im[li];
+        });
+      });
+    }
+    static_for<0, NB>([&](auto BI) {
+      constexpr int b = decltype(BI)::value;
+      if constexpr (NS > 1) {
+        const int j = t + b * TT;
+        const int m = (j % NS) * (N / (NS * R));
+        T wr[R], wi[R];
+        wr[1] = twr[m];
+        wi[1] = INV ? -twi[m] : twi[m];
+        static_for<2, R>([&](auto RI) {
+          constexpr int r = decltype(RI)::value;
+          constexpr int a = (r % 2 == 0) ? r / 2 : r - 1;
+          constexpr int c = (r % 2 == 0) ? r / 2 : 1;
+          wr[r] = wr[a] * wr[c] - wi[a] * wi[c];
+          wi[r] = wr[a] * wi[c] + wi[a] * wr[c];
+        });
+        static_for<1, R>([&](auto RI) {
+          constexpr int r = decltype(RI)::value;
+          const T xr = vr[b * R + r], xi = vi[b * R + r];
+          vr[b * R + r] = xr * wr[r] - xi * wi[r];
+          vi[b * R + r] = xr * wi[r] + xi * wr[r];
+        });
+      }
+      dft<R, INV>(vr + b * R, vi + b * R);
+    });
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Launch geometry.
+// Rows: T = N/P threads per line, LINES lines per workgroup (>= 256 threads).
+// Columns: C adjacent columns per workgroup, whole column resident; the tile
+// is sized so C*N complex values fit the LDS budget.
+// ---------------------------------------------------------------------------
+template <int N>
+struct PChoice {
+  static constexpr int P = N < 16 ? N : 16;
+};
+
+template <typename T, int N>
+struct RowCfg {
+  static constexpr int P = PChoice<N>::P;
+  static constexpr int T_LINE = N / P;
+  static constexpr int LINES = cmax(1, 256 / T_LINE);
+  static constexpr int WG = LINES * T_LINE;
+  static constexpr int LSTRIDE = N + N / 16 + 1;  // padded line stride in LDS
+  static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
+  static constexpr int LDS_ELEMS = NEED_LDS ? LINES * LSTRIDE : 1;
+};
+
+template <typename T, int N>
+struct ColCfg {
+  static constexpr int P = PChoice<N>::P;
+  static constexpr int T_LINE = N / P;
+  static constexpr int BUDGET = sizeof(T) == 4 ? 16384 : 8192;  // complex values in LDS
+  static constexpr int C = cmax(1, cmin(64, cmin(BUDGET / N, 1024 / T_LINE)));
+  static constexpr int WG = C * T_LINE;
+  static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
+  static constexpr int LDS_ELEMS = NEED_LDS ? (C * N + (C * N) / 32 + 1) : 1;
+};
+
+// LDS index maps.
+struct RowLIdx {
+  int base;
+  __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+};
+template <int C>
+struct ColLIdx {
+  int col;
+  __device__ __forceinline__ int operator()(int pos) const {
+    const int a = pos * C + col;
+    return a + (a >> 5);
+  }
+};
+
+// Remap a linear workgroup id so that consecutive logical tiles land on the
+// same XCD (hardware deals workgroups round-robin over the 8 XCDs), which
+// lets neighbouring column tiles share L2 lines.
+__device__ __forceinline__ int xcd_remap(int b, int g) {
+  if ((g & 7) != 0) return b;
+  return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+}  // namespace doe

@@ -1,0 +1,751 @@
+// Phase retrieval: Gerchberg–Saxton / weighted GS in the image-plane
+// (angular-spectrum) and far-field (unitary DFT) geometries.
+//
+// ref: src/doe.js:327-437 gerchbergSaxton, 455-546 weightedGerchbergSaxton,
+//      639-730 fourierCgh.
+//
+// Far-field loop (2 passes per iteration).  With c = (-1)^(x+y) and
+// g = (-1)^(nx/2+ny/2), fftshift.F.ifftshift(x) = g*c.*F(c.*x) at the same
+// index, so the loop runs on x~ = c.*x: the spectrum F(x~) is already in the
+// reference's centred index space (target/ROI/weights need no remap), and the
+// update x~' = illum .* z/|z| with z = F^-1(Y~') carries no checkerboard at
+// all.  Pass B (columns): F -> measure, project, WGS update -> F^-1.
+// Pass A (rows): F^-1 -> phase, re-impose illumination -> F.
+//
+// Image-plane loop.  The two angular-spectrum propagations of an iteration
+// are split into column passes (spatial-domain work fused with the column
+// FFTs) and row passes (row FFT, x H or conj H, inverse row FFT).  Under WGS
+// the weight renormalisation is a grid-wide scalar needed before the
+// projection, so the measurement (A1a) and projection (A1b) are separate
+// column passes.
+#include <cmath>
+#include <limits>
+#include <vector>
+
+#include "ctx.hpp"
+#include "ops_prop.hpp"
+
+namespace doe {
+
+int check_field(const doe_ctx* ctx, const doe_field* f);
+
+// Per-design device state (one per field of a batch).
+struct GsState {
+  double best_rmse;
+  double nrm;  // weight normalisation: w = w' * nrm
+  int best_phase_slot;
+  int best_amp_slot;
+  int improved;
+  int pad;
+};
+
+template <typename T>
+struct GsBuf {
+  const T* illum;
+  const T* tgt;
+  const T* roi;  // nullable: whole plane
+  T* w;          // unnormalised weights w' (nullable for GS)
+  T* phase[2];
+  T* amp[2];
+  GsState* state;
+  double* err_hist;
+  double* eff_hist;
+  int nx, ny;
+  double dxdy_x, dxdy_y;  // dx, dy for the image-plane efficiency expression
+  double norm;            // far field: 1/sqrt(nx*ny)
+  double gsign;           // far field: (-1)^(nx/2+ny/2)
+  double beta;
+  double feedback;
+  int escape;
+  int weighted;
+};
+
+// reduction slots
+enum { R_ERRNUM, R_ERRDEN, R_INROI, R_TOTAL, R_RNUM, R_RDEN, R_WSUM, R_TSUM, R_N };
+
+template <typename T>
+__device__ __forceinline__ T clampv(T v, T lo, T hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// WGS weight update of doe.js:514-521 for one ROI pixel; returns w'.
+template <typename T>
+__device__ __forceinline__ T wgs_update(T w, T measured, T desired, T beta) {
+  const T ratio = measured > T(1e-30) ? desired / measured : T(2);
+  T wn = w * pow(clampv(ratio, T(0.25), T(4)), beta);
+  return clampv(wn, T(1e-4), T(1e4));
+}
+
+// ---------------------------------------------------------------------------
+// Far field
+// ---------------------------------------------------------------------------
+// Initial row pass: x~ = c .* illum .* exp(i*phi0), then row FFT.
+template <typename T>
+struct OpFFInit {
+  static constexpr int NACC = 0;
+  GsBuf<T> b;
+  const T* phi0;
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int x, int y, U& re, U& im, double*) const {
+    const long k = (long)y * b.nx + x;
+    const T il = b.illum[k];
+    T s, c;
+    if constexpr (sizeof(T) == 8) sincos(phi0[k], &s, &c);
+    else sincosf(phi0[k], &s, &c);
+    const T cb = ((x + y) & 1) ? T(-1) : T(1);
+    re = cb * il * c;
+    im = cb * il * s;
+  }
+  template <typename U>
+  __device__ __forceinline__ void mid(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+};
+
+// Column pass B: measurement, projection and WGS update (doe.js:675-712 and
+// 490-521) on the centred-index spectrum.
+template <typename T>
+struct OpFFB {
+  static constexpr int NACC = R_N;
+  GsBuf<T> b;
+  double* partials;
+  unsigned* counters;
+  int it;
+  int write_amp;  // GS: last iteration writes |Y| to amp[0]
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double* acc) const {
+    const long k = (long)y * b.nx + x;
+    const T a = sqrt(re * re + im * im) * T(b.norm);
+    const T m = b.roi ? b.roi[k] : T(1);
+    const T t0 = b.tgt[k];
+    T w = T(1);
+    if (b.weighted) w = b.w[k] * T(b.state->nrm);
+    const T t = t0 * w;
+    acc[R_TOTAL] += double(a) * double(a);
+    if (m > T(0)) {
+      acc[R_INROI] += double(a) * double(a) * double(m);
+      const double d = (double(a) - double(t)) * double(m);
+      acc[R_ERRNUM] += d * d;
+      acc[R_ERRDEN] += double(t) * double(t);
+      if (b.weighted) {
+        const double d0 = double(a) - double(t0);
+        acc[R_RNUM] += d0 * d0;
+        acc[R_RDEN] += double(t0) * double(t0);
+        const T wn = wgs_update<T>(w, a, t0, T(b.beta));
+        b.w[k] = wn;
+        acc[R_WSUM] += double(wn) * double(wn) * double(t0) * double(t0);
+        acc[R_TSUM] += double(t0) * double(t0);
+      }
+    }
+    if (b.weighted) {
+      b.amp[1 - b.state->best_amp_slot][k] = a;
+    } else if (write_amp) {
+      b.amp[0][k] = a;
+    }
+    const T tp = m > T(0) ? t : a;
+    if (a > T(0)) {
+      const T s = tp / a;
+      re *= s;
+      im *= s;
+    } else {
+      const T cb = ((x + y) & 1) ? T(-1) : T(1);
+      re = T(b.gsign) * cb * tp / T(b.norm);
+      im = T(0);
+    }
+  }
+  __device__ __forceinline__ void finalize(int, const double* s) const {
+    GsState* st = b.state;
+    const double err = s[R_ERRDEN] > 0 ? sqrt(s[R_ERRNUM] / s[R_ERRDEN]) : 0.0;
+    const double eff = s[R_TOTAL] > 0 ? s[R_INROI] / s[R_TOTAL] : 0.0;
+    if (b.weighted) {
+      const double rmse = s[R_RDEN] > 0 ? sqrt(s[R_RNUM] / s[R_RDEN]) : 0.0;
+      if (rmse < st->best_rmse) {
+        st->best_rmse = rmse;
+        st->improved = 1;
+        st->best_amp_slot = 1 - st->best_amp_slot;
+      } else {
+        st->improved = 0;
+      }
+      st->nrm = s[R_WSUM] > 0 ? sqrt(s[R_TSUM] / s[R_WSUM]) : 1.0;
+      b.err_hist[it] = st->best_rmse;
+    } else {
+      b.err_hist[it] = err;
+    }
+    b.eff_hist[it] = eff;
+  }
+};
+
+// Row pass A: phase of the inverse transform, illumination re-imposed.
+template <typename T>
+struct OpFFA {
+  static constexpr int NACC = 0;
+  GsBuf<T> b;
+  int write_phase;  // GS: last iteration; WGS: when the iterate improved
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double*) const {
+    const long k = (long)y * b.nx + x;
+    const T cb = ((x + y) & 1) ? T(-1) : T(1);
+    const bool wp = b.weighted ? (b.state->improved != 0) : (write_phase != 0);
+    if (wp) b.phase[0][k] = atan2(cb * im, cb * re);
+    const T r = sqrt(re * re + im * im);
+    const T il = b.illum[k];
+    if (r > T(0)) {
+      const T s = il / r;
+      re *= s;
+      im *= s;
+    } else {
+      re = cb * il;
+      im = T(0);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Image plane
+// ---------------------------------------------------------------------------
+// Initial column pass: x = illum .* exp(i*phi0), then column FFT.
+template <typename T>
+struct OpIPInit {
+  static constexpr int NACC = 0;
+  GsBuf<T> b;
+  const T* phi0;
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int x, int y, U& re, U& im, double*) const {
+    const long k = (long)y * b.nx + x;
+    const T il = b.illum[k];
+    T s, c;
+    if constexpr (sizeof(T) == 8) sincos(phi0[k], &s, &c);
+    else sincosf(phi0[k], &s, &c);
+    re = il * c;
+    im = il * s;
+  }
+  template <typename U>
+  __device__ __forceinline__ void mid(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+};
+
+// GS measurement + projection of doe.js:359-397 on the target-plane field U
+// with weights w = w' * nrm (constant during the pass).
+template <typename T>
+__device__ __forceinline__ void ip_measure_project(const GsBuf<T>& b, int it, long k, T& re, T& im,
+                                                   double* acc) {
+  const T a = sqrt(re * re + im * im);
+  const T i2 = a * a;
+  const T m = b.roi ? b.roi[k] : T(1);
+  T w = T(1);
+  if (b.weighted) w = b.w[k] * T(b.state->nrm);
+  const T t = b.tgt[k] * w;
+  acc[R_TOTAL] += double(i2);
+  if (m > T(0)) {
+    acc[R_INROI] += double(i2) * double(m);
+    if (t > T(0) || i2 > T(0)) {
+      const double d = double(a) * double(m) - double(t) * double(m);
+      acc[R_ERRNUM] += d * d;
+      acc[R_ERRDEN] += double(t) * double(t);
+    }
+  }
+  const T fb = it >= b.escape ? T(b.feedback) : T(0);
+  T na;
+  if (m > T(0)) {
+    na = t;
+    if (fb > T(0) && a > T(0)) na = t + fb * (a - t);
+  } else {
+    na = a;  // a*(1-fb) + fb*a: the field is left as it is outside the ROI
+  }
+  if (a > T(0)) {
+    const T s = na / a;
+    re *= s;
+    im *= s;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void ip_finalize_measure(const GsBuf<T>& b, int it, const double* s) {
+#pragma clang fp contract(off)
+  const double err = s[R_ERRDEN] > 0 ? sqrt(s[R_ERRNUM] / s[R_ERRDEN]) : 0.0;
+  const double eff =
+      s[R_TOTAL] > 0 ? (s[R_INROI] * b.dxdy_x * b.dxdy_y) / (s[R_TOTAL] * b.dxdy_x * b.dxdy_y) : 0.0;
+  if (!b.weighted) b.err_hist[it] = err;
+  b.eff_hist[it] = eff;
+}
+
+// A1 (GS, or WGS iteration 0): column F^-1 -> measure + project -> column F.
+template <typename T>
+struct OpIPMP {
+  static constexpr int NACC = R_N;
+  GsBuf<T> b;
+  double* partials;
+  unsigned* counters;
+  int it;
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double* acc) const {
+    ip_measure_project<T>(b, it, (long)y * b.nx + x, re, im, acc);
+  }
+  __device__ __forceinline__ void finalize(int, const double* s) const { ip_finalize_measure<T>(b, it, s); }
+};
+
+// A1b (WGS): the pass loads the stored target-plane field U; measure +
+// project with the renormalised weights, then column F.
+template <typename T>
+struct OpIPProj {
+  static constexpr int NACC = R_N;
+  GsBuf<T> b;
+  double* partials;
+  unsigned* counters;
+  int it;
+  __device__ __forceinline__ void pre(int, int x, int y, T& re, T& im, double* acc) const {
+    ip_measure_project<T>(b, it, (long)y * b.nx + x, re, im, acc);
+  }
+  template <typename U>
+  __device__ __forceinline__ void mid(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void finalize(int, const double* s) const { ip_finalize_measure<T>(b, it, s); }
+};
+
+// A1a (WGS, and the closing pass of GS): column F^-1 -> U stored spatially;
+// amplitude of the reported iterate; WGS best-iterate rmse of the previous
+// iteration (prev_it) and the weight update (doe.js:490-524).
+template <typename T>
+struct OpIPMeas {
+  static constexpr int NACC = R_N;
+  GsBuf<T> b;
+  double* partials;
+  unsigned* counters;
+  int prev_it;
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void mid(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void post(int, int x, int y, T& re, T& im, double* acc) const {
+    const long k = (long)y * b.nx + x;
+    const T a = sqrt(re * re + im * im);
+    if (!b.weighted) {
+      b.amp[0][k] = a;
+      return;
+    }
+    b.amp[1 - b.state->best_amp_slot][k] = a;
+    const T m = b.roi ? b.roi[k] : T(1);
+    if (m > T(0)) {
+      const T t0 = b.tgt[k];
+      const double d0 = double(a) - double(t0);
+      acc[R_RNUM] += d0 * d0;
+      acc[R_RDEN] += double(t0) * double(t0);
+      const T w = b.w[k] * T(b.state->nrm);
+      const T wn = wgs_update<T>(w, a, t0, T(b.beta));
+      b.w[k] = wn;
+      acc[R_WSUM] += double(wn) * double(wn) * double(t0) * double(t0);
+      acc[R_TSUM] += double(t0) * double(t0);
+    }
+  }
+  __device__ __forceinline__ void finalize(int, const double* s) const {
+    if (!b.weighted) return;
+    GsState* st = b.state;
+    const double rmse = s[R_RDEN] > 0 ? sqrt(s[R_RNUM] / s[R_RDEN]) : 0.0;
+    if (rmse < st->best_rmse) {
+      st->best_rmse = rmse;
+      st->best_phase_slot = 1 - st->best_phase_slot;
+      st->best_amp_slot = 1 - st->best_amp_slot;
+    }
+    st->nrm = s[R_WSUM] > 0 ? sqrt(s[R_TSUM] / s[R_WSUM]) : 1.0;
+    b.err_hist[prev_it] = st->best_rmse;
+  }
+};
+
+// A2: column F^-1 -> phase -> illumination re-imposed -> column F.
+template <typename T>
+struct OpIPPhase {
+  static constexpr int NACC = 0;
+  GsBuf<T> b;
+  template <typename U>
+  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
+  template <typename U>
+  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
+  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double*) const {
+    const long k = (long)y * b.nx + x;
+    const int slot = b.weighted ? 1 - b.state->best_phase_slot : 0;
+    b.phase[slot][k] = atan2(im, re);
+    const T r = sqrt(re * re + im * im);
+    const T il = b.illum[k];
+    if (r > T(0)) {
+      const T s = il / r;
+      re *= s;
+      im *= s;
+    } else {
+      re = il;  // atan2(0, 0) = 0
+      im = T(0);
+    }
+  }
+};
+
+template <typename T>
+__global__ void k_gs_init_state(GsState* st, T* w, const T* roi, long n, int weighted) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    st->best_rmse = __builtin_inf();
+    st->nrm = 1.0;
+    st->best_phase_slot = 0;
+    st->best_amp_slot = 0;
+    st->improved = 0;
+    st->pad = 0;
+  }
+  if (weighted && i < n) w[i] = (roi ? (roi[i] > T(0) ? T(1) : T(0)) : T(1));
+}
+
+template <typename T>
+__global__ void k_from_f64_t(const double* __restrict__ a, T* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = T(a[i]);
+}
+template <typename T>
+__global__ void k_to_f64_t(const T* __restrict__ a, double* __restrict__ b, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double(a[i]);
+}
+
+// xorshift32 random phase of doe.js:65-86: seed>>>0 || 1; phi = (u*2-1)*pi.
+static void random_phase(std::vector<double>& p, unsigned seed) {
+  uint32_t s = seed ? seed : 1u;
+  for (double& v : p) {
+    s ^= s << 13;
+    s ^= s >> 17;
+    s ^= s << 5;
+    v = (double(s) / 4294967296.0 * 2 - 1) * kPi;
+  }
+}
+
+// Device workspace of one design.
+template <typename T>
+struct GsWork {
+  std::vector<void*> allocs;
+  GsBuf<T> b{};
+  T* fre = nullptr;
+  T* fim = nullptr;
+  T* phi0 = nullptr;
+  T* roi_d = nullptr;
+  double* hist = nullptr;
+  ~GsWork() {
+    for (void* p : allocs) hipFree(p);
+  }
+  int alloc(void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("doe_gs_run: device allocation failed");
+      return DOE_ERR_OOM;
+    }
+    allocs.push_back(*p);
+    return DOE_OK;
+  }
+};
+
+template <typename T>
+static int upload_t(doe_ctx* ctx, const double* host, T* dst, long n) {
+  int rc = ensure_stage(ctx, size_t(n) * sizeof(double));
+  if (rc) return rc;
+  DOE_HIP(hipMemcpyAsync(ctx->stage, host, size_t(n) * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream));
+  hipLaunchKernelGGL(k_from_f64_t<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     (const double*)ctx->stage, dst, n);
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+template <typename T>
+static int download_t(doe_ctx* ctx, const T* src, double* host, long n) {
+  int rc = ensure_stage(ctx, size_t(n) * sizeof(double));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_to_f64_t<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     src, (double*)ctx->stage, n);
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(hipMemcpyAsync(host, ctx->stage, size_t(n) * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+template <typename T>
+int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const double* tgt,
+                  const double* roi, const double* init_phase, unsigned seed, double* phase_out,
+                  double* error_out, double* efficiency_out, double* amplitude_out,
+                  double* intensity_out, double* weights_out, double* best_rmse_out) {
+  const int nx = plan->nx, ny = plan->ny, K = plan->iterations;
+  const long n = (long)nx * ny;
+  const bool weighted = plan->algorithm == DOE_ALG_WGS;
+  const bool far = plan->mode == DOE_MODE_FAR_FIELD;
+  GsWork<T> W;
+  const size_t tb = size_t(n) * sizeof(T);
+  void* p = nullptr;
+  int rc;
+#define ALLOC(dst, bytes)                         \
+  do {                                            \
+    rc = W.alloc(&p, (bytes));                    \
+    if (rc) return rc;                            \
+    dst = (decltype(dst))p;                       \
+  } while (0)
+  T *illum_d, *tgt_d, *w_d = nullptr;
+  ALLOC(W.fre, tb);
+  ALLOC(W.fim, tb);
+  ALLOC(illum_d, tb);
+  ALLOC(tgt_d, tb);
+  ALLOC(W.phi0, tb);
+  if (roi) ALLOC(W.roi_d, tb);
+  if (weighted) ALLOC(w_d, tb);
+  T* ph[2];
+  T* am[2];
+  ALLOC(ph[0], tb);
+  ALLOC(ph[1], tb);
+  ALLOC(am[0], tb);
+  ALLOC(am[1], tb);
+  GsState* st_d;
+  ALLOC(st_d, sizeof(GsState));
+  ALLOC(W.hist, sizeof(double) * 2 * size_t(K > 0 ? K : 1));
+#undef ALLOC
+  rc = upload_t<T>(ctx, illum, illum_d, n);
+  if (!rc) rc = upload_t<T>(ctx, tgt, tgt_d, n);
+  if (!rc && roi) rc = upload_t<T>(ctx, roi, W.roi_d, n);
+  std::vector<double> phi;
+  if (!init_phase) {
+    phi.assign(n, 0.0);
+    if (far) random_phase(phi, seed);  // fourierCgh default (doe.js:651)
+  }
+  if (!rc) rc = upload_t<T>(ctx, init_phase ? init_phase : phi.data(), W.phi0, n);
+  if (rc) return rc;
+  DOE_HIP(hipMemsetAsync(ph[0], 0, tb, ctx->stream));
+  DOE_HIP(hipMemsetAsync(ph[1], 0, tb, ctx->stream));
+  DOE_HIP(hipMemsetAsync(W.hist, 0, sizeof(double) * 2 * size_t(K > 0 ? K : 1), ctx->stream));
+  hipLaunchKernelGGL(k_gs_init_state<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                     st_d, w_d, (const T*)W.roi_d, n, weighted ? 1 : 0);
+  DOE_HIP(hipGetLastError());
+
+  GsBuf<T>& b = W.b;
+  b.illum = illum_d;
+  b.tgt = tgt_d;
+  b.roi = W.roi_d;
+  b.w = w_d;
+  b.phase[0] = ph[0];
+  b.phase[1] = ph[1];
+  b.amp[0] = am[0];
+  b.amp[1] = am[1];
+  b.state = st_d;
+  b.err_hist = W.hist;
+  b.eff_hist = W.hist + (K > 0 ? K : 1);
+  b.nx = nx;
+  b.ny = ny;
+  b.dxdy_x = plan->dx;
+  b.dxdy_y = plan->dy;
+  b.norm = 1 / std::sqrt(double(nx) * double(ny));
+  b.gsign = (((nx >> 1) + (ny >> 1)) & 1) ? -1.0 : 1.0;
+  b.beta = plan->beta;
+  b.feedback = plan->feedback;
+  b.escape = plan->escape;
+  b.weighted = weighted ? 1 : 0;
+
+  Twiddle twx, twy;
+  rc = get_twiddle(ctx, nx, sizeof(T), &twx);
+  if (rc) return rc;
+  rc = get_twiddle(ctx, ny, sizeof(T), &twy);
+  if (rc) return rc;
+  // reduction scratch: partials per column tile, one counter per field
+  int tiles = 0;
+  with_n_gs(ny, [&](auto NN) { tiles = col_tiles<T, decltype(NN)::value>(nx); });
+  rc = ensure_reduce(ctx, size_t(tiles) * R_N, 1);
+  if (rc) return rc;
+  T* re = W.fre;
+  T* im = W.fim;
+  hipError_t e = hipSuccess;
+
+  if (far) {
+    OpFFInit<T> op0{b, W.phi0};
+    with_n_gs(nx, [&](auto NN) {
+      e = launch_row<T, T, decltype(NN)::value, 0, false, false>(ctx, "ff_init_row", op0, re, im, re, im, ny, 1, twx);
+    });
+    DOE_HIP(e);
+    for (int it = 0; it < K; ++it) {
+      OpFFB<T> opb{b, ctx->partials, ctx->counters, it, it == K - 1 ? 1 : 0};
+      with_n_gs(ny, [&](auto NN) {
+        e = launch_col<T, T, decltype(NN)::value, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, nx, 1, twy);
+      });
+      DOE_HIP(e);
+      OpFFA<T> opa{b, it == K - 1 ? 1 : 0};
+      with_n_gs(nx, [&](auto NN) {
+        e = launch_row<T, T, decltype(NN)::value, 1, true, false>(ctx, "ff_row_phase", opa, re, im, re, im, ny, 1, twx);
+      });
+      DOE_HIP(e);
+    }
+  } else {
+    OpAsmH h{};
+    std::vector<AsmField> hp(1);
+    hp[0].two_pi_z = 2 * kPi * plan->z;
+    hp[0].lim_x = hp[0].lim_y = hp[0].inner_x = hp[0].inner_y = 0;
+    hp[0].band = 0;  // the design loop never band-limits (doe.js:352, 400, 419)
+    hp[0].zero = plan->z == 0 ? 1 : 0;
+    void* dp = nullptr;
+    rc = upload_params(ctx, hp.data(), sizeof(AsmField), &dp);
+    if (rc) return rc;
+    h.fp = (const AsmField*)dp;
+    h.dfx = 1 / (nx * plan->dx);
+    h.dfy = 1 / (ny * plan->dy);
+    h.inv_l2 = 1 / (plan->lambda * plan->lambda);
+    h.nx = nx;
+    h.ny = ny;
+    h.evanescent = 1;
+    OpAsmH hc = h;
+    hc.conj = 1;
+    h.conj = 0;
+    auto row_h = [&](const OpAsmH& op, const char* tag) {
+      with_n_gs(nx, [&](auto NN) {
+        e = launch_row<T, T, decltype(NN)::value, 1, false, true>(ctx, tag, op, re, im, re, im, ny, 1, twx);
+      });
+    };
+    OpIPInit<T> op0{b, W.phi0};
+    with_n_gs(ny, [&](auto NN) {
+      e = launch_col<T, T, decltype(NN)::value, 0, false, false>(ctx, "ip_init_col", op0, re, im, re, im, nx, 1, twy);
+    });
+    DOE_HIP(e);
+    row_h(h, "ip_row_h");
+    DOE_HIP(e);
+    for (int it = 0; it < K; ++it) {
+      if (!weighted || it == 0) {
+        OpIPMP<T> op{b, ctx->partials, ctx->counters, it};
+        with_n_gs(ny, [&](auto NN) {
+          e = launch_col<T, T, decltype(NN)::value, 1, true, false>(ctx, "ip_col_project", op, re, im, re, im, nx, 1, twy);
+        });
+        DOE_HIP(e);
+      } else {
+        OpIPMeas<T> om{b, ctx->partials, ctx->counters, it - 1};
+        with_n_gs(ny, [&](auto NN) {
+          e = launch_col<T, T, decltype(NN)::value, 0, true, false>(ctx, "ip_col_measure", om, re, im, re, im, nx, 1, twy);
+        });
+        DOE_HIP(e);
+        OpIPProj<T> op{b, ctx->partials, ctx->counters, it};
+        with_n_gs(ny, [&](auto NN) {
+          e = launch_col<T, T, decltype(NN)::value, 0, false, false>(ctx, "ip_col_reproject", op, re, im, re, im, nx, 1, twy);
+        });
+        DOE_HIP(e);
+      }
+      row_h(hc, "ip_row_hconj");
+      DOE_HIP(e);
+      OpIPPhase<T> opp{b};
+      with_n_gs(ny, [&](auto NN) {
+        e = launch_col<T, T, decltype(NN)::value, 1, true, false>(ctx, "ip_col_phase", opp, re, im, re, im, nx, 1, twy);
+      });
+      DOE_HIP(e);
+      row_h(h, "ip_row_h");
+      DOE_HIP(e);
+    }
+    // closing measurement: GS final forward amplitude (doe.js:418-426); WGS
+    // best check + weight update of the last iteration.
+    OpIPMeas<T> om{b, ctx->partials, ctx->counters, K - 1};
+    with_n_gs(ny, [&](auto NN) {
+      e = launch_col<T, T, decltype(NN)::value, 0, true, false>(ctx, "ip_col_measure", om, re, im, re, im, nx, 1, twy);
+    });
+    DOE_HIP(e);
+  }
+
+  // ---- outputs ----------------------------------------------------------
+  GsState st{};
+  DOE_HIP(hipMemcpyAsync(&st, st_d, sizeof(st), hipMemcpyDeviceToHost, ctx->stream));
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  const int pslot = (weighted && !far) ? st.best_phase_slot : 0;
+  const int aslot = weighted ? st.best_amp_slot : 0;
+  rc = download_t<T>(ctx, ph[pslot], phase_out, n);
+  if (rc) return rc;
+  if (amplitude_out || intensity_out) {
+    std::vector<double> a(n);
+    rc = download_t<T>(ctx, am[aslot], a.data(), n);
+    if (rc) return rc;
+    for (long i = 0; i < n; ++i) {
+      if (amplitude_out) amplitude_out[i] = a[i];
+      if (intensity_out) intensity_out[i] = a[i] * a[i];
+    }
+  }
+  if (K > 0) {
+    std::vector<double> hist(2 * size_t(K));
+    DOE_HIP(hipMemcpy(hist.data(), W.hist, sizeof(double) * 2 * size_t(K), hipMemcpyDeviceToHost));
+    if (error_out) std::copy(hist.begin(), hist.begin() + K, error_out);
+    if (efficiency_out) std::copy(hist.begin() + K, hist.end(), efficiency_out);
+  }
+  if (weights_out) {
+    if (weighted) {
+      rc = download_t<T>(ctx, w_d, weights_out, n);
+      if (rc) return rc;
+      for (long i = 0; i < n; ++i) weights_out[i] *= st.nrm;
+    } else {
+      std::fill(weights_out, weights_out + n, 0.0);
+    }
+  }
+  if (best_rmse_out) *best_rmse_out = weighted ? st.best_rmse : std::numeric_limits<double>::quiet_NaN();
+  return DOE_OK;
+}
+
+#define DOE_GS_INST(T)                                                                          \
+  template int run_gs<T>(doe_ctx*, const doe_gs_plan*, const double*, const double*, const double*, \
+                         const double*, unsigned, double*, double*, double*, double*, double*,    \
+                         double*, double*);
+#if DOE_PART == 0
+extern template int run_gs<float>(doe_ctx*, const doe_gs_plan*, const double*, const double*,
+                                  const double*, const double*, unsigned, double*, double*, double*,
+                                  double*, double*, double*, double*);
+extern template int run_gs<double>(doe_ctx*, const doe_gs_plan*, const double*, const double*,
+                                   const double*, const double*, unsigned, double*, double*,
+                                   double*, double*, double*, double*, double*);
+#elif DOE_PART == 1
+DOE_GS_INST(float)
+#elif DOE_PART == 2
+DOE_GS_INST(double)
+#endif
+
+}  // namespace doe
+
+#if DOE_PART == 0
+using namespace doe;
+
+extern "C" int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* illumination,
+                          const double* target_amp, const double* roi, const double* init_phase,
+                          unsigned seed, double* phase_out, double* error_out,
+                          double* efficiency_out, double* amplitude_out, double* intensity_out,
+                          double* weights_out, double* best_rmse_out) {
+  DOE_REQUIRE(ctx && plan && illumination && target_amp && phase_out, DOE_ERR_INVALID,
+              "doe_gs_run: null argument");
+  doe_field probe;
+  probe.nx = plan->nx;
+  probe.ny = plan->ny;
+  probe.batch = 1;
+  int rc = check_field(ctx, &probe);
+  if (rc) return rc;
+  DOE_REQUIRE(plan->iterations >= 0, DOE_ERR_INVALID, "doe_gs_run: negative iteration count");
+  DOE_REQUIRE(plan->mode == DOE_MODE_IMAGE_PLANE || plan->mode == DOE_MODE_FAR_FIELD,
+              DOE_ERR_INVALID, "doe_gs_run: unknown mode");
+  DOE_REQUIRE(plan->algorithm == DOE_ALG_GS || plan->algorithm == DOE_ALG_WGS, DOE_ERR_INVALID,
+              "doe_gs_run: unknown algorithm");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const int saved = ctx->flags;
+  ctx->flags |= DOE_CTX_ASYNC;  // the loop only synchronises at the end
+  if (plan->dtype == DOE_F64)
+    rc = run_gs<double>(ctx, plan, illumination, target_amp, roi, init_phase, seed, phase_out,
+                        error_out, efficiency_out, amplitude_out, intensity_out, weights_out,
+                        best_rmse_out);
+  else if (plan->dtype == DOE_F32)
+    rc = run_gs<float>(ctx, plan, illumination, target_amp, roi, init_phase, seed, phase_out,
+                       error_out, efficiency_out, amplitude_out, intensity_out, weights_out,
+                       best_rmse_out);
+  else {
+    set_error("doe_gs_run: dtype must be DOE_F32 or DOE_F64");
+    rc = DOE_ERR_UNSUPPORTED;
+  }
+  ctx->flags = saved;
+  hipStreamSynchronize(ctx->stream);
+  return rc;
+}
+#endif  // DOE_PART == 0

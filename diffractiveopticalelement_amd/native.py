@@ -1,0 +1,305 @@
+"""ctypes binding of ``libdoehip.so`` (the C ABI declared in ``include/doehip.h``).
+
+This is plumbing for the Python tests and the benchmark; the product's host
+interface for users of the toolkit is the JavaScript drop-in in ``js/`` over
+the N-API addon.  Loading fails loudly when the library has not been built:
+there is no CPU fallback anywhere on this path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libdoehip.so"
+
+DOE_OK = 0
+DOE_F32, DOE_F64, DOE_F16, DOE_BF16 = 0, 1, 2, 3
+DTYPES = {"f32": DOE_F32, "f64": DOE_F64, "f16": DOE_F16, "bf16": DOE_BF16}
+MODE_IMAGE_PLANE, MODE_FAR_FIELD = 0, 1
+ALG_GS, ALG_WGS = 0, 1
+
+c_double_p = C.POINTER(C.c_double)
+
+
+class DoeError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(msg)
+        self.status = status
+
+
+class GsPlan(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int),
+        ("algorithm", C.c_int),
+        ("iterations", C.c_int),
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("lambda_", C.c_double),
+        ("z", C.c_double),
+        ("beta", C.c_double),
+        ("feedback", C.c_double),
+        ("escape", C.c_int),
+        ("dtype", C.c_int),
+    ]
+
+
+# Every symbol the header declares: (name, restype, argtypes)
+_VP = C.c_void_p
+SIGNATURES = [
+    ("doe_version", C.c_int, []),
+    ("doe_last_error", C.c_char_p, []),
+    ("doe_status_string", C.c_char_p, [C.c_int]),
+    ("doe_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("doe_ctx_create", C.c_int, [C.c_int, _VP, C.POINTER(_VP)]),
+    ("doe_ctx_destroy", C.c_int, [_VP]),
+    ("doe_ctx_sync", C.c_int, [_VP]),
+    ("doe_ctx_stream", C.c_int, [_VP, C.POINTER(_VP)]),
+    ("doe_ctx_set_flags", C.c_int, [_VP, C.c_int]),
+    ("doe_ctx_profile_report", C.c_int, [_VP, C.c_char_p, C.c_long]),
+    ("doe_timer_start", C.c_int, [_VP]),
+    ("doe_timer_stop", C.c_int, [_VP, c_double_p]),
+    ("doe_field_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)]),
+    ("doe_field_wrap", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)]),
+    ("doe_field_free", C.c_int, [_VP]),
+    ("doe_field_info", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(_VP), C.POINTER(_VP)]),
+    ("doe_field_upload", C.c_int, [_VP, _VP, c_double_p, c_double_p]),
+    ("doe_field_download", C.c_int, [_VP, _VP, c_double_p, c_double_p]),
+    ("doe_field_copy", C.c_int, [_VP, _VP, _VP]),
+    ("doe_fft2d", C.c_int, [_VP, _VP, C.c_int]),
+    ("doe_fft2d_host", C.c_int, [_VP, c_double_p, c_double_p, C.c_long, C.c_int, C.c_int, C.c_int,
+                                 C.c_int]),
+    ("doe_asm", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p, C.c_int,
+                          C.c_int, C.c_double]),
+    ("doe_asm_transfer", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, C.c_int, C.c_double, c_double_p]),
+    ("doe_fresnel", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p,
+                              C.c_int, C.c_double, C.c_double, c_double_p, c_double_p]),
+    ("doe_lens", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p,
+                           c_double_p, c_double_p]),
+    ("doe_gs_run", C.c_int, [_VP, C.POINTER(GsPlan), c_double_p, c_double_p, c_double_p,
+                             c_double_p, C.c_uint, c_double_p, c_double_p, c_double_p,
+                             c_double_p, c_double_p, c_double_p, c_double_p]),
+]
+
+_lib: C.CDLL | None = None
+
+
+def load(path: Path | str | None = None) -> C.CDLL:
+    """Load libdoehip.so and declare every entry point.  Raises if absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback for the HIP path)"
+        )
+    lib = C.CDLL(str(p))
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != DOE_OK:
+        lib = load()
+        raise DoeError(status, f"{lib.doe_status_string(status).decode()}: "
+                               f"{lib.doe_last_error().decode()}")
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(c_double_p)
+
+
+def _darr(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class Context:
+    """One HIP stream on one device (stream=None: the library's own)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.doe_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h)))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.doe_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self) -> None:
+        check(self.lib.doe_ctx_sync(self.h))
+
+    def set_flags(self, asynchronous: bool = False, profile: bool = False) -> None:
+        check(self.lib.doe_ctx_set_flags(self.h, (1 if asynchronous else 0) | (2 if profile else 0)))
+
+    def profile_report(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        check(self.lib.doe_ctx_profile_report(self.h, buf, len(buf)))
+        out = {}
+        for ln in buf.value.decode().splitlines():
+            tag, cnt, ms = ln.split()
+            out[tag] = (int(cnt), float(ms))
+        return out
+
+    def timer_start(self) -> None:
+        check(self.lib.doe_timer_start(self.h))
+
+    def timer_stop(self) -> float:
+        ms = C.c_double()
+        check(self.lib.doe_timer_stop(self.h, C.byref(ms)))
+        return ms.value
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(self.lib.doe_ctx_stream(self.h, C.byref(s)))
+        return s.value or 0
+
+    # ---- fields -------------------------------------------------------
+    def field(self, nx: int, ny: int, batch: int = 1, dtype: str = "f32") -> "Field":
+        h = C.c_void_p()
+        check(self.lib.doe_field_create(self.h, nx, ny, batch, DTYPES[dtype], C.byref(h)))
+        return Field(self, h, nx, ny, batch, dtype)
+
+    def wrap(self, re, im, nx: int, ny: int, batch: int = 1, dtype: str = "f32") -> "Field":
+        """Non-owning view of device memory (e.g. torch tensors' data_ptr())."""
+        h = C.c_void_p()
+        check(self.lib.doe_field_wrap(self.h, C.c_void_p(re), C.c_void_p(im), nx, ny, batch,
+                                      DTYPES[dtype], C.byref(h)))
+        return Field(self, h, nx, ny, batch, dtype)
+
+    def upload(self, re, im=None, dtype: str = "f32") -> "Field":
+        re = _darr(re)
+        im = np.zeros_like(re) if im is None else _darr(im)
+        if re.ndim == 2:
+            re, im = re[None], im[None]
+        b, ny, nx = re.shape
+        f = self.field(nx, ny, b, dtype)
+        check(self.lib.doe_field_upload(self.h, f.h, _dptr(re), _dptr(np.ascontiguousarray(im))))
+        return f
+
+    # ---- operations ---------------------------------------------------
+    def fft2d(self, f: "Field", inverse: bool = False) -> None:
+        check(self.lib.doe_fft2d(self.h, f.h, int(inverse)))
+
+    def fft2d_host(self, re: np.ndarray, im: np.ndarray, nx: int, ny: int, inverse=False,
+                   dtype: str = "f64") -> None:
+        check(self.lib.doe_fft2d_host(self.h, _dptr(re), _dptr(im), re.size, nx, ny,
+                                      int(inverse), DTYPES[dtype]))
+
+    def asm(self, fin: "Field", fout: "Field", dx, dy, lam, z, band_limit=True, evanescent=True,
+            soft_edge=0.02) -> None:
+        zz = _darr(np.broadcast_to(np.asarray(z, dtype=np.float64), (fin.batch,)))
+        check(self.lib.doe_asm(self.h, fin.h, fout.h, dx, dy, lam, _dptr(zz), int(band_limit),
+                               int(evanescent), soft_edge))
+
+    def asm_transfer(self, nx, ny, dx, dy, lam, z, band_limit=True, evanescent=True,
+                     soft_edge=0.02, dtype="f64"):
+        f = self.field(nx, ny, 1, dtype)
+        frac = C.c_double()
+        check(self.lib.doe_asm_transfer(self.h, f.h, dx, dy, lam, z, int(band_limit),
+                                        int(evanescent), soft_edge, C.byref(frac)))
+        re, im = f.download()
+        f.free()
+        return re[0], im[0], frac.value
+
+    def fresnel(self, fin: "Field", fout: "Field", dx, dy, lam, z, inverse=False, in_dx=0.0,
+                in_dy=0.0):
+        zz = _darr(np.broadcast_to(np.asarray(z, dtype=np.float64), (fin.batch,)))
+        odx = np.zeros(fin.batch)
+        ody = np.zeros(fin.batch)
+        check(self.lib.doe_fresnel(self.h, fin.h, fout.h, dx, dy, lam, _dptr(zz), int(inverse),
+                                   in_dx, in_dy, _dptr(odx), _dptr(ody)))
+        return odx, ody
+
+    def lens(self, fin: "Field", fout: "Field", dx, dy, lam, f):
+        ff = _darr(np.broadcast_to(np.asarray(f, dtype=np.float64), (fin.batch,)))
+        odx = np.zeros(fin.batch)
+        ody = np.zeros(fin.batch)
+        check(self.lib.doe_lens(self.h, fin.h, fout.h, dx, dy, lam, _dptr(ff), _dptr(odx),
+                                _dptr(ody)))
+        return odx, ody
+
+    def gs_run(self, plan: GsPlan, illumination, target_amp, roi=None, init_phase=None,
+               seed: int = 12345):
+        n = plan.nx * plan.ny
+        ill = _darr(illumination).ravel()
+        tgt = _darr(target_amp).ravel()
+        roi_a = None if roi is None else _darr(roi).ravel()
+        ip = None if init_phase is None else _darr(init_phase).ravel()
+        out = {
+            "phase": np.zeros(n), "error": np.zeros(plan.iterations),
+            "efficiency": np.zeros(plan.iterations), "amplitude": np.zeros(n),
+            "intensity": np.zeros(n), "weights": np.zeros(n), "rmse": np.zeros(1),
+        }
+        check(self.lib.doe_gs_run(
+            self.h, C.byref(plan), _dptr(ill), _dptr(tgt),
+            _dptr(roi_a) if roi_a is not None else None,
+            _dptr(ip) if ip is not None else None, C.c_uint(seed & 0xFFFFFFFF),
+            _dptr(out["phase"]), _dptr(out["error"]), _dptr(out["efficiency"]),
+            _dptr(out["amplitude"]), _dptr(out["intensity"]), _dptr(out["weights"]),
+            _dptr(out["rmse"])))
+        out["rmse"] = float(out["rmse"][0])
+        return out
+
+
+class Field:
+    def __init__(self, ctx: Context, h, nx, ny, batch, dtype):
+        self.ctx, self.h, self.nx, self.ny, self.batch, self.dtype = ctx, h, nx, ny, batch, dtype
+
+    def download(self):
+        n = self.nx * self.ny * self.batch
+        re = np.zeros(n)
+        im = np.zeros(n)
+        check(self.ctx.lib.doe_field_download(self.ctx.h, self.h, _dptr(re), _dptr(im)))
+        shape = (self.batch, self.ny, self.nx)
+        return re.reshape(shape), im.reshape(shape)
+
+    def upload(self, re, im) -> None:
+        re = _darr(re).reshape(-1)
+        im = _darr(im).reshape(-1)
+        check(self.ctx.lib.doe_field_upload(self.ctx.h, self.h, _dptr(re), _dptr(im)))
+
+    def ptrs(self):
+        re = C.c_void_p()
+        im = C.c_void_p()
+        check(self.ctx.lib.doe_field_info(self.h, None, None, None, None, C.byref(re), C.byref(im)))
+        return re.value, im.value
+
+    def free(self) -> None:
+        if self.h:
+            self.ctx.lib.doe_field_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def header_symbols() -> list[str]:
+    """Names of every function declared in include/doehip.h."""
+    import re as _re
+    text = (PKG.parent / "include" / "doehip.h").read_text()
+    return sorted(set(_re.findall(r"\b(doe_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,153 @@
+/*
+ * doehip.h — C ABI of the MI355X (gfx950) FFT / free-space propagation /
+ * phase-retrieval library (libdoehip.so).
+ *
+ * This is the native boundary behind the diffractive-optical-element toolkit's
+ * JavaScript entry points.  Every function returns an int status (DOE_OK on
+ * success) and never throws; doe_last_error() gives the thread's last message.
+ * Device memory is owned by the library (or by the caller for wrapped fields);
+ * host pointers are borrowed only for the duration of a call.  Calls are
+ * synchronous with respect to the host unless stated otherwise: when they
+ * return, output buffers (host or device) hold valid results.  A context owns
+ * one HIP stream and is not re-entrant.
+ *
+ * Data layout: a field is a batch of `batch` complex nx-by-ny grids stored as
+ * two planar arrays (real, imaginary), row-major with idx = x + y*nx, fields
+ * back to back — the reference's ComplexField layout (src/field.js:22-40).
+ * Grid sides are powers of two in [2, 8192].
+ *
+ * Citations "ref:" point into the reference toolkit's sources.
+ */
+#ifndef DOEHIP_H
+#define DOEHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DOE_OK 0
+#define DOE_ERR_INVALID 1     /* bad argument: non power-of-two, null, ... */
+#define DOE_ERR_HIP 2         /* HIP runtime error */
+#define DOE_ERR_OOM 3         /* device allocation failed */
+#define DOE_ERR_UNSUPPORTED 4 /* size/dtype combination not built */
+#define DOE_ERR_SHAPE 5       /* plane length does not match nx*ny */
+#define DOE_ERR_STATE 6       /* call order (e.g. handle from another ctx) */
+
+/* Storage precision of a field.  F32/F64 compute in their own precision;
+ * F16/BF16 store half-width values and compute in FP32. */
+#define DOE_F32 0
+#define DOE_F64 1
+#define DOE_F16 2
+#define DOE_BF16 3
+
+typedef struct doe_ctx doe_ctx;
+typedef struct doe_field doe_field;
+
+/* ---- library / context ------------------------------------------------- */
+int doe_version(void);
+const char* doe_last_error(void);
+const char* doe_status_string(int status);
+int doe_device_count(int* count);
+/* stream: an existing hipStream_t to launch on, or NULL for a private one. */
+int doe_ctx_create(int device, void* stream, doe_ctx** out);
+int doe_ctx_destroy(doe_ctx* ctx);
+int doe_ctx_sync(doe_ctx* ctx);
+int doe_ctx_stream(doe_ctx* ctx, void** stream);
+
+/* Measurement plumbing (no counterpart in the reference, which times with
+ * wall clocks only, tools/bench.mjs:39-47).  DOE_CTX_ASYNC: entry points on
+ * device-resident fields return without synchronising the stream.
+ * DOE_CTX_PROFILE: every kernel launch is bracketed by HIP events on the
+ * context stream; doe_ctx_profile_report synchronises and writes one line
+ * per kernel tag, "tag count total_ms", then clears the records.
+ * doe_timer_start/stop bracket a region with two events on the stream. */
+#define DOE_CTX_ASYNC 1
+#define DOE_CTX_PROFILE 2
+int doe_ctx_set_flags(doe_ctx* ctx, int flags);
+int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap);
+int doe_timer_start(doe_ctx* ctx);
+int doe_timer_stop(doe_ctx* ctx, double* ms);
+
+/* ---- device-resident planar fields (ref: src/field.js:22-196) --------- */
+int doe_field_create(doe_ctx* ctx, int nx, int ny, int batch, int dtype, doe_field** out);
+/* Non-owning view of caller device memory (re, im: batch*nx*ny elements). */
+int doe_field_wrap(doe_ctx* ctx, void* re, void* im, int nx, int ny, int batch, int dtype,
+                   doe_field** out);
+int doe_field_free(doe_field* f);
+int doe_field_info(const doe_field* f, int* nx, int* ny, int* batch, int* dtype, void** re,
+                   void** im);
+/* Host Float64 planes (batch*nx*ny each) <-> device storage (converted). */
+int doe_field_upload(doe_ctx* ctx, doe_field* f, const double* re, const double* im);
+int doe_field_download(doe_ctx* ctx, doe_field* f, double* re, double* im);
+int doe_field_copy(doe_ctx* ctx, const doe_field* src, doe_field* dst);
+
+/* ---- FFT (ref: src/fft.js:148-154 fft2d) ------------------------------- *
+ * In place on every grid of the batch.  Forward is the unscaled DFT with
+ * exp(-2*pi*i*n*k/N); inverse uses exp(+...) and scales by 1/(nx*ny). */
+int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
+/* Host drop-in of fft2d(re, im, nx, ny, inverse) (ref: src/fft.js:148):
+ * uploads the Float64 planes, transforms in `dtype`, downloads in place.
+ * Returns DOE_ERR_SHAPE if len != nx*ny (the reference's length error). */
+int doe_fft2d_host(doe_ctx* ctx, double* re, double* im, long len, int nx, int ny, int inverse,
+                   int dtype);
+
+/* ---- angular spectrum (ref: src/propagate.js:57-133) ------------------- *
+ * out = ASM_z(in) for each grid b with distance z[b] (in may equal out).
+ * band_limit: Matsushima–Shimobaba local limit with raised-cosine edge of
+ * relative width soft_edge; evanescent: 1 zeroes evanescent waves, 0 keeps
+ * them as a pure phase (ref: propagate.js:88-95).  z[b] == 0 copies. */
+int doe_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double dy,
+            double lambda, const double* z, int band_limit, int evanescent, double soft_edge);
+/* Transfer function H on the centred grid (ref: propagate.js:57-102) into a
+ * batch-1 field; also returns the propagating fraction. */
+int doe_asm_transfer(doe_ctx* ctx, doe_field* out, double dx, double dy, double lambda, double z,
+                     int band_limit, int evanescent, double soft_edge,
+                     double* propagating_fraction);
+
+/* ---- single-FFT Fresnel (ref: src/propagate.js:155-316) ---------------- *
+ * inverse = 0: forward propagation to the magnified grid; dx, dy are the
+ * input pitch and out_dx/out_dy (nullable) receive lambda*|z|/(N*dx).
+ * inverse = 1: exact inverse; dx, dy are the *output* (element) pitch and
+ * in_dx/in_dy the magnified pitch of the input (ref: propagate.js:246-250). */
+int doe_fresnel(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double dy,
+                double lambda, const double* z, int inverse, double in_dx, double in_dy,
+                double* out_dx, double* out_dy);
+
+/* ---- lens focal-plane transform (ref: src/propagate.js:342-374) -------- */
+int doe_lens(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double dy,
+             double lambda, const double* f, double* out_dx, double* out_dy);
+
+/* ---- phase retrieval (ref: src/doe.js:327-546, 639-730) ---------------- */
+#define DOE_MODE_IMAGE_PLANE 0 /* angular-spectrum loop, gerchbergSaxton */
+#define DOE_MODE_FAR_FIELD 1   /* unitary DFT loop, fourierCgh */
+#define DOE_ALG_GS 0
+#define DOE_ALG_WGS 1 /* weightedGerchbergSaxton wrapping the mode's base loop */
+
+typedef struct doe_gs_plan {
+  int mode;       /* DOE_MODE_* */
+  int algorithm;  /* DOE_ALG_* */
+  int iterations;
+  int nx, ny;
+  double dx, dy, lambda, z;
+  double beta;     /* WGS exponent (ref default 0.9) */
+  double feedback; /* HIO feedback, image-plane GS only (ref: doe.js:380-388) */
+  int escape;      /* iterations before feedback starts */
+  int dtype;       /* DOE_F32 or DOE_F64 compute */
+} doe_gs_plan;
+
+/* Host-array phase retrieval.  All planes are nx*ny Float64.  roi may be NULL
+ * (whole plane); init_phase may be NULL (far-field: random from seed as the
+ * reference's randomPhase; image-plane: flat).  Outputs (all nullable except
+ * phase_out): phase (best phase under WGS), per-iteration error and
+ * efficiency histories (iterations entries), amplitude and intensity of the
+ * reported iterate, final weights, best rmse. */
+int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* illumination,
+               const double* target_amp, const double* roi, const double* init_phase,
+               unsigned seed, double* phase_out, double* error_out, double* efficiency_out,
+               double* amplitude_out, double* intensity_out, double* weights_out,
+               double* best_rmse_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DOEHIP_H */

@@ -1,0 +1,178 @@
+// ORACLE job runner (test infrastructure only).  Usage:
+//   node oracle/run.mjs <dir>
+// <dir>/job.json = {"op": ..., "args": {...}}; array inputs are little-endian
+// Float64 files <dir>/in_<name>.f64; array outputs are written as
+// <dir>/out_<name>.f64 and scalars/short lists to <dir>/result.json.
+import fs from 'fs';
+import path from 'path';
+import { fft2d, dft2dNaive } from './js/fft.js';
+import { ComplexField } from './js/field.js';
+import {
+  angularSpectrumTransferFunction,
+  propagateAngularSpectrumInPlace,
+  propagateFresnelFFT,
+  propagateFresnelFFTInverse,
+  focalPlaneSpectrum,
+} from './js/propagate.js';
+import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase } from './js/doe.js';
+import { ProjectorSystem } from './js/projector.js';
+import { ssim, rmsError, correlation, regionStats } from './js/metrics.js';
+
+const dir = process.argv[2];
+const job = JSON.parse(fs.readFileSync(path.join(dir, 'job.json'), 'utf8'));
+const a = job.args || {};
+
+function inp(name) {
+  const p = path.join(dir, 'in_' + name + '.f64');
+  if (!fs.existsSync(p)) return null;
+  const b = fs.readFileSync(p);
+  return new Float64Array(b.buffer.slice(b.byteOffset, b.byteOffset + b.byteLength));
+}
+function out(name, arr) {
+  fs.writeFileSync(path.join(dir, 'out_' + name + '.f64'), Buffer.from(Float64Array.from(arr).buffer));
+}
+function fieldOf(nx, ny, dx, dy) {
+  const f = new ComplexField(nx, ny, dx, dy);
+  f.re.set(inp('re'));
+  f.im.set(inp('im'));
+  return f;
+}
+function nowMs() {
+  const t = process.hrtime();
+  return t[0] * 1e3 + t[1] / 1e6;
+}
+
+const res = {};
+const op = job.op;
+if (op === 'fft2d' || op === 'dft2d') {
+  const re = inp('re');
+  const im = inp('im');
+  if (op === 'fft2d') {
+    fft2d(re, im, a.nx, a.ny, !!a.inverse);
+    out('re', re);
+    out('im', im);
+  } else {
+    const r = dft2dNaive(re, im, a.nx, a.ny, !!a.inverse);
+    out('re', r.re);
+    out('im', r.im);
+  }
+} else if (op === 'asm') {
+  const f = fieldOf(a.nx, a.ny, a.dx, a.dy);
+  propagateAngularSpectrumInPlace(f, a.lambda, a.z, { bandLimit: a.bandLimit, evanescent: a.evanescent, softEdge: a.softEdge });
+  out('re', f.re);
+  out('im', f.im);
+} else if (op === 'transfer') {
+  const H = angularSpectrumTransferFunction(a.nx, a.ny, a.dx, a.dy, a.lambda, a.z, { bandLimit: a.bandLimit, evanescent: a.evanescent, softEdge: a.softEdge });
+  out('re', H.re);
+  out('im', H.im);
+  res.propagatingFraction = H.propagatingFraction;
+} else if (op === 'fresnel') {
+  const f = fieldOf(a.nx, a.ny, a.dx, a.dy);
+  const r = propagateFresnelFFT(f, a.lambda, a.z, {});
+  out('re', r.field.re);
+  out('im', r.field.im);
+  res.dx = r.dx;
+  res.dy = r.dy;
+  res.undersampled = r.undersampled;
+  res.zCritical = r.zCritical;
+} else if (op === 'fresnel_inv') {
+  const f = fieldOf(a.nx, a.ny, a.in_dx, a.in_dy);
+  const r = propagateFresnelFFTInverse(f, a.lambda, a.z, { dx: a.dx, dy: a.dy });
+  out('re', r.re);
+  out('im', r.im);
+} else if (op === 'lens') {
+  const f = fieldOf(a.nx, a.ny, a.dx, a.dy);
+  const r = focalPlaneSpectrum(f, a.lambda, a.f);
+  out('re', r.re);
+  out('im', r.im);
+  res.dx = r.dx;
+  res.dy = r.dy;
+} else if (op === 'gs') {
+  // one phase-retrieval run with explicit inputs
+  const opts = {
+    nx: a.nx, ny: a.ny, dx: a.dx, dy: a.dy, lambda: a.lambda, z: a.z, iterations: a.iterations,
+    beta: a.beta, feedback: a.feedback || 0, escape: a.escape || 0, seed: a.seed,
+    roi: inp('roi'), initPhase: inp('phase0'),
+  };
+  const base = a.mode === 'far-field' ? fourierCgh : gerchbergSaxton;
+  const r = a.algorithm === 'wgs'
+    ? weightedGerchbergSaxton(inp('illum'), inp('target'), opts, base)
+    : base(inp('illum'), inp('target'), opts);
+  out('phase', r.phase);
+  out('amplitude', r.amplitude);
+  out('intensity', r.intensity);
+  if (r.weights) out('weights', r.weights);
+  res.error = r.error;
+  res.efficiency = r.efficiency;
+  res.rmse = r.rmse === undefined ? null : r.rmse;
+} else if (op === 'random_phase') {
+  out('phase', randomPhase(a.nx, a.ny, a.seed));
+} else if (op === 'projector') {
+  // ProjectorSystem(cfg).design(opts); simulate(); screenMetrics()
+  const sys = new ProjectorSystem(a.cfg);
+  const t0 = nowMs();
+  const r = sys.design(a.design || {});
+  const t1 = nowMs();
+  const field = sys.simulate(a.simulate || {});
+  const m = sys.screenMetrics(field);
+  const t2 = nowMs();
+  out('phase', r.phase);
+  out('illumination', r.illumination);
+  out('target_amp', r.targetAmp);
+  out('target', sys.target.irradiance);
+  out('roi', sys.target.roi);
+  out('screen_re', field.re);
+  out('screen_im', field.im);
+  res.error = r.error;
+  res.efficiency = r.efficiency;
+  res.algorithm = r.algorithm;
+  res.mode = r.mode;
+  res.z = sys.z;
+  res.screen_dx = field.dx;
+  res.metrics = m;
+  res.design_ms = t1 - t0;
+  res.simulate_metrics_ms = t2 - t1;
+} else if (op === 'metrics') {
+  const p = inp('plane');
+  const t = inp('target');
+  const m = inp('mask');
+  res.rmse = rmsError(p, t, m);
+  res.correlation = correlation(p, t, m);
+  res.ssim = ssim(p, t, a.nx, a.ny, { window: 8, weight: a.weight || null });
+  res.stats = regionStats(p, m);
+} else if (op === 'time') {
+  // CPU baseline timing of one unit of work on a single JS thread,
+  // 1 warm-up + reps (the reference's bench.mjs methodology).
+  const n = a.n;
+  const reps = a.reps || 3;
+  let fn;
+  if (a.what === 'fft2d') {
+    const re = new Float64Array(n * n).map(function (_, k) { return Math.sin(k * 0.37); });
+    const im = new Float64Array(n * n).map(function (_, k) { return Math.cos(k * 0.11); });
+    fn = function () { fft2d(re, im, n, n, false); };
+  } else if (a.what === 'asm') {
+    const f = new ComplexField(n, n, a.dx, a.dx);
+    for (let k = 0; k < f.size; k++) { f.re[k] = Math.sin(k * 0.37); f.im[k] = Math.cos(k * 0.11); }
+    fn = function () { propagateAngularSpectrumInPlace(f, a.lambda, a.z, { bandLimit: true }); };
+  } else if (a.what === 'fresnel') {
+    const f = new ComplexField(n, n, a.dx, a.dx);
+    for (let k = 0; k < f.size; k++) { f.re[k] = Math.sin(k * 0.37); f.im[k] = Math.cos(k * 0.11); }
+    fn = function () { propagateFresnelFFT(f, a.lambda, a.z, {}); };
+  } else if (a.what === 'wgs_ff') {
+    const ill = new Float64Array(n * n).fill(1);
+    const tgt = new Float64Array(n * n).fill(1);
+    fn = function () {
+      weightedGerchbergSaxton(ill, tgt, { nx: n, ny: n, iterations: a.iterations || 1, initPhase: randomPhase(n, n, 1) }, fourierCgh);
+    };
+  } else {
+    throw new Error('unknown timing target ' + a.what);
+  }
+  fn();
+  const t0 = nowMs();
+  for (let i = 0; i < reps; i++) fn();
+  res.ms = (nowMs() - t0) / reps;
+  res.node = process.version;
+} else {
+  throw new Error('unknown op ' + op);
+}
+fs.writeFileSync(path.join(dir, 'result.json'), JSON.stringify(res));

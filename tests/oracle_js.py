@@ -1,0 +1,38 @@
+"""Call the FP64 JavaScript oracle (oracle/run.mjs) from pytest.
+
+The oracle is the checker only: tests compare the HIP path against it.
+"""
+from __future__ import annotations
+
+import json
+import shutil
+import subprocess
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+RUNNER = ROOT / "oracle" / "run.mjs"
+
+
+def node_available() -> bool:
+    return shutil.which("node") is not None
+
+
+def run(op: str, args: dict | None = None, inputs: dict | None = None, timeout: float = 600):
+    """Run one oracle job; returns (arrays: dict[str, ndarray], result: dict)."""
+    with tempfile.TemporaryDirectory(prefix="doe_oracle_") as d:
+        dp = Path(d)
+        (dp / "job.json").write_text(json.dumps({"op": op, "args": args or {}}))
+        for name, arr in (inputs or {}).items():
+            if arr is None:
+                continue
+            np.ascontiguousarray(np.asarray(arr, dtype="<f8")).ravel().tofile(dp / f"in_{name}.f64")
+        proc = subprocess.run(["node", str(RUNNER), str(dp)], capture_output=True, text=True,
+                              timeout=timeout)
+        if proc.returncode != 0:
+            raise RuntimeError(f"oracle {op} failed:\n{proc.stderr[-4000:]}")
+        res = json.loads((dp / "result.json").read_text())
+        arrays = {p.name[4:-4]: np.fromfile(p, dtype="<f8") for p in dp.glob("out_*.f64")}
+    return arrays, res
