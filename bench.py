@@ -1,0 +1,177 @@
+"""Benchmark of the FFT / propagation hot path on one MI355X.
+
+Workload (the batched-propagation configuration): 16 complex FP32 fields of
+2048 x 2048, amplitude a Gaussian of waist 0.3*N*dx times a seeded uniform
+phase, pitch 4 um, lambda 633 nm, per-field z uniform in [10, 100] mm.
+
+A step is one fft2d (forward) over the whole batch, inputs resident in HBM.
+`value` is the step's algorithmic HBM throughput: 32 B per point (FP32
+planar complex read + write in the row pass and in the column pass) x points
+/ step time.  The same run times the rest of the configuration's pipeline
+(band-limited angular spectrum forward + inverse, single-FFT Fresnel) and
+reports it under "pipeline".  `roofline` covers the dominant kernel of the
+step, timed with HIP events on the library's stream; `cpu_baseline` is the
+FP64 JavaScript oracle (the reference's algorithm: radix-2 FFT) on one node
+thread, on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import shutil
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0
+B, N = 16, 2048
+DX, LAM = 4e-6, 633e-9
+
+
+def cpu_baseline(n: int = 2048, reps: int = 8) -> dict | None:
+    """Oracle fft2d (FP64, radix-2, one node thread) on `reps` 2048^2 transforms."""
+    if shutil.which("node") is None:
+        return None
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_js as oj  # the oracle is the CPU baseline's code, nothing else
+
+    _, res = oj.run("time", {"what": "fft2d", "n": n, "reps": reps}, timeout=900)
+    ms = res["ms"]
+    gbs = 32 * n * n / (ms * 1e-3) / 1e9
+    return {
+        "value": round(gbs, 4),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"{reps} forward fft2d of {n}x{n} complex FP64 by the JS oracle (radix-2 DIT, "
+                   f"strided columns) on one node {res.get('node')} thread, 1 warm-up; "
+                   f"{ms:.1f} ms per transform; GB/s counts the same 32 B/pt as the GPU value"),
+        "host": platform.processor() or platform.machine(),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the hot path shards by instance "
+              "with no communication); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from diffractiveopticalelement_amd import native as nat
+
+    ctx = nat.Context(0)
+    rng = np.random.default_rng(0)
+    c = (np.arange(N) - N / 2) * DX
+    w = 0.3 * N * DX
+    amp = np.exp(-(c[None, :] ** 2 + c[:, None] ** 2) / w ** 2)
+    zs = rng.uniform(0.010, 0.100, B)
+    field = ctx.field(N, N, B, "f32")
+    scratch = ctx.field(N, N, B, "f32")
+    for b in range(B):  # upload field by field (host memory stays small)
+        ph = rng.uniform(0, 2 * np.pi, (N, N))
+        view = ctx.wrap(field.ptrs()[0] + b * N * N * 4, field.ptrs()[1] + b * N * N * 4, N, N, 1, "f32")
+        view.upload(amp * np.cos(ph), amp * np.sin(ph))
+        view.free()
+    pts = B * N * N
+
+    ctx.set_flags(asynchronous=True, profile=False)
+    for _ in range(args.warmup):
+        ctx.fft2d(field)
+    ctx.sync()
+    ctx.set_flags(asynchronous=True, profile=True)
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        ctx.fft2d(field)
+    ms_events = ctx.timer_stop()
+    t1 = time.perf_counter()
+    rep = ctx.profile_report()
+    ms_step = ms_events / args.steps
+    value = 32 * pts / (ms_step * 1e-3) / 1e9
+
+    kernels = {}
+    for tag, (cnt, tot) in rep.items():
+        avg_us = tot / cnt * 1e3
+        kernels[tag] = {"launches": cnt, "avg_us": round(avg_us, 2),
+                        "gbs_alg": round(16 * pts / (avg_us * 1e-6) / 1e9, 1)}
+    dom_tag = max(kernels, key=lambda k: kernels[k]["avg_us"])
+    dom = kernels[dom_tag]
+
+    # rest of the configuration's pipeline: ASM fwd + inv (band-limited), Fresnel fwd
+    ctx.set_flags(asynchronous=True, profile=False)
+    ctx.asm(field, scratch, DX, DX, LAM, zs)
+    ctx.asm(scratch, scratch, DX, DX, LAM, -zs)
+    ctx.fresnel(field, scratch, DX, DX, LAM, zs)
+    ctx.sync()
+    reps = max(3, args.steps // 4)
+    ctx.set_flags(asynchronous=True, profile=True)
+    ctx.timer_start()
+    for _ in range(reps):
+        ctx.asm(field, scratch, DX, DX, LAM, zs)
+        ctx.asm(scratch, scratch, DX, DX, LAM, -zs)
+        ctx.fresnel(field, scratch, DX, DX, LAM, zs)
+    ms_pipe = ctx.timer_stop() / reps
+    prep = ctx.profile_report()
+    pipe_bytes = (48 + 48 + 32) * pts
+    pipeline = {
+        "ms": round(ms_pipe, 4),
+        "gbs_alg": round(pipe_bytes / (ms_pipe * 1e-3) / 1e9, 1),
+        "bytes_per_pt": {"asm": 48, "fresnel": 32},
+        "kernels": {t: {"launches": c, "avg_us": round(s / c * 1e3, 2)} for t, (c, s) in prep.items()},
+    }
+
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": "fft2d HBM GB/s (16 x 2048^2 complex FP32 batch; WGS time-to-RMSE<=1% at 1024^2 reported separately)",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "wall_ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Gaussian amplitude x seeded uniform phase, seed 0)",
+        "config": {"workload": "batched propagation: 16 x 2048^2 complex FP32, pitch 4 um, "
+                               "lambda 633 nm, z ~ U[10,100] mm; step = fft2d forward",
+                   "batch": B, "n": N},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_tag,
+            "achieved": dom["gbs_alg"],
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(dom["gbs_alg"] / PEAK_HBM_GBS, 4),
+            "traffic": None,
+            "alg_bytes_per_launch": 16 * pts,
+        },
+        "kernels": kernels,
+        "pipeline": pipeline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -235,6 +235,17 @@ int dispatch_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
   return DOE_ERR_INVALID;
 }
 
+int dispatch_fft_rows(doe_ctx* ctx, doe_field* f, int inverse) {
+  switch (f->dtype) {
+    case DOE_F32: return run_fft_rows<float, float>(ctx, f, inverse);
+    case DOE_F64: return run_fft_rows<double, double>(ctx, f, inverse);
+    case DOE_F16: return run_fft_rows<float, __half>(ctx, f, inverse);
+    case DOE_BF16: return run_fft_rows<float, __hip_bfloat16>(ctx, f, inverse);
+  }
+  set_error("unknown dtype");
+  return DOE_ERR_INVALID;
+}
+
 }  // namespace doe
 
 using namespace doe;
@@ -518,6 +529,38 @@ int doe_fft2d_host(doe_ctx* ctx, double* re, double* im, long len, int nx, int n
   if (!rc) rc = doe_field_download(ctx, f, re, im);
   doe_field_free(f);
   return rc;
+}
+
+int doe_fft1d_host(doe_ctx* ctx, double* re, double* im, long len, long n, long stride,
+                   long offset, int inverse, int dtype) {
+  DOE_REQUIRE(ctx && re && im, DOE_ERR_INVALID, "null argument");
+  if (n <= 1) return DOE_OK;  // ref: fft.js:87
+  if (!is_pow2(n)) {
+    set_error("fft1d: length must be a power of two, got " + std::to_string(n));
+    return DOE_ERR_INVALID;
+  }
+  DOE_REQUIRE(n <= kMaxN, DOE_ERR_UNSUPPORTED, "fft1d: length above 8192 is not built");
+  DOE_REQUIRE(stride != 0 && offset >= 0 && offset < len && offset + (n - 1) * stride >= 0 &&
+                  offset + (n - 1) * stride < len,
+              DOE_ERR_SHAPE, "fft1d: strided slice leaves the planes");
+  std::vector<double> br(2 * n, 0.0), bi(2 * n, 0.0);
+  for (long i = 0; i < n; ++i) {
+    br[i] = re[offset + i * stride];
+    bi[i] = im[offset + i * stride];
+  }
+  doe_field* f = nullptr;
+  int rc = doe_field_create(ctx, (int)n, 2, 1, dtype, &f);
+  if (rc) return rc;
+  rc = doe_field_upload(ctx, f, br.data(), bi.data());
+  if (!rc) rc = dispatch_fft_rows(ctx, f, inverse);
+  if (!rc) rc = doe_field_download(ctx, f, br.data(), bi.data());
+  doe_field_free(f);
+  if (rc) return rc;
+  for (long i = 0; i < n; ++i) {
+    re[offset + i * stride] = br[i];
+    im[offset + i * stride] = bi[i];
+  }
+  return DOE_OK;
 }
 
 }  // extern "C"

@@ -187,6 +187,8 @@ inline int col_tiles(int nx) {
 // (T, S) instantiation is compiled as a separate object, DOE_PART).
 template <typename T, typename S>
 int run_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
+template <typename T, typename S>
+int run_fft_rows(doe_ctx* ctx, doe_field* f, int inverse);
 struct OpAsmH;
 struct ChirpField;
 template <typename T, typename S>

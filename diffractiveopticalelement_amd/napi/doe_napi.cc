@@ -1,0 +1,418 @@
+// doe_napi.node — thin N-API (v8) addon over the C ABI of libdoehip.so.
+//
+// The JavaScript drop-in modules (../js/*.js) call these functions; every
+// non-OK status of the library becomes a thrown Error carrying the library's
+// message (which reproduces the reference's messages where the reference
+// throws, e.g. "fft2d: plane length ... does not match NxM").
+//
+// One process-wide library context on device 0 is created on first use.
+#include <node_api.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "doehip.h"
+
+namespace {
+
+doe_ctx* g_ctx = nullptr;
+
+bool throw_status(napi_env env, int rc) {
+  if (rc == DOE_OK) return false;
+  std::string msg = doe_last_error();
+  if (msg.empty()) msg = doe_status_string(rc);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return true;
+}
+
+doe_ctx* ctx(napi_env env) {
+  if (!g_ctx) {
+    int rc = doe_ctx_create(0, nullptr, &g_ctx);
+    if (rc != DOE_OK) {
+      throw_status(env, rc);
+      g_ctx = nullptr;
+    }
+  }
+  return g_ctx;
+}
+
+#define CHECK_NAPI(call)                                        \
+  do {                                                          \
+    if ((call) != napi_ok) {                                    \
+      napi_throw_error(env, nullptr, "N-API call failed: " #call); \
+      return nullptr;                                           \
+    }                                                           \
+  } while (0)
+
+struct Args {
+  napi_value v[16];
+  size_t n = 16;
+};
+
+bool get_args(napi_env env, napi_callback_info info, Args& a) {
+  return napi_get_cb_info(env, info, &a.n, a.v, nullptr, nullptr) == napi_ok;
+}
+
+double num(napi_env env, napi_value v, double dflt = 0.0) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_number) return dflt;
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+bool boolean(napi_env env, napi_value v, bool dflt) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b = dflt;
+    napi_get_value_bool(env, v, &b);
+    return b;
+  }
+  if (t == napi_number) return num(env, v) != 0;
+  return dflt;
+}
+
+// Float64Array -> pointer + length (nullptr when the value is not one).
+double* f64(napi_env env, napi_value v, size_t* len) {
+  bool is = false;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return nullptr;
+  napi_typedarray_type t;
+  size_t n = 0;
+  void* data = nullptr;
+  napi_value ab;
+  size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &t, &n, &data, &ab, &off) != napi_ok) return nullptr;
+  if (t != napi_float64_array) return nullptr;
+  if (len) *len = n;
+  return (double*)data;
+}
+
+napi_value make_f64(napi_env env, size_t n, double** data) {
+  napi_value ab, arr;
+  void* p = nullptr;
+  napi_create_arraybuffer(env, n * sizeof(double), &p, &ab);
+  napi_create_typedarray(env, napi_float64_array, n, ab, 0, &arr);
+  *data = (double*)p;
+  return arr;
+}
+
+napi_value make_num(napi_env env, double d) {
+  napi_value v;
+  napi_create_double(env, d, &v);
+  return v;
+}
+
+void set(napi_env env, napi_value obj, const char* k, napi_value v) {
+  napi_set_named_property(env, obj, k, v);
+}
+
+doe_field* field_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
+  return (doe_field*)p;
+}
+
+// ---------------------------------------------------------------------------
+napi_value Version(napi_env env, napi_callback_info) {
+  return make_num(env, doe_version());
+}
+
+napi_value DeviceCount(napi_env env, napi_callback_info) {
+  int c = 0;
+  doe_device_count(&c);
+  return make_num(env, c);
+}
+
+// fft2d(re, im, nx, ny, inverse, dtype) — in place on host Float64Arrays
+napi_value Fft2d(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 4) {
+    napi_throw_type_error(env, nullptr, "fft2d(re, im, nx, ny, inverse, dtype)");
+    return nullptr;
+  }
+  size_t lr = 0, li = 0;
+  double* re = f64(env, a.v[0], &lr);
+  double* im = f64(env, a.v[1], &li);
+  if (!re || !im) {
+    napi_throw_type_error(env, nullptr, "fft2d: re and im must be Float64Array planes");
+    return nullptr;
+  }
+  const int nx = (int)num(env, a.v[2]);
+  const int ny = (int)num(env, a.v[3]);
+  const int inv = a.n > 4 && boolean(env, a.v[4], false);
+  const int dt = a.n > 5 ? (int)num(env, a.v[5], DOE_F64) : DOE_F64;
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  long len = (long)lr;
+  if (li != lr) len = -1;  // mismatched planes: reported as a length error
+  throw_status(env, doe_fft2d_host(c, re, im, len < 0 ? (long)li : len, nx, ny, inv, dt));
+  return nullptr;
+}
+
+// fft1d(re, im, n, stride, offset, inverse, dtype) — one strided line
+napi_value Fft1d(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 3) {
+    napi_throw_type_error(env, nullptr, "fft1d(re, im, n, stride, offset, inverse, dtype)");
+    return nullptr;
+  }
+  size_t lr = 0, li = 0;
+  double* re = f64(env, a.v[0], &lr);
+  double* im = f64(env, a.v[1], &li);
+  if (!re || !im) {
+    napi_throw_type_error(env, nullptr, "fft1d: re and im must be Float64Array planes");
+    return nullptr;
+  }
+  const long n = (long)num(env, a.v[2]);
+  const long stride = a.n > 3 ? (long)num(env, a.v[3], 1) : 1;
+  const long offset = a.n > 4 ? (long)num(env, a.v[4], 0) : 0;
+  const int inv = a.n > 5 && boolean(env, a.v[5], false);
+  const int dt = a.n > 6 ? (int)num(env, a.v[6], DOE_F64) : DOE_F64;
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  throw_status(env, doe_fft1d_host(c, re, im, (long)std::min(lr, li), n, stride, offset, inv, dt));
+  return nullptr;
+}
+
+void finalize_field(napi_env, void* data, void*) { doe_field_free((doe_field*)data); }
+
+// fieldCreate(nx, ny, batch, dtype) -> external handle
+napi_value FieldCreate(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  doe_field* f = nullptr;
+  const int rc = doe_field_create(c, (int)num(env, a.v[0]), (int)num(env, a.v[1]),
+                                  a.n > 2 ? (int)num(env, a.v[2], 1) : 1,
+                                  a.n > 3 ? (int)num(env, a.v[3], DOE_F32) : DOE_F32, &f);
+  if (throw_status(env, rc)) return nullptr;
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, f, finalize_field, nullptr, &ext));
+  return ext;
+}
+
+napi_value FieldUpload(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* f = field_of(env, a.v[0]);
+  double* re = f64(env, a.v[1], nullptr);
+  double* im = f64(env, a.v[2], nullptr);
+  if (!f || !re || !im) {
+    napi_throw_type_error(env, nullptr, "fieldUpload(handle, re, im)");
+    return nullptr;
+  }
+  throw_status(env, doe_field_upload(ctx(env), f, re, im));
+  return nullptr;
+}
+
+napi_value FieldDownload(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* f = field_of(env, a.v[0]);
+  double* re = f64(env, a.v[1], nullptr);
+  double* im = f64(env, a.v[2], nullptr);
+  if (!f || !re || !im) {
+    napi_throw_type_error(env, nullptr, "fieldDownload(handle, re, im)");
+    return nullptr;
+  }
+  throw_status(env, doe_field_download(ctx(env), f, re, im));
+  return nullptr;
+}
+
+// per-batch distances: a number or a Float64Array
+std::vector<double> zlist(napi_env env, napi_value v, int batch) {
+  size_t n = 0;
+  double* p = f64(env, v, &n);
+  if (p) return std::vector<double>(p, p + n);
+  return std::vector<double>(batch, num(env, v));
+}
+
+int batch_of(doe_field* f) {
+  int b = 1;
+  doe_field_info(f, nullptr, nullptr, &b, nullptr, nullptr, nullptr);
+  return b;
+}
+
+// asm(hin, hout, dx, dy, lambda, z, bandLimit, evanescent, softEdge)
+napi_value Asm(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* fi = field_of(env, a.v[0]);
+  doe_field* fo = field_of(env, a.v[1]);
+  if (!fi || !fo || a.n < 9) {
+    napi_throw_type_error(env, nullptr, "asm(hin, hout, dx, dy, lambda, z, bandLimit, evanescent, softEdge)");
+    return nullptr;
+  }
+  std::vector<double> z = zlist(env, a.v[5], batch_of(fi));
+  throw_status(env, doe_asm(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
+                            z.data(), boolean(env, a.v[6], true), boolean(env, a.v[7], true),
+                            num(env, a.v[8], 0.02)));
+  return nullptr;
+}
+
+// asmTransfer(nx, ny, dx, dy, lambda, z, bandLimit, evanescent, softEdge)
+napi_value AsmTransfer(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  const int nx = (int)num(env, a.v[0]), ny = (int)num(env, a.v[1]);
+  doe_field* f = nullptr;
+  if (throw_status(env, doe_field_create(c, nx, ny, 1, DOE_F64, &f))) return nullptr;
+  double frac = 0;
+  int rc = doe_asm_transfer(c, f, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
+                            num(env, a.v[5]), boolean(env, a.v[6], true), boolean(env, a.v[7], true),
+                            num(env, a.v[8], 0.02), &frac);
+  napi_value obj = nullptr;
+  if (rc == DOE_OK) {
+    double *re, *im;
+    napi_value vre = make_f64(env, (size_t)nx * ny, &re);
+    napi_value vim = make_f64(env, (size_t)nx * ny, &im);
+    rc = doe_field_download(c, f, re, im);
+    napi_create_object(env, &obj);
+    set(env, obj, "re", vre);
+    set(env, obj, "im", vim);
+    set(env, obj, "propagatingFraction", make_num(env, frac));
+  }
+  doe_field_free(f);
+  if (throw_status(env, rc)) return nullptr;
+  return obj;
+}
+
+// fresnel(hin, hout, dx, dy, lambda, z, inverse, inDx, inDy) -> {dx, dy}
+napi_value Fresnel(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* fi = field_of(env, a.v[0]);
+  doe_field* fo = field_of(env, a.v[1]);
+  if (!fi || !fo || a.n < 7) {
+    napi_throw_type_error(env, nullptr, "fresnel(hin, hout, dx, dy, lambda, z, inverse, inDx, inDy)");
+    return nullptr;
+  }
+  const int b = batch_of(fi);
+  std::vector<double> z = zlist(env, a.v[5], b), odx(b), ody(b);
+  const int rc = doe_fresnel(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
+                             z.data(), boolean(env, a.v[6], false), a.n > 7 ? num(env, a.v[7]) : 0,
+                             a.n > 8 ? num(env, a.v[8]) : 0, odx.data(), ody.data());
+  if (throw_status(env, rc)) return nullptr;
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "dx", make_num(env, odx[0]));
+  set(env, obj, "dy", make_num(env, ody[0]));
+  return obj;
+}
+
+// lens(hin, hout, dx, dy, lambda, f) -> {dx, dy}
+napi_value Lens(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* fi = field_of(env, a.v[0]);
+  doe_field* fo = field_of(env, a.v[1]);
+  if (!fi || !fo || a.n < 6) {
+    napi_throw_type_error(env, nullptr, "lens(hin, hout, dx, dy, lambda, f)");
+    return nullptr;
+  }
+  const int b = batch_of(fi);
+  std::vector<double> f = zlist(env, a.v[5], b), odx(b), ody(b);
+  const int rc = doe_lens(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
+                          f.data(), odx.data(), ody.data());
+  if (throw_status(env, rc)) return nullptr;
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "dx", make_num(env, odx[0]));
+  set(env, obj, "dy", make_num(env, ody[0]));
+  return obj;
+}
+
+double prop(napi_env env, napi_value obj, const char* k, double dflt) {
+  napi_value v;
+  bool has = false;
+  napi_has_named_property(env, obj, k, &has);
+  if (!has) return dflt;
+  napi_get_named_property(env, obj, k, &v);
+  return num(env, v, dflt);
+}
+
+// gsRun(plan, illumination, targetAmp, roi|null, initPhase|null, seed)
+napi_value GsRun(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  if (a.n < 3) {
+    napi_throw_type_error(env, nullptr, "gsRun(plan, illumination, targetAmp, roi, initPhase, seed)");
+    return nullptr;
+  }
+  doe_gs_plan p{};
+  p.mode = (int)prop(env, a.v[0], "mode", DOE_MODE_IMAGE_PLANE);
+  p.algorithm = (int)prop(env, a.v[0], "algorithm", DOE_ALG_GS);
+  p.iterations = (int)prop(env, a.v[0], "iterations", 60);
+  p.nx = (int)prop(env, a.v[0], "nx", 0);
+  p.ny = (int)prop(env, a.v[0], "ny", p.nx);
+  p.dx = prop(env, a.v[0], "dx", 8e-6);
+  p.dy = prop(env, a.v[0], "dy", p.dx);
+  p.lambda = prop(env, a.v[0], "lambda", 532e-9);
+  p.z = prop(env, a.v[0], "z", 0.03);
+  p.beta = prop(env, a.v[0], "beta", 0.9);
+  p.feedback = prop(env, a.v[0], "feedback", 0.0);
+  p.escape = (int)prop(env, a.v[0], "escape", 0);
+  p.dtype = (int)prop(env, a.v[0], "dtype", DOE_F32);
+  const size_t n = (size_t)p.nx * (size_t)p.ny;
+  size_t l1 = 0, l2 = 0, l3 = 0, l4 = 0;
+  double* ill = f64(env, a.v[1], &l1);
+  double* tgt = f64(env, a.v[2], &l2);
+  double* roi = a.n > 3 ? f64(env, a.v[3], &l3) : nullptr;
+  double* ip = a.n > 4 ? f64(env, a.v[4], &l4) : nullptr;
+  if (!ill || !tgt || l1 != n || l2 != n || (roi && l3 != n) || (ip && l4 != n)) {
+    napi_throw_error(env, nullptr, "gsRun: planes must be Float64Array of length nx*ny");
+    return nullptr;
+  }
+  const unsigned seed = a.n > 5 ? (unsigned)(long long)num(env, a.v[5], 12345) : 12345u;
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  double *ph, *am, *in, *we, *er, *ef;
+  napi_value vph = make_f64(env, n, &ph), vam = make_f64(env, n, &am), vin = make_f64(env, n, &in),
+             vwe = make_f64(env, n, &we);
+  const size_t K = p.iterations > 0 ? (size_t)p.iterations : 0;
+  napi_value ver = make_f64(env, K, &er), vef = make_f64(env, K, &ef);
+  double rmse = 0;
+  const int rc = doe_gs_run(c, &p, ill, tgt, roi, ip, seed, ph, K ? er : nullptr, K ? ef : nullptr,
+                            am, in, we, &rmse);
+  if (throw_status(env, rc)) return nullptr;
+  napi_value obj;
+  napi_create_object(env, &obj);
+  set(env, obj, "phase", vph);
+  set(env, obj, "amplitude", vam);
+  set(env, obj, "intensity", vin);
+  set(env, obj, "weights", vwe);
+  set(env, obj, "error", ver);
+  set(env, obj, "efficiency", vef);
+  set(env, obj, "rmse", make_num(env, rmse));
+  return obj;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  napi_property_descriptor d[] = {
+      {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"deviceCount", nullptr, DeviceCount, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fft2d", nullptr, Fft2d, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fft1d", nullptr, Fft1d, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldCreate", nullptr, FieldCreate, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldUpload", nullptr, FieldUpload, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldDownload", nullptr, FieldDownload, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"asm", nullptr, Asm, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"asmTransfer", nullptr, AsmTransfer, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fresnel", nullptr, Fresnel, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"lens", nullptr, Lens, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"gsRun", nullptr, GsRun, nullptr, nullptr, nullptr, napi_default, nullptr},
+  };
+  napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -74,6 +74,8 @@ SIGNATURES = [
     ("doe_fft2d", C.c_int, [_VP, _VP, C.c_int]),
     ("doe_fft2d_host", C.c_int, [_VP, c_double_p, c_double_p, C.c_long, C.c_int, C.c_int, C.c_int,
                                  C.c_int]),
+    ("doe_fft1d_host", C.c_int, [_VP, c_double_p, c_double_p, C.c_long, C.c_long, C.c_long,
+                                 C.c_long, C.c_int, C.c_int]),
     ("doe_asm", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p, C.c_int,
                           C.c_int, C.c_double]),
     ("doe_asm_transfer", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double, C.c_double,

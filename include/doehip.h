@@ -91,6 +91,13 @@ int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
 int doe_fft2d_host(doe_ctx* ctx, double* re, double* im, long len, int nx, int ny, int inverse,
                    int dtype);
 
+/* Host drop-in of fft1d(re, im, n, stride, offset, inverse) (ref:
+ * src/fft.js:86-139): one strided line of `n` points of planes of length
+ * `len`; n <= 1 is a no-op, a non power of two is DOE_ERR_INVALID with the
+ * reference's message. */
+int doe_fft1d_host(doe_ctx* ctx, double* re, double* im, long len, long n, long stride,
+                   long offset, int inverse, int dtype);
+
 /* ---- angular spectrum (ref: src/propagate.js:57-133) ------------------- *
  * out = ASM_z(in) for each grid b with distance z[b] (in may equal out).
  * band_limit: Matsushima–Shimobaba local limit with raised-cosine edge of

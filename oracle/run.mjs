@@ -132,6 +132,15 @@ if (op === 'fft2d' || op === 'dft2d') {
   res.metrics = m;
   res.design_ms = t1 - t0;
   res.simulate_metrics_ms = t2 - t1;
+} else if (op === 'projector_eval') {
+  // screenMetrics(simulate()) of a given element phase (e.g. one designed on
+  // the GPU) under ProjectorSystem(cfg)
+  const sys = new ProjectorSystem(a.cfg);
+  sys.result = { phase: inp('phase'), illumination: sys.illumination() };
+  const field = sys.simulate(a.simulate || {});
+  res.metrics = sys.screenMetrics(field);
+  out('screen_re', field.re);
+  out('screen_im', field.im);
 } else if (op === 'metrics') {
   const p = inp('plane');
   const t = inp('target');

@@ -1,0 +1,187 @@
+// GPU check of the JavaScript drop-in (diffractiveopticalelement_amd/js)
+// against the FP64 oracle, run by tests/test_gpu_dropin.py.  Prints one JSON
+// object per check and exits non-zero if any check failed.
+import * as gpu from '../../diffractiveopticalelement_amd/js/index.js';
+import * as offt from '../../oracle/js/fft.js';
+import * as oprop from '../../oracle/js/propagate.js';
+import { ComplexField as OField } from '../../oracle/js/field.js';
+import { ProjectorSystem as OProjector } from '../../oracle/js/projector.js';
+import { designDOE as oDesign } from '../../oracle/js/doe.js';
+
+const results = [];
+function check(name, ok, info) {
+  results.push({ name, ok: !!ok, info });
+  console.log(JSON.stringify({ name, ok: !!ok, info }));
+}
+function rel(ar, ai, br, bi) {
+  let n = 0;
+  let d = 0;
+  for (let k = 0; k < ar.length; k++) {
+    n += (ar[k] - br[k]) ** 2 + (ai[k] - bi[k]) ** 2;
+    d += br[k] ** 2 + bi[k] ** 2;
+  }
+  return Math.sqrt(n / d);
+}
+function lcg(n, seed) {
+  let s = seed;
+  const a = new Float64Array(n);
+  for (let i = 0; i < n; i++) {
+    s = (1103515245 * s + 12345) % 2147483648;
+    a[i] = s / 2147483648 - 0.5;
+  }
+  return a;
+}
+function pair(nx, ny, dx, seed) {
+  const g = new gpu.ComplexField(nx, ny, dx, dx);
+  const o = new OField(nx, ny, dx, dx);
+  const re = lcg(nx * ny, seed);
+  const im = lcg(nx * ny, seed + 1);
+  g.re.set(re);
+  g.im.set(im);
+  o.re.set(re);
+  o.im.set(im);
+  return [g, o];
+}
+const LAM = 532e-9;
+
+for (const prec of ['f32', 'f64']) {
+  gpu.setPrecision(prec);
+  const tol = prec === 'f64' ? 1e-12 : 1e-5;
+  // fft2d / fft1d (ref: fft.js:86-154)
+  {
+    const re = lcg(64 * 32, 3);
+    const im = lcg(64 * 32, 4);
+    const a = [Float64Array.from(re), Float64Array.from(im)];
+    const b = [Float64Array.from(re), Float64Array.from(im)];
+    gpu.fft2d(a[0], a[1], 64, 32, false);
+    offt.fft2d(b[0], b[1], 64, 32, false);
+    check(`fft2d ${prec}`, rel(a[0], a[1], b[0], b[1]) < tol, rel(a[0], a[1], b[0], b[1]));
+    gpu.fft2d(a[0], a[1], 64, 32, true);
+    check(`fft2d round trip ${prec}`, rel(a[0], a[1], re, im) < tol, rel(a[0], a[1], re, im));
+    const c = [Float64Array.from(re), Float64Array.from(im)];
+    const d = [Float64Array.from(re), Float64Array.from(im)];
+    gpu.fft1d(c[0], c[1], 32, 64, 5, false);
+    offt.fft1d(d[0], d[1], 32, 64, 5, false);
+    check(`fft1d strided ${prec}`, rel(c[0], c[1], d[0], d[1]) < tol, rel(c[0], c[1], d[0], d[1]));
+  }
+  // propagators (ref: propagate.js)
+  for (const band of [false, true]) {
+    const [g, o] = pair(128, 128, 8e-6, 11);
+    const gr = gpu.propagateAngularSpectrum(g, LAM, 0.04, { bandLimit: band });
+    const or = oprop.propagateAngularSpectrum(o, LAM, 0.04, { bandLimit: band });
+    const e = rel(gr.re, gr.im, or.re, or.im);
+    check(`asm band=${band} ${prec}`, e < tol, e);
+    gpu.propagateAngularSpectrumInPlace(g, LAM, 0.04, { bandLimit: band });
+    oprop.propagateAngularSpectrumInPlace(o, LAM, 0.04, { bandLimit: band });
+    const e2 = rel(g.re, g.im, o.re, o.im);
+    check(`asm in place band=${band} ${prec}`, e2 < tol, e2);
+  }
+  {
+    const [g, o] = pair(128, 128, 8e-6, 21);
+    const fg = gpu.propagateFresnelFFT(g, LAM, 0.03);
+    const fo = oprop.propagateFresnelFFT(o, LAM, 0.03);
+    const e = rel(fg.field.re, fg.field.im, fo.field.re, fo.field.im);
+    check(`fresnel ${prec}`, e < tol && fg.dx === fo.dx && fg.undersampled === fo.undersampled, e);
+    const bg = gpu.propagateFresnelFFTInverse(fg.field, LAM, 0.03, { dx: 8e-6, dy: 8e-6 });
+    const bo = oprop.propagateFresnelFFTInverse(fo.field, LAM, 0.03, { dx: 8e-6, dy: 8e-6 });
+    const e2 = rel(bg.re, bg.im, bo.re, bo.im);
+    check(`fresnel inverse ${prec}`, e2 < tol, e2);
+    const lg = gpu.focalPlaneSpectrum(g, LAM, 0.05);
+    const lo = oprop.focalPlaneSpectrum(o, LAM, 0.05);
+    const e3 = rel(lg.re, lg.im, lo.re, lo.im);
+    check(`lens ${prec}`, e3 < tol && lg.dx === lo.dx, e3);
+    let threw = false;
+    try {
+      gpu.propagateFresnelFFTInverse(fg.field, LAM, 0.03, {});
+    } catch (err) {
+      threw = /pass the DOE-plane pitch/.test(err.message);
+    }
+    check('fresnel inverse without pitch throws', threw);
+  }
+  {
+    const Hg = gpu.angularSpectrumTransferFunction(64, 64, 8e-6, 8e-6, LAM, 0.05, { softEdge: 0.01 });
+    const Ho = oprop.angularSpectrumTransferFunction(64, 64, 8e-6, 8e-6, LAM, 0.05, { softEdge: 0.01 });
+    const e = rel(Hg.re, Hg.im, Ho.re, Ho.im);
+    check('transfer function', e < 1e-12 && Hg.propagatingFraction === Ho.propagatingFraction, e);
+  }
+}
+
+// device mirror semantics: host writes after a device op are honoured
+{
+  gpu.setPrecision('f64');
+  const [g, o] = pair(32, 32, 8e-6, 5);
+  gpu.propagateAngularSpectrumInPlace(g, LAM, 0.02, { bandLimit: false });
+  g.re[7] += 1.0; // host write on the downloaded mirror
+  oprop.propagateAngularSpectrumInPlace(o, LAM, 0.02, { bandLimit: false });
+  o.re[7] += 1.0;
+  gpu.propagateAngularSpectrumInPlace(g, LAM, -0.02, { bandLimit: false });
+  oprop.propagateAngularSpectrumInPlace(o, LAM, -0.02, { bandLimit: false });
+  check('mirror write-back', rel(g.re, g.im, o.re, o.im) < 1e-12, rel(g.re, g.im, o.re, o.im));
+}
+
+// ProjectorSystem end to end: design on the GPU, screenMetrics within 1e-3
+function metricsClose(a, b, keys, tol) {
+  const bad = keys.filter((k) => !(Math.abs(a[k] - b[k]) < tol));
+  return { ok: bad.length === 0, bad: bad.map((k) => [k, a[k], b[k]]) };
+}
+const KEYS = ['rmse', 'efficiency', 'correlation', 'ssim', 'fluxFraction', 'zeroOrder', 'uniformity', 'flatness'];
+for (const prec of ['f32', 'f64']) {
+  gpu.setPrecision(prec);
+  for (const mode of ['image-plane', 'far-field']) {
+    const cfg = { n: 128, pitch: 8e-6, lambda: 532e-9, mode, iterations: 40, seed: 12345, target: { kind: 'text', text: 'DOE' } };
+    if (mode === 'image-plane') cfg.distance = 0.03;
+    const gs = new gpu.ProjectorSystem(cfg);
+    const os = new OProjector(cfg);
+    const rg = gs.design();
+    const ro = os.design();
+    const mg = gs.screenMetrics(gs.simulate());
+    const mo = os.screenMetrics(os.simulate());
+    const c = metricsClose(mg, mo, KEYS, 1e-3);
+    check(`projector ${mode} ${prec} screenMetrics`, c.ok, c.bad);
+    let maxe = 0;
+    for (let i = 0; i < 10; i++) maxe = Math.max(maxe, Math.abs(rg.error[i] - ro.error[i]), Math.abs(rg.efficiency[i] - ro.efficiency[i]));
+    check(`projector ${mode} ${prec} first 10 iterates`, maxe < 1e-4, maxe);
+    check(`projector ${mode} ${prec} algorithm`, rg.algorithm === ro.algorithm && rg.mode === ro.mode, [rg.algorithm, rg.mode]);
+    if (mode === 'image-plane') {
+      // the reference's own thresholds (ref test/projector.test.js:62-84)
+      check(`projector thresholds ${prec}`, mg.rmse < 0.06 && mg.correlation > 0.98 && mg.efficiency > 0.55 &&
+        mg.ssim > 0.95 && mg.zeroOrder < 0.01 && mg.fluxFraction > 0.55 && mg.snr > 15, mg);
+    }
+  }
+}
+
+// Parseval through design + simulate (ref test/projector.test.js:86-102), FP64 mode
+{
+  gpu.setPrecision('f64');
+  const sys = new gpu.ProjectorSystem({ n: 64, pitch: 8e-6, lambda: 532e-9, distance: 0.03, iterations: 10, target: { kind: 'disc' } });
+  sys.design({ iterations: 10 });
+  const irr = sys.simulate().intensity();
+  let screen = 0;
+  for (const v of irr) screen += v;
+  let inc = 0;
+  for (const v of sys.illumination()) inc += v * v;
+  check('Parseval through design+simulate (f64)', Math.abs(screen / inc - 1) < 1e-9, screen / inc - 1);
+}
+
+// designDOE determinism and wrapped phase (ref test/doe.test.js:350-382)
+{
+  gpu.setPrecision('f32');
+  const t = new Float64Array(64 * 64);
+  for (let j = 0; j < 64; j++) for (let i = 0; i < 64; i++) t[i + j * 64] = Math.hypot(i - 32, j - 32) <= 20 ? 1 : 0;
+  const cfg = { nx: 64, ny: 64, dx: 8e-6, lambda: LAM, z: 0.03, iterations: 10, target: t, mode: 'image-plane' };
+  const a = gpu.designDOE(cfg);
+  const b = gpu.designDOE(cfg);
+  let md = 0;
+  let wrapped = true;
+  for (let k = 0; k < a.phase.length; k++) {
+    md = Math.max(md, Math.abs(a.phase[k] - b.phase[k]));
+    if (Math.abs(a.phase[k]) > Math.PI + 1e-6) wrapped = false;
+  }
+  check('designDOE deterministic', md < 1e-12 && wrapped && a.error.length === 10, md);
+  const ref = oDesign(cfg);
+  check('designDOE error history vs oracle', Math.abs(a.error[9] - ref.error[9]) < 1e-3, [a.error[9], ref.error[9]]);
+}
+
+const failed = results.filter((r) => !r.ok);
+console.log(JSON.stringify({ summary: true, total: results.length, failed: failed.length }));
+process.exit(failed.length ? 1 : 0);
