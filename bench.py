@@ -66,6 +66,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also copy the 512 MiB batch once (device-to-device), so a profiler's "
+                         "byte counters can be scaled against a copy of known size")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the hot path shards by instance "
@@ -88,6 +91,10 @@ def main() -> int:
         view.upload(amp * np.cos(ph), amp * np.sin(ph))
         view.free()
     pts = B * N * N
+    if args.copy_calibration:
+        ctx.sync()
+        ctx.lib.doe_field_copy(ctx.h, field.h, scratch.h)
+        ctx.sync()
 
     ctx.set_flags(asynchronous=True, profile=False)
     for _ in range(args.warmup):
