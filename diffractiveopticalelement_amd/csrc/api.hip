@@ -74,6 +74,17 @@ int get_twiddle(doe_ctx* ctx, int n, int tsize, Twiddle* out) {
   return DOE_OK;
 }
 
+int make_plan2(doe_ctx* ctx, int nx, int ny, int tsize, int n1, Plan2* p) {
+  p->nx = nx;
+  p->ny = ny;
+  p->n1 = n1;
+  p->n2 = ny / n1;
+  int rc = get_twiddle(ctx, nx, tsize, &p->tx);
+  if (!rc) rc = get_twiddle(ctx, ny, tsize, &p->ty);
+  if (!rc) rc = get_twiddle(ctx, p->n2, tsize, &p->tc);
+  return rc;
+}
+
 hipEvent_t prof_event(doe_ctx* ctx) {
   if (!ctx->ev_pool.empty()) {
     hipEvent_t e = ctx->ev_pool.back();
@@ -130,6 +141,22 @@ int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf) {
     DOE_HIP(hipMemset(ctx->counters, 0, cap * sizeof(unsigned)));
     ctx->counters_cap = cap;
   }
+  return DOE_OK;
+}
+
+int ensure_work(doe_ctx* ctx, size_t bytes) {
+  if (bytes <= ctx->work_cap) return DOE_OK;
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->work_re) DOE_HIP(hipFree(ctx->work_re));
+  if (ctx->work_im) DOE_HIP(hipFree(ctx->work_im));
+  ctx->work_re = ctx->work_im = nullptr;
+  ctx->work_cap = 0;
+  if (hipMalloc(&ctx->work_re, bytes) != hipSuccess || hipMalloc(&ctx->work_im, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("work buffer allocation failed");
+    return DOE_ERR_OOM;
+  }
+  ctx->work_cap = bytes;
   return DOE_OK;
 }
 
@@ -315,6 +342,8 @@ int doe_ctx_destroy(doe_ctx* ctx) {
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->partials) hipFree(ctx->partials);
   if (ctx->counters) hipFree(ctx->counters);
+  if (ctx->work_re) hipFree(ctx->work_re);
+  if (ctx->work_im) hipFree(ctx->work_im);
   for (auto& r : ctx->prof) {
     hipEventDestroy(r.a);
     hipEventDestroy(r.b);

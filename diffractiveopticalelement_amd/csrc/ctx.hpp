@@ -77,6 +77,10 @@ struct doe_ctx {
   size_t partials_cap = 0;  // doubles
   unsigned* counters = nullptr;
   size_t counters_cap = 0;
+  // work planes for passes that permute rows (split column passes)
+  void* work_re = nullptr;
+  void* work_im = nullptr;
+  size_t work_cap = 0;  // bytes per plane
 };
 
 struct doe_field {
@@ -97,6 +101,8 @@ int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev);
 // Ensure `n` doubles of partials and `nf` zeroed counters.
 int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf);
 int ensure_stage(doe_ctx* ctx, size_t bytes);
+// Ensure two work planes of `bytes` each (ctx->work_re / work_im).
+int ensure_work(doe_ctx* ctx, size_t bytes);
 
 // Compile-time dispatch over the supported power-of-two line lengths.
 template <class F>
@@ -115,7 +121,8 @@ inline bool with_n(int n, F&& f) {
 }
 constexpr int kMaxN = 8192;
 
-// Phase-retrieval loops are built for sides 16..4096.
+// Phase-retrieval loops are built for sides 64..4096 (their row-pass
+// reductions need every workgroup inside one field).
 template <class F>
 inline bool with_n_gs(int n, F&& f) {
   switch (n) {
@@ -123,8 +130,8 @@ inline bool with_n_gs(int n, F&& f) {
   case v:                                    \
     f(std::integral_constant<int, v>{});     \
     return true;
-    DOE_CASE(16) DOE_CASE(32) DOE_CASE(64) DOE_CASE(128) DOE_CASE(256) DOE_CASE(512)
-    DOE_CASE(1024) DOE_CASE(2048) DOE_CASE(4096)
+    DOE_CASE(64) DOE_CASE(128) DOE_CASE(256) DOE_CASE(512) DOE_CASE(1024) DOE_CASE(2048)
+    DOE_CASE(4096)
 #undef DOE_CASE
     default:
       return false;
@@ -150,38 +157,110 @@ inline hipError_t finish(doe_ctx* ctx) {
   return hipStreamSynchronize(ctx->stream);
 }
 
-template <typename T, typename S, int N, int MODE, bool INV1, bool INV2, class Op>
+// Column split N1 of the four-step column factorisation for an nx-by-ny grid
+// (1 = plain passes).  FP32 compute only; the row workgroup must hold N1 rows.
+template <typename T>
+inline int col_split(int nx, int ny) {
+  if (sizeof(T) == 8) return 1;
+  int n1 = ny == 1024 ? 2 : ((ny == 2048 || ny == 4096) ? 4 : 1);
+  if (nx < 256 || (long)nx * n1 > 16384) n1 = 1;
+  return n1;
+}
+template <typename T, int NX, int N1>
+constexpr bool split_ok() {
+  return N1 == 1 || (sizeof(T) == 4 && NX >= 256 && NX * N1 <= 16384);
+}
+template <class F>
+inline bool with_n1(int n1, F&& f) {
+  switch (n1) {
+    case 1: f(std::integral_constant<int, 1>{}); return true;
+    case 2: f(std::integral_constant<int, 2>{}); return true;
+    case 4: f(std::integral_constant<int, 4>{}); return true;
+    default: return false;
+  }
+}
+
+// Twiddle tables of a 2-D plan: rows (nx), cross-row step (ny), columns (N2).
+struct Plan2 {
+  int nx = 0, ny = 0, n1 = 1, n2 = 0;
+  Twiddle tx, ty, tc;
+};
+int make_plan2(doe_ctx* ctx, int nx, int ny, int tsize, int n1, Plan2* p);
+
+template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
+          bool XOUT, class Op>
 inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const S* ire,
-                             const S* iim, S* ore, S* oim, int ny, int batch, const Twiddle& tw) {
-  using CF = RowCfg<T, N>;
-  const long total = (long)ny * batch;
-  const long grid = (total + CF::LINES - 1) / CF::LINES;
-  prof_begin(ctx, tag);
-  hipLaunchKernelGGL((k_row_pass<T, S, N, MODE, INV1, INV2, Op>), dim3(unsigned(grid)),
-                     dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, ny, total,
-                     (const T*)tw.re, (const T*)tw.im);
-  prof_end(ctx);
-  return hipGetLastError();
+                             const S* iim, S* ore, S* oim, int batch, const Plan2& pl) {
+  if constexpr (!split_ok<T, NX, N1>()) {
+    return hipErrorInvalidValue;  // never selected by col_split
+  } else {
+    using CF = RowCfg<T, NX, N1>;
+    const long groups = (long)(pl.ny / N1) * batch;
+    const long grid = (groups + CF::G - 1) / CF::G;
+    if constexpr (Op::NACC > 0) {
+      if ((pl.ny / N1) % CF::G != 0) return hipErrorInvalidValue;
+    }
+    prof_begin(ctx, tag);
+    hipLaunchKernelGGL((k_row_pass<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op>),
+                       dim3(unsigned(grid)), dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim,
+                       pl.ny, groups, (const T*)pl.tx.re, (const T*)pl.tx.im, (const T*)pl.ty.re,
+                       (const T*)pl.ty.im);
+    prof_end(ctx);
+    return hipGetLastError();
+  }
 }
 
-template <typename T, typename S, int N, int MODE, bool INV1, bool INV2, class Op>
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op>
 inline hipError_t launch_col(doe_ctx* ctx, const char* tag, const Op& op, const S* ire,
-                             const S* iim, S* ore, S* oim, int nx, int batch, const Twiddle& tw) {
-  using CF = ColCfg<T, N>;
-  const int tiles = (nx + CF::C - 1) / CF::C;
-  const long grid = (long)tiles * batch;
+                             const S* iim, S* ore, S* oim, int batch, const Plan2& pl) {
+  using CF = ColCfg<T, N2>;
+  const int tiles = (pl.nx + CF::C - 1) / CF::C;
+  const long grid = (long)tiles * pl.n1 * batch;
   prof_begin(ctx, tag);
-  hipLaunchKernelGGL((k_col_pass<T, S, N, MODE, INV1, INV2, Op>), dim3(unsigned(grid)),
-                     dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, nx, tiles,
-                     (const T*)tw.re, (const T*)tw.im);
+  hipLaunchKernelGGL((k_col_pass<T, S, N2, MODE, INV1, INV2, Op>), dim3(unsigned(grid)),
+                     dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, pl.nx, tiles, pl.n1,
+                     (const T*)pl.tc.re, (const T*)pl.tc.im);
   prof_end(ctx);
   return hipGetLastError();
 }
 
-template <typename T, int N>
-inline int col_tiles(int nx) {
-  return (nx + ColCfg<T, N>::C - 1) / ColCfg<T, N>::C;
+// Row pass at the plan's split (instantiates only the valid (NX, N1) pairs).
+template <typename T, typename S, int MODE, bool INV1, bool INV2, bool XIN, bool XOUT, class Op,
+          class NF>
+inline hipError_t row_pass(doe_ctx* ctx, const char* tag, const Op& op, const S* ire, const S* iim,
+                           S* ore, S* oim, int batch, const Plan2& pl, NF&& with_sizes) {
+  hipError_t e = hipErrorInvalidValue;
+  with_sizes(pl.nx, [&](auto NN) {
+    constexpr int NX = decltype(NN)::value;
+    with_n1(pl.n1, [&](auto MM) {
+      constexpr int N1 = decltype(MM)::value;
+      e = launch_row<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op>(ctx, tag, op, ire, iim, ore,
+                                                                    oim, batch, pl);
+    });
+  });
+  return e;
 }
+
+template <typename T, typename S, int MODE, bool INV1, bool INV2, class Op, class NF>
+inline hipError_t col_pass(doe_ctx* ctx, const char* tag, const Op& op, const S* ire, const S* iim,
+                           S* ore, S* oim, int batch, const Plan2& pl, NF&& with_sizes) {
+  hipError_t e = hipErrorInvalidValue;
+  with_sizes(pl.n2, [&](auto NN) {
+    e = launch_col<T, S, decltype(NN)::value, MODE, INV1, INV2, Op>(ctx, tag, op, ire, iim, ore,
+                                                                     oim, batch, pl);
+  });
+  return e;
+}
+
+// size dispatchers usable as the `with_sizes` argument
+struct AllSizes {
+  template <class F>
+  bool operator()(int n, F&& f) const { return with_n(n, f); }
+};
+struct GsSizes {
+  template <class F>
+  bool operator()(int n, F&& f) const { return with_n_gs(n, f); }
+};
 
 // Family entry points (defined in their own translation units; each
 // (T, S) instantiation is compiled as a separate object, DOE_PART).

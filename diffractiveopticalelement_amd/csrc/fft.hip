@@ -6,51 +6,59 @@ namespace doe {
 
 template <typename T, typename S>
 int run_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
-  Twiddle twx, twy;
-  int rc = get_twiddle(ctx, f->nx, sizeof(T), &twx);
-  if (rc) return rc;
-  rc = get_twiddle(ctx, f->ny, sizeof(T), &twy);
+  Plan2 pl;
+  int rc = make_plan2(ctx, f->nx, f->ny, sizeof(T), col_split<T>(f->nx, f->ny), &pl);
   if (rc) return rc;
   S* re = (S*)f->re;
   S* im = (S*)f->im;
-  hipError_t e = hipSuccess;
+  // A split column pass reads rows in one order and writes them in another,
+  // so it cannot run in place: the two passes ping-pong through the work
+  // planes (same HBM traffic as in place).
+  S* wre = re;
+  S* wim = im;
+  if (pl.n1 > 1) {
+    rc = ensure_work(ctx, f->count() * sizeof(S));
+    if (rc) return rc;
+    wre = (S*)ctx->work_re;
+    wim = (S*)ctx->work_im;
+  }
   const OpNone op{};
-  with_n(f->nx, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    if (inverse)
-      e = launch_row<T, S, N, 0, true, false>(ctx, "fft_row_inv", op, re, im, re, im, f->ny, f->batch, twx);
-    else
-      e = launch_row<T, S, N, 0, false, false>(ctx, "fft_row_fwd", op, re, im, re, im, f->ny, f->batch, twx);
-  });
-  DOE_HIP(e);
-  with_n(f->ny, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    if (inverse)
-      e = launch_col<T, S, N, 0, true, false>(ctx, "fft_col_inv", op, re, im, re, im, f->nx, f->batch, twy);
-    else
-      e = launch_col<T, S, N, 0, false, false>(ctx, "fft_col_fwd", op, re, im, re, im, f->nx, f->batch, twy);
-  });
-  DOE_HIP(e);
+  hipError_t e;
+  if (!inverse) {
+    // rows (+ the column split's first steps), then the column blocks
+    e = row_pass<T, S, 0, false, false, false, true>(ctx, "fft_row_fwd", op, re, im, wre, wim,
+                                                      f->batch, pl, AllSizes{});
+    DOE_HIP(e);
+    e = col_pass<T, S, 0, false, false>(ctx, "fft_col_fwd", op, wre, wim, re, im, f->batch, pl,
+                                        AllSizes{});
+    DOE_HIP(e);
+  } else {
+    e = col_pass<T, S, 0, true, false>(ctx, "fft_col_inv", op, re, im, wre, wim, f->batch, pl,
+                                       AllSizes{});
+    DOE_HIP(e);
+    e = row_pass<T, S, 0, true, false, true, false>(ctx, "fft_row_inv", op, wre, wim, re, im,
+                                                     f->batch, pl, AllSizes{});
+    DOE_HIP(e);
+  }
   return DOE_OK;
 }
 
 // Row pass only: a batch of independent 1-D transforms along x.
 template <typename T, typename S>
 int run_fft_rows(doe_ctx* ctx, doe_field* f, int inverse) {
-  Twiddle twx;
-  int rc = get_twiddle(ctx, f->nx, sizeof(T), &twx);
+  Plan2 pl;
+  int rc = make_plan2(ctx, f->nx, f->ny, sizeof(T), 1, &pl);
   if (rc) return rc;
   S* re = (S*)f->re;
   S* im = (S*)f->im;
-  hipError_t e = hipSuccess;
   const OpNone op{};
-  with_n(f->nx, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    if (inverse)
-      e = launch_row<T, S, N, 0, true, false>(ctx, "fft_row_inv", op, re, im, re, im, f->ny, f->batch, twx);
-    else
-      e = launch_row<T, S, N, 0, false, false>(ctx, "fft_row_fwd", op, re, im, re, im, f->ny, f->batch, twx);
-  });
+  hipError_t e;
+  if (inverse)
+    e = row_pass<T, S, 0, true, false, false, false>(ctx, "fft_row_inv", op, re, im, re, im,
+                                                      f->batch, pl, AllSizes{});
+  else
+    e = row_pass<T, S, 0, false, false, false, false>(ctx, "fft_row_fwd", op, re, im, re, im,
+                                                       f->batch, pl, AllSizes{});
   DOE_HIP(e);
   return DOE_OK;
 }

@@ -194,25 +194,13 @@ __device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_r
 }
 
 // ---------------------------------------------------------------------------
-// Launch geometry.
-// Rows: T = N/P threads per line, LINES lines per workgroup (>= 256 threads).
-// Columns: C adjacent columns per workgroup, whole column resident; the tile
+// Launch geometry (the row pass geometry lives with the row kernel).
+// Columns: C adjacent columns per workgroup, whole line resident; the tile
 // is sized so C*N complex values fit the LDS budget.
 // ---------------------------------------------------------------------------
 template <int N>
 struct PChoice {
   static constexpr int P = N < 16 ? N : 16;
-};
-
-template <typename T, int N>
-struct RowCfg {
-  static constexpr int P = PChoice<N>::P;
-  static constexpr int T_LINE = N / P;
-  static constexpr int LINES = cmax(1, 256 / T_LINE);
-  static constexpr int WG = LINES * T_LINE;
-  static constexpr int LSTRIDE = N + N / 16 + 1;  // padded line stride in LDS
-  static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
-  static constexpr int LDS_ELEMS = NEED_LDS ? LINES * LSTRIDE : 1;
 };
 
 template <typename T, int N>

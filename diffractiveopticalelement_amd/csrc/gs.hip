@@ -9,15 +9,15 @@
 // index, so the loop runs on x~ = c.*x: the spectrum F(x~) is already in the
 // reference's centred index space (target/ROI/weights need no remap), and the
 // update x~' = illum .* z/|z| with z = F^-1(Y~') carries no checkerboard at
-// all.  Pass B (columns): F -> measure, project, WGS update -> F^-1.
+// all.  Pass B (column blocks): F -> measure, project, WGS update -> F^-1.
 // Pass A (rows): F^-1 -> phase, re-impose illumination -> F.
 //
-// Image-plane loop.  The two angular-spectrum propagations of an iteration
-// are split into column passes (spatial-domain work fused with the column
-// FFTs) and row passes (row FFT, x H or conj H, inverse row FFT).  Under WGS
-// the weight renormalisation is a grid-wide scalar needed before the
-// projection, so the measurement (A1a) and projection (A1b) are separate
-// column passes.
+// Image-plane loop.  Each angular-spectrum propagation is a row pass
+// (inverse rows -> spatial-domain work -> forward rows, with the column
+// split's cross-row steps) and a column pass (F . H or conj H . F^-1 on the
+// column blocks).  Under WGS the weight renormalisation is a grid-wide scalar
+// needed before the projection, so the measurement (A1a) and projection
+// (A1b) are separate row passes.
 #include <cmath>
 #include <limits>
 #include <vector>
@@ -551,37 +551,28 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   b.escape = plan->escape;
   b.weighted = weighted ? 1 : 0;
 
-  Twiddle twx, twy;
-  rc = get_twiddle(ctx, nx, sizeof(T), &twx);
+  Plan2 pl;
+  rc = make_plan2(ctx, nx, ny, sizeof(T), col_split<T>(nx, ny), &pl);
   if (rc) return rc;
-  rc = get_twiddle(ctx, ny, sizeof(T), &twy);
-  if (rc) return rc;
-  // reduction scratch: partials per column tile, one counter per field
-  int tiles = 0;
-  with_n_gs(ny, [&](auto NN) { tiles = col_tiles<T, decltype(NN)::value>(nx); });
-  rc = ensure_reduce(ctx, size_t(tiles) * R_N, 1);
+  // reduction scratch: partials per workgroup (row or column pass), one
+  // arrival counter per field
+  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_N, 1);
   if (rc) return rc;
   T* re = W.fre;
   T* im = W.fim;
-  hipError_t e = hipSuccess;
+  const GsSizes sz{};
 
   if (far) {
+    // x~ = c .* illum .* exp(i phi0) -> rows forward
     OpFFInit<T> op0{b, W.phi0};
-    with_n_gs(nx, [&](auto NN) {
-      e = launch_row<T, T, decltype(NN)::value, 0, false, false>(ctx, "ff_init_row", op0, re, im, re, im, ny, 1, twx);
-    });
-    DOE_HIP(e);
+    DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ff_init_row", op0, re, im, re, im, 1,
+                                                            pl, sz)));
     for (int it = 0; it < K; ++it) {
       OpFFB<T> opb{b, ctx->partials, ctx->counters, it, it == K - 1 ? 1 : 0};
-      with_n_gs(ny, [&](auto NN) {
-        e = launch_col<T, T, decltype(NN)::value, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, nx, 1, twy);
-      });
-      DOE_HIP(e);
+      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, 1, pl, sz)));
       OpFFA<T> opa{b, it == K - 1 ? 1 : 0};
-      with_n_gs(nx, [&](auto NN) {
-        e = launch_row<T, T, decltype(NN)::value, 1, true, false>(ctx, "ff_row_phase", opa, re, im, re, im, ny, 1, twx);
-      });
-      DOE_HIP(e);
+      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ff_row_phase", opa, re, im, re, im, 1,
+                                                           pl, sz)));
     }
   } else {
     OpAsmH h{};
@@ -603,54 +594,35 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     OpAsmH hc = h;
     hc.conj = 1;
     h.conj = 0;
-    auto row_h = [&](const OpAsmH& op, const char* tag) {
-      with_n_gs(nx, [&](auto NN) {
-        e = launch_row<T, T, decltype(NN)::value, 1, false, true>(ctx, tag, op, re, im, re, im, ny, 1, twx);
-      });
-    };
+    // x = illum .* exp(i phi0) -> rows forward -> column blocks F . H . F^-1
     OpIPInit<T> op0{b, W.phi0};
-    with_n_gs(ny, [&](auto NN) {
-      e = launch_col<T, T, decltype(NN)::value, 0, false, false>(ctx, "ip_init_col", op0, re, im, re, im, nx, 1, twy);
-    });
-    DOE_HIP(e);
-    row_h(h, "ip_row_h");
-    DOE_HIP(e);
+    DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_init_row", op0, re, im, re, im, 1,
+                                                           pl, sz)));
+    DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
     for (int it = 0; it < K; ++it) {
       if (!weighted || it == 0) {
         OpIPMP<T> op{b, ctx->partials, ctx->counters, it};
-        with_n_gs(ny, [&](auto NN) {
-          e = launch_col<T, T, decltype(NN)::value, 1, true, false>(ctx, "ip_col_project", op, re, im, re, im, nx, 1, twy);
-        });
-        DOE_HIP(e);
+        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_project", op, re, im, re, im,
+                                                             1, pl, sz)));
       } else {
         OpIPMeas<T> om{b, ctx->partials, ctx->counters, it - 1};
-        with_n_gs(ny, [&](auto NN) {
-          e = launch_col<T, T, decltype(NN)::value, 0, true, false>(ctx, "ip_col_measure", om, re, im, re, im, nx, 1, twy);
-        });
-        DOE_HIP(e);
+        DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re,
+                                                              im, 1, pl, sz)));
         OpIPProj<T> op{b, ctx->partials, ctx->counters, it};
-        with_n_gs(ny, [&](auto NN) {
-          e = launch_col<T, T, decltype(NN)::value, 0, false, false>(ctx, "ip_col_reproject", op, re, im, re, im, nx, 1, twy);
-        });
-        DOE_HIP(e);
+        DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_row_reproject", op, re, im, re,
+                                                               im, 1, pl, sz)));
       }
-      row_h(hc, "ip_row_hconj");
-      DOE_HIP(e);
+      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_hconj", hc, re, im, re, im, 1, pl, sz)));
       OpIPPhase<T> opp{b};
-      with_n_gs(ny, [&](auto NN) {
-        e = launch_col<T, T, decltype(NN)::value, 1, true, false>(ctx, "ip_col_phase", opp, re, im, re, im, nx, 1, twy);
-      });
-      DOE_HIP(e);
-      row_h(h, "ip_row_h");
-      DOE_HIP(e);
+      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_phase", opp, re, im, re, im, 1,
+                                                           pl, sz)));
+      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
     }
     // closing measurement: GS final forward amplitude (doe.js:418-426); WGS
     // best check + weight update of the last iteration.
     OpIPMeas<T> om{b, ctx->partials, ctx->counters, K - 1};
-    with_n_gs(ny, [&](auto NN) {
-      e = launch_col<T, T, decltype(NN)::value, 0, true, false>(ctx, "ip_col_measure", om, re, im, re, im, nx, 1, twy);
-    });
-    DOE_HIP(e);
+    DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re, im, 1,
+                                                          pl, sz)));
   }
 
   // ---- outputs ----------------------------------------------------------
@@ -724,6 +696,8 @@ extern "C" int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* i
   probe.batch = 1;
   int rc = check_field(ctx, &probe);
   if (rc) return rc;
+  DOE_REQUIRE(plan->nx >= 64 && plan->ny >= 64 && plan->nx <= 4096 && plan->ny <= 4096,
+              DOE_ERR_UNSUPPORTED, "doe_gs_run: grid sides must be in [64, 4096]");
   DOE_REQUIRE(plan->iterations >= 0, DOE_ERR_INVALID, "doe_gs_run: negative iteration count");
   DOE_REQUIRE(plan->mode == DOE_MODE_IMAGE_PLANE || plan->mode == DOE_MODE_FAR_FIELD,
               DOE_ERR_INVALID, "doe_gs_run: unknown mode");

@@ -1,28 +1,36 @@
 // Row and column pass kernels built on the line engine.
 //
-// A pass loads whole lines (rows or column tiles) of a planar complex batch,
-// runs an optional elementwise `pre` hook, one FFT, and either stores (MODE 0)
-// or runs a `mid` hook and a second FFT in the opposite stage order (MODE 1),
-// then a `post` hook, and stores.  Hooks see (field, line, position) so the
-// propagators' transfer functions, chirps and the phase-retrieval projections
-// are applied where the data already sits in registers.
+// The 2-D transform of an nx-by-ny grid is split into two passes over HBM.
+// The column transform of length ny = N1*N2 uses the four-step
+// factorisation
+//     X[k1 + N1*k2] = sum_n2 W_N2^(n2*k2) * W_ny^(n2*k1) * sum_n1 x[N2*n1 + n2] W_N1^(n1*k1)
+// so that its first two steps (an N1-point DFT across the rows
+// {n2 + N2*n1} and the twiddle W_ny^(n2*k1)) run inside the ROW pass, on
+// rows a workgroup already holds, and the COLUMN pass only performs N2-point
+// transforms over contiguous blocks of N2 rows.  That keeps the column pass's
+// tiles short enough to be C = 32 columns wide (128-byte row segments) while
+// each point is still read and written once per pass.  Between the passes
+// the column axis is in "four-step order": memory row N2*k1 + n2 holds the
+// twiddled sub-transform value for (k1, n2).  N1 = 1 degenerates to the plain
+// row/column passes.
 //
-// Ops with NACC > 0 accumulate FP64 partial sums per thread; the column pass
-// reduces them per workgroup into `partials` and the last workgroup of each
-// field (per-field arrival counter) folds that field's partials in a fixed
-// order and calls op.finalize(field, sums) — deterministic, graph-safe (the
-// counter is reset by the finalizing block; no memset needed).
+// A pass loads whole lines, runs an optional elementwise `pre` hook, one FFT,
+// and either stores (MODE 0) or runs a `mid` hook and a second FFT in the
+// opposite stage order (MODE 1), then a `post` hook, and stores.  Hooks see
+// (field, x, y) in natural order.  Ops with NACC > 0 accumulate FP64 partial
+// sums per thread; each workgroup reduces them into `partials`, and the last
+// workgroup of each field (per-field arrival counter) folds that field's
+// partials in a fixed order and calls op.finalize(field, sums) —
+// deterministic, and the counter is reset by the finalizing workgroup, so no
+// memset is needed between launches.
 #pragma once
 #include "fft_engine.hpp"
 
 namespace doe {
 
-struct NoAcc {};
-
 // Block-wide deterministic reduction of NACC doubles; result valid in tid 0.
 template <int NACC>
 __device__ __forceinline__ void block_sum(double (&acc)[NACC], double* sh, int tid, int nthr) {
-  // wave-level tree with shuffles (64-wide wavefronts)
 #pragma unroll
   for (int k = 0; k < NACC; ++k) {
     double v = acc[k];
@@ -47,207 +55,326 @@ __device__ __forceinline__ void block_sum(double (&acc)[NACC], double* sh, int t
   }
 }
 
+// Per-workgroup partials + last-arriving-workgroup finalisation for one field.
+// `wg_in_field` workgroups contribute to field f; `slot` indexes partials.
+template <class Op, int NTHR>
+__device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::NACC], int tid,
+                                                int f, long slot, int wg_in_field,
+                                                long first_slot) {
+  constexpr int NA = Op::NACC;
+  __shared__ double sh[16 * NA];
+  __shared__ int last;
+  block_sum<NA>(acc, sh, tid, NTHR);
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) op.partials[slot * NA + k] = acc[k];
+    __threadfence();
+    const unsigned prev = atomicAdd(op.counters + f, 1u);
+    last = (prev == unsigned(wg_in_field - 1));
+  }
+  __syncthreads();
+  if (last) {
+    __threadfence();
+    double s[NA];
+#pragma unroll
+    for (int k = 0; k < NA; ++k) s[k] = 0.0;
+    for (int w = tid; w < wg_in_field; w += NTHR) {
+#pragma unroll
+      for (int k = 0; k < NA; ++k) s[k] += op.partials[(first_slot + w) * NA + k];
+    }
+    block_sum<NA>(s, sh, tid, NTHR);
+    if (tid == 0) {
+      op.finalize(f, s);
+      op.counters[f] = 0u;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Row pass.  Lines are rows (contiguous); `lines_per_field` = ny.
+// Row pass geometry: G groups of N1 rows, T_LINE = NX/P threads per row.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int N, int MODE, bool INV1, bool INV2, class Op>
-__global__ void __launch_bounds__((RowCfg<T, N>::WG))
+template <typename T, int NX, int N1>
+struct RowCfg {
+  static constexpr int P = PChoice<NX>::P;
+  static constexpr int T_LINE = NX / P;
+  static constexpr int G = cmax(1, 256 / (T_LINE * N1));
+  static constexpr int LINES = G * N1;
+  static constexpr int WG = LINES * T_LINE;
+  static constexpr int LSTRIDE = NX + NX / 16 + 1;  // padded line stride in LDS
+  static constexpr bool NEED_LDS = LinePlan<NX, P>::NST > 1 || N1 > 1;
+  static constexpr int LDS_ELEMS = NEED_LDS ? LINES * LSTRIDE : 1;
+  static_assert(WG <= 1024, "row workgroup too large");
+};
+
+// Steps 1+2 of the four-step column factorisation across the N1 rows of each
+// group (forward: DFT then twiddle; inverse: conjugate twiddle then inverse
+// DFT scaled by 1/N1).  Registers hold positions posf(i) of line `l`.
+template <typename T, int NX, int P, int N1, bool INV, class PosF>
+__device__ __forceinline__ void cross_rows(T (&vr)[P], T (&vi)[P], int l, int t, const PosF& posf,
+                                           T* lre, T* lim, int n2, const T* __restrict__ twyr,
+                                           const T* __restrict__ twyi) {
+  if constexpr (N1 > 1) {
+    constexpr int T_LINE = NX / P;
+    constexpr int LS = NX + NX / 16 + 1;
+    const int g = l / N1;
+    const int n1 = l - g * N1;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const int pos = posf(i);
+      const int a = l * LS + pos + (pos >> 4);
+      lre[a] = vr[i];
+      lim[a] = vi[i];
+    }
+    __syncthreads();
+    // every thread of the group transforms P/N1 positions across the rows
+    const int j = n1 * T_LINE + t;
+    T wr[N1], wi[N1];
+    wr[0] = T(1);
+    wi[0] = T(0);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      wr[k1] = twyr[n2 * k1];
+      wi[k1] = INV ? -twyi[n2 * k1] : twyi[n2 * k1];
+    }
+#pragma unroll
+    for (int m = 0; m < P / N1; ++m) {
+      const int x = j + m * (N1 * T_LINE);
+      const int xo = x + (x >> 4);
+      T ur[N1], ui[N1];
+#pragma unroll
+      for (int r = 0; r < N1; ++r) {
+        ur[r] = lre[(g * N1 + r) * LS + xo];
+        ui[r] = lim[(g * N1 + r) * LS + xo];
+      }
+      if constexpr (INV) {
+#pragma unroll
+        for (int r = 1; r < N1; ++r) {
+          const T a = ur[r] * wr[r] - ui[r] * wi[r];
+          ui[r] = ur[r] * wi[r] + ui[r] * wr[r];
+          ur[r] = a;
+        }
+        dft<N1, true>(ur, ui);
+        constexpr T s = T(1) / T(N1);
+#pragma unroll
+        for (int r = 0; r < N1; ++r) {
+          ur[r] *= s;
+          ui[r] *= s;
+        }
+      } else {
+        dft<N1, false>(ur, ui);
+#pragma unroll
+        for (int r = 1; r < N1; ++r) {
+          const T a = ur[r] * wr[r] - ui[r] * wi[r];
+          ui[r] = ur[r] * wi[r] + ui[r] * wr[r];
+          ur[r] = a;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < N1; ++r) {
+        lre[(g * N1 + r) * LS + xo] = ur[r];
+        lim[(g * N1 + r) * LS + xo] = ui[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const int pos = posf(i);
+      const int a = l * LS + pos + (pos >> 4);
+      vr[i] = lre[a];
+      vi[i] = lim[a];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row pass.  Group gi = (field f, n2) owns rows y = n2 + N2*n1, n1 < N1.
+// XIN: the input's column axis is in four-step order (apply the inverse
+// cross-row step first); XOUT: leave the output in four-step order.
+// ---------------------------------------------------------------------------
+template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
+          bool XOUT, class Op>
+__global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
     k_row_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
-               S* __restrict__ oim, int lines_per_field, long total_lines,
-               const T* __restrict__ twr, const T* __restrict__ twi) {
-  using CF = RowCfg<T, N>;
+               S* __restrict__ oim, int ny, long total_groups, const T* __restrict__ twr,
+               const T* __restrict__ twi, const T* __restrict__ twyr, const T* __restrict__ twyi) {
+  using CF = RowCfg<T, NX, N1>;
   constexpr int P = CF::P;
-  using PL = LinePlan<N, P>;
+  using PL = LinePlan<NX, P>;
   __shared__ T lre[CF::LDS_ELEMS];
   __shared__ T lim[CF::LDS_ELEMS];
+  const int N2 = ny / N1;
   const int tid = threadIdx.x;
-  const int lw = tid / CF::T_LINE;
-  const int t = tid - lw * CF::T_LINE;
-  const long line = (long)blockIdx.x * CF::LINES + lw;
-  const bool active = line < total_lines;
-  const int f = active ? int(line / lines_per_field) : 0;
-  const int y = active ? int(line - (long)f * lines_per_field) : 0;
-  const long base = line * (long)N;
+  const int l = tid / CF::T_LINE;
+  const int t = tid - l * CF::T_LINE;
+  const int g = l / N1;
+  const int n1 = l - g * N1;
+  const long gi = (long)blockIdx.x * CF::G + g;
+  const bool active = gi < total_groups;
+  const int f = active ? int(gi / N2) : 0;
+  const int n2 = active ? int(gi - (long)f * N2) : 0;
+  const int y = n2 + N2 * n1;
+  const long base = ((long)f * ny + y) * (long)NX;
+  constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
   T vr[P], vi[P];
-  double acc[Op::NACC > 0 ? Op::NACC : 1];
+  double acc[NA];
 #pragma unroll
-  for (int k = 0; k < (Op::NACC > 0 ? Op::NACC : 1); ++k) acc[k] = 0.0;
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const int pos = PL::template in_pos<false>(t, i);
       vr[i] = ld<T>(ire + base + pos);
       vi[i] = ld<T>(iim + base + pos);
-      op.pre(f, pos, y, vr[i], vi[i], acc);
     }
   } else {
 #pragma unroll
     for (int i = 0; i < P; ++i) vr[i] = vi[i] = T(0);
   }
-  const RowLIdx li{lw * CF::LSTRIDE};
-  fft_line<T, N, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
-  constexpr T invN = T(1) / T(N);
+  if constexpr (XIN)
+    cross_rows<T, NX, P, N1, true>(
+        vr, vi, l, t, [&](int i) { return PL::template in_pos<false>(t, i); }, lre, lim, n2, twyr,
+        twyi);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) op.pre(f, PL::template in_pos<false>(t, i), y, vr[i], vi[i], acc);
+  }
+  const RowLIdx li{l * CF::LSTRIDE};
+  fft_line<T, NX, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
+  constexpr T invN = T(1) / T(NX);
+  constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<false>(t, i);
         if constexpr (INV1) {
           vr[i] *= invN;
           vi[i] *= invN;
         }
-        op.mid(f, pos, y, vr[i], vi[i], acc);
+        op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
       }
     }
-    fft_line<T, N, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
-    if (active) {
+    fft_line<T, NX, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
+  }
+  constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
+  if (active) {
 #pragma unroll
-      for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<true>(t, i);
-        if constexpr (INV2) {
-          vr[i] *= invN;
-          vi[i] *= invN;
-        }
-        op.post(f, pos, y, vr[i], vi[i], acc);
-        st(ore + base + pos, vr[i]);
-        st(oim + base + pos, vi[i]);
+    for (int i = 0; i < P; ++i) {
+      if constexpr (INV_LAST) {
+        vr[i] *= invN;
+        vi[i] *= invN;
       }
-    }
-  } else {
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<false>(t, i);
-        if constexpr (INV1) {
-          vr[i] *= invN;
-          vi[i] *= invN;
-        }
-        op.post(f, pos, y, vr[i], vi[i], acc);
-        st(ore + base + pos, vr[i]);
-        st(oim + base + pos, vi[i]);
-      }
+      op.post(f, PL::template out_pos<REV_OUT>(t, i), y, vr[i], vi[i], acc);
     }
   }
-  static_assert(Op::NACC == 0, "row passes carry no reductions");
+  if constexpr (XOUT)
+    cross_rows<T, NX, P, N1, false>(
+        vr, vi, l, t, [&](int i) { return PL::template out_pos<REV_OUT>(t, i); }, lre, lim, n2,
+        twyr, twyi);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const int pos = PL::template out_pos<REV_OUT>(t, i);
+      st(ore + base + pos, vr[i]);
+      st(oim + base + pos, vi[i]);
+    }
+  }
+  if constexpr (Op::NACC > 0) {
+    // a workgroup's groups never span two fields (the launcher checks G | N2)
+    const int wg_in_field = N2 / CF::G;
+    reduce_finalize<Op, CF::WG>(op, acc, tid, f, blockIdx.x, wg_in_field, (long)f * wg_in_field);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Column pass.  A workgroup owns C adjacent columns of one field (all ny rows)
-// — `tiles` = ceil(nx / C) per field.
+// Column pass over N2-point lines.  A workgroup owns C adjacent columns of
+// block k1 of one field.  Row of line position p:
+//   forward (MODE 0, !INV1): load N2*k1 + p (four-step), store k1 + N1*p
+//   inverse (MODE 0, INV1):  load k1 + N1*p (natural),   store N2*k1 + p
+//   fused   (MODE 1):        load and store N2*k1 + p, natural index in mid
+// Hooks see the natural row index y = k1 + N1*p.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int N, int MODE, bool INV1, bool INV2, class Op>
-__global__ void __launch_bounds__((ColCfg<T, N>::WG))
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op>
+__global__ void __launch_bounds__((ColCfg<T, N2>::WG))
     k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
-               S* __restrict__ oim, int nx, int tiles, const T* __restrict__ twr,
+               S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,
                const T* __restrict__ twi) {
-  using CF = ColCfg<T, N>;
+  using CF = ColCfg<T, N2>;
   constexpr int P = CF::P;
   constexpr int C = CF::C;
-  using PL = LinePlan<N, P>;
+  using PL = LinePlan<N2, P>;
   __shared__ T lre[CF::LDS_ELEMS];
   __shared__ T lim[CF::LDS_ELEMS];
   const int tid = threadIdx.x;
   const int col = tid % C;
   const int t = tid / C;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int f = lb / tiles;
-  const int tile = lb - f * tiles;
+  const int per_field = tiles * n1cnt;
+  const int f = lb / per_field;
+  const int rem = lb - f * per_field;
+  const int k1 = rem / tiles;
+  const int tile = rem - k1 * tiles;
   const int x = tile * C + col;
   const bool active = x < nx;
-  const long fbase = (long)f * nx * N;
+  const int ny = N2 * n1cnt;
+  const long fbase = (long)f * nx * ny;
+  constexpr bool NAT_IN = MODE == 0 && INV1;
+  constexpr bool NAT_OUT = MODE == 0 && !INV1;
+  constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
   T vr[P], vi[P];
-  double acc[Op::NACC > 0 ? Op::NACC : 1];
+  double acc[NA];
 #pragma unroll
-  for (int k = 0; k < (Op::NACC > 0 ? Op::NACC : 1); ++k) acc[k] = 0.0;
+  for (int k = 0; k < NA; ++k) acc[k] = 0.0;
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-      const int pos = PL::template in_pos<false>(t, i);
-      const long a = fbase + (long)pos * nx + x;
+      const int p = PL::template in_pos<false>(t, i);
+      const int row = NAT_IN ? k1 + n1cnt * p : N2 * k1 + p;
+      const long a = fbase + (long)row * nx + x;
       vr[i] = ld<T>(ire + a);
       vi[i] = ld<T>(iim + a);
-      op.pre(f, x, pos, vr[i], vi[i], acc);
+      op.pre(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
     }
   } else {
 #pragma unroll
     for (int i = 0; i < P; ++i) vr[i] = vi[i] = T(0);
   }
   const ColLIdx<C> li{col};
-  fft_line<T, N, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
-  constexpr T invN = T(1) / T(N);
+  fft_line<T, N2, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
+  constexpr T invN = T(1) / T(N2);
+  constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<false>(t, i);
+        const int p = PL::template out_pos<false>(t, i);
         if constexpr (INV1) {
           vr[i] *= invN;
           vi[i] *= invN;
         }
-        op.mid(f, x, pos, vr[i], vi[i], acc);
+        op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
       }
     }
-    fft_line<T, N, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
-    if (active) {
+    fft_line<T, N2, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
+  }
+  constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
+  if (active) {
 #pragma unroll
-      for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<true>(t, i);
-        if constexpr (INV2) {
-          vr[i] *= invN;
-          vi[i] *= invN;
-        }
-        op.post(f, x, pos, vr[i], vi[i], acc);
-        const long a = fbase + (long)pos * nx + x;
-        st(ore + a, vr[i]);
-        st(oim + a, vi[i]);
+    for (int i = 0; i < P; ++i) {
+      const int p = PL::template out_pos<REV_OUT>(t, i);
+      if constexpr (INV_LAST) {
+        vr[i] *= invN;
+        vi[i] *= invN;
       }
-    }
-  } else {
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        const int pos = PL::template out_pos<false>(t, i);
-        if constexpr (INV1) {
-          vr[i] *= invN;
-          vi[i] *= invN;
-        }
-        op.post(f, x, pos, vr[i], vi[i], acc);
-        const long a = fbase + (long)pos * nx + x;
-        st(ore + a, vr[i]);
-        st(oim + a, vi[i]);
-      }
+      op.post(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
+      const int row = NAT_OUT ? k1 + n1cnt * p : N2 * k1 + p;
+      const long a = fbase + (long)row * nx + x;
+      st(ore + a, vr[i]);
+      st(oim + a, vi[i]);
     }
   }
   if constexpr (Op::NACC > 0) {
-    constexpr int NA = Op::NACC;
-    __shared__ double sh[16 * NA];
-    __shared__ int last;
-    block_sum<NA>(acc, sh, tid, CF::WG);
-    if (tid == 0) {
-#pragma unroll
-      for (int k = 0; k < NA; ++k) op.partials[(long)lb * NA + k] = acc[k];
-      __threadfence();
-      const unsigned prev = atomicAdd(op.counters + f, 1u);
-      last = (prev == unsigned(tiles - 1));
-    }
-    __syncthreads();
-    if (last) {
-      __threadfence();
-      // fold this field's tiles in a fixed order: thread k sums component k
-      // over tile strides, then a fixed tree over strides.
-      double s[NA];
-#pragma unroll
-      for (int k = 0; k < NA; ++k) s[k] = 0.0;
-      for (int tl = tid; tl < tiles; tl += CF::WG) {
-#pragma unroll
-        for (int k = 0; k < NA; ++k) s[k] += op.partials[(long)(f * tiles + tl) * NA + k];
-      }
-      block_sum<NA>(s, sh, tid, CF::WG);
-      if (tid == 0) {
-        op.finalize(f, s);
-        op.counters[f] = 0u;
-      }
-    }
+    reduce_finalize<Op, CF::WG>(op, acc, tid, f, lb, per_field, (long)f * per_field);
   }
 }
 

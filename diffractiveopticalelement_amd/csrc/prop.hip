@@ -38,61 +38,59 @@ static AsmField asm_field_params(int nx, int ny, double dx, double dy, double la
 
 template <typename T, typename S>
 int run_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, const OpAsmH& op) {
-  Twiddle twx, twy;
-  int rc = get_twiddle(ctx, in->nx, sizeof(T), &twx);
-  if (rc) return rc;
-  rc = get_twiddle(ctx, in->ny, sizeof(T), &twy);
+  Plan2 pl;
+  int rc = make_plan2(ctx, in->nx, in->ny, sizeof(T), col_split<T>(in->nx, in->ny), &pl);
   if (rc) return rc;
   const S* ire = (const S*)in->re;
   const S* iim = (const S*)in->im;
   S* re = (S*)out->re;
   S* im = (S*)out->im;
-  hipError_t e = hipSuccess;
   const OpNone none{};
-  with_n(in->nx, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    e = launch_row<T, S, N, 0, false, false>(ctx, "asm_row_fwd", none, ire, iim, re, im, in->ny, in->batch, twx);
-  });
-  DOE_HIP(e);
-  with_n(in->ny, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    e = launch_col<T, S, N, 1, false, true>(ctx, "asm_col_h", op, re, im, re, im, in->nx, in->batch, twy);
-  });
-  DOE_HIP(e);
-  with_n(in->nx, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    e = launch_row<T, S, N, 0, true, false>(ctx, "asm_row_inv", none, re, im, re, im, in->ny, in->batch, twx);
-  });
-  DOE_HIP(e);
+  // rows forward (+ column split) -> column blocks F . H . F^-1 -> rows inverse
+  DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "asm_row_fwd", none, ire, iim, re, im,
+                                                         in->batch, pl, AllSizes{})));
+  DOE_HIP((col_pass<T, S, 1, false, true>(ctx, "asm_col_h", op, re, im, re, im, in->batch, pl,
+                                          AllSizes{})));
+  DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "asm_row_inv", none, re, im, re, im,
+                                                        in->batch, pl, AllSizes{})));
   return DOE_OK;
 }
 
-// Two-pass chirp transform: row FFT with a pre hook, column FFT with a post
-// hook.  INV selects the inverse DFT for both passes.
+// Two-pass chirp transforms.  Forward (Fresnel forward, lens): row pass with a
+// pre hook, then the column blocks with a post hook.  Inverse (Fresnel
+// inverse): column blocks with a pre hook on the natural-order input, then
+// the row pass with a post hook.
 template <typename T, typename S, bool INV, int PRE, int POST>
 static int run_chirp2(doe_ctx* ctx, const doe_field* in, doe_field* out, const ChirpField* dfp) {
-  Twiddle twx, twy;
-  int rc = get_twiddle(ctx, in->nx, sizeof(T), &twx);
-  if (rc) return rc;
-  rc = get_twiddle(ctx, in->ny, sizeof(T), &twy);
+  Plan2 pl;
+  int rc = make_plan2(ctx, in->nx, in->ny, sizeof(T), col_split<T>(in->nx, in->ny), &pl);
   if (rc) return rc;
   const S* ire = (const S*)in->re;
   const S* iim = (const S*)in->im;
   S* re = (S*)out->re;
   S* im = (S*)out->im;
-  hipError_t e = hipSuccess;
-  const OpChirp<PRE, 0> op_row{dfp, in->nx, in->ny};
-  const OpChirp<0, POST> op_col{dfp, in->nx, in->ny};
-  with_n(in->nx, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    e = launch_row<T, S, N, 0, INV, false>(ctx, "chirp_row", op_row, ire, iim, re, im, in->ny, in->batch, twx);
-  });
-  DOE_HIP(e);
-  with_n(in->ny, [&](auto NN) {
-    constexpr int N = decltype(NN)::value;
-    e = launch_col<T, S, N, 0, INV, false>(ctx, "chirp_col", op_col, re, im, re, im, in->nx, in->batch, twy);
-  });
-  DOE_HIP(e);
+  // the split column pass permutes rows: route it through the work planes
+  S* wre = re;
+  S* wim = im;
+  if (pl.n1 > 1) {
+    rc = ensure_work(ctx, in->count() * sizeof(S));
+    if (rc) return rc;
+    wre = (S*)ctx->work_re;
+    wim = (S*)ctx->work_im;
+  }
+  const OpChirp<PRE, 0> op_first{dfp, in->nx, in->ny};
+  const OpChirp<0, POST> op_last{dfp, in->nx, in->ny};
+  if constexpr (!INV) {
+    DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "chirp_row", op_first, ire, iim, wre,
+                                                           wim, in->batch, pl, AllSizes{})));
+    DOE_HIP((col_pass<T, S, 0, false, false>(ctx, "chirp_col", op_last, wre, wim, re, im, in->batch,
+                                             pl, AllSizes{})));
+  } else {
+    DOE_HIP((col_pass<T, S, 0, true, false>(ctx, "chirp_col", op_first, ire, iim, wre, wim,
+                                            in->batch, pl, AllSizes{})));
+    DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "chirp_row", op_last, wre, wim, re, im,
+                                                          in->batch, pl, AllSizes{})));
+  }
   return DOE_OK;
 }
 
