@@ -1,5 +1,6 @@
 // C-ABI entry points: context, fields, conversions, fft2d.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -326,7 +327,21 @@ int doe_ctx_create(int device, void* stream, doe_ctx** out) {
     }
     ctx->own_stream = true;
   }
+  if (const char* env = std::getenv("DOE_CHUNK_MIB")) ctx->chunk_bytes = std::atol(env) << 20;
   *out = ctx;
+  return DOE_OK;
+}
+
+int doe_ctx_set_chunk_bytes(doe_ctx* ctx, long bytes) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  DOE_REQUIRE(bytes >= 0, DOE_ERR_INVALID, "chunk size must be >= 0");
+  ctx->chunk_bytes = bytes;
+  return DOE_OK;
+}
+
+int doe_ctx_get_chunk_bytes(doe_ctx* ctx, long* bytes) {
+  DOE_REQUIRE(ctx && bytes, DOE_ERR_INVALID, "null argument");
+  *bytes = ctx->chunk_bytes;
   return DOE_OK;
 }
 
@@ -378,7 +393,12 @@ int doe_ctx_set_flags(doe_ctx* ctx, int flags) {
 int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap) {
   DOE_REQUIRE(ctx && buf && cap > 0, DOE_ERR_INVALID, "null argument");
   DOE_HIP(hipStreamSynchronize(ctx->stream));
-  std::map<std::string, std::pair<long, double>> agg;
+  struct Agg {
+    long n = 0;
+    double ms = 0;
+    long pts = 0;
+  };
+  std::map<std::string, Agg> agg;
   std::vector<std::string> order;
   for (auto& r : ctx->prof) {
     float ms = 0.f;
@@ -386,10 +406,11 @@ int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap) {
     auto it = agg.find(r.tag);
     if (it == agg.end()) {
       order.push_back(r.tag);
-      agg[r.tag] = {1, ms};
+      agg[r.tag] = Agg{1, ms, r.pts};
     } else {
-      it->second.first += 1;
-      it->second.second += ms;
+      it->second.n += 1;
+      it->second.ms += ms;
+      it->second.pts += r.pts;
     }
     ctx->ev_pool.push_back(r.a);
     ctx->ev_pool.push_back(r.b);
@@ -398,7 +419,7 @@ int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap) {
   std::string out;
   for (auto& k : order) {
     char line[256];
-    snprintf(line, sizeof line, "%s %ld %.6f\n", k.c_str(), agg[k].first, agg[k].second);
+    snprintf(line, sizeof line, "%s %ld %.6f %ld\n", k.c_str(), agg[k].n, agg[k].ms, agg[k].pts);
     out += line;
   }
   DOE_REQUIRE((long)out.size() < cap, DOE_ERR_INVALID, "report buffer too small");

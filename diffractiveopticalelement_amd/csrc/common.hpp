@@ -90,4 +90,28 @@ constexpr double c16(int m) {
 }
 constexpr double s16(int m) { return c16(m - 4); }  // sin(x) = cos(x - pi/2)
 
+// cos(k*pi/16), k = 0..8
+constexpr double cos_pi16(int k) {
+  switch (k) {
+    case 0: return 1.0;
+    case 1: return 0.98078528040323044913;
+    case 2: return 0.92387953251128675613;
+    case 3: return 0.83146961230254523708;
+    case 4: return 0.70710678118654752440;
+    case 5: return 0.55557023301960222474;
+    case 6: return 0.38268343236508977173;
+    case 7: return 0.19509032201612826785;
+    default: return 0.0;
+  }
+}
+// cos(2*pi*m/32) and sin(2*pi*m/32) for any integer m
+constexpr double c32(int m) {
+  m &= 31;
+  if (m <= 8) return cos_pi16(m);
+  if (m <= 16) return -cos_pi16(16 - m);
+  if (m <= 24) return -cos_pi16(m - 16);
+  return cos_pi16(32 - m);
+}
+constexpr double s32(int m) { return c32(m - 8); }
+
 }  // namespace doe

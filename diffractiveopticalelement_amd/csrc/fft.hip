@@ -2,6 +2,8 @@
 #include "ctx.hpp"
 #include "ops_prop.hpp"
 
+#include <algorithm>
+
 namespace doe {
 
 template <typename T, typename S>
@@ -9,36 +11,39 @@ int run_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
   Plan2 pl;
   int rc = make_plan2(ctx, f->nx, f->ny, sizeof(T), col_split<T>(f->nx, f->ny), &pl);
   if (rc) return rc;
-  S* re = (S*)f->re;
-  S* im = (S*)f->im;
+  const size_t plane = size_t(f->nx) * size_t(f->ny);
+  const int cf = chunk_fields(ctx, plane * 2 * sizeof(S), f->batch);
   // A split column pass reads rows in one order and writes them in another,
-  // so it cannot run in place: the two passes ping-pong through the work
-  // planes (same HBM traffic as in place).
-  S* wre = re;
-  S* wim = im;
+  // so it cannot run in place: the two passes ping-pong through work planes
+  // of one chunk (same HBM traffic as in place; the chunk's work planes are
+  // re-read from the memory-side cache).
+  S* wre = nullptr;
+  S* wim = nullptr;
   if (pl.n1 > 1) {
-    rc = ensure_work(ctx, f->count() * sizeof(S));
+    rc = ensure_work(ctx, size_t(cf) * plane * sizeof(S));
     if (rc) return rc;
     wre = (S*)ctx->work_re;
     wim = (S*)ctx->work_im;
   }
   const OpNone op{};
-  hipError_t e;
-  if (!inverse) {
-    // rows (+ the column split's first steps), then the column blocks
-    e = row_pass<T, S, 0, false, false, false, true>(ctx, "fft_row_fwd", op, re, im, wre, wim,
-                                                      f->batch, pl, AllSizes{});
-    DOE_HIP(e);
-    e = col_pass<T, S, 0, false, false>(ctx, "fft_col_fwd", op, wre, wim, re, im, f->batch, pl,
-                                        AllSizes{});
-    DOE_HIP(e);
-  } else {
-    e = col_pass<T, S, 0, true, false>(ctx, "fft_col_inv", op, re, im, wre, wim, f->batch, pl,
-                                       AllSizes{});
-    DOE_HIP(e);
-    e = row_pass<T, S, 0, true, false, true, false>(ctx, "fft_row_inv", op, wre, wim, re, im,
-                                                     f->batch, pl, AllSizes{});
-    DOE_HIP(e);
+  for (int f0 = 0; f0 < f->batch; f0 += cf) {
+    const int nb = std::min(cf, f->batch - f0);
+    S* re = (S*)f->re + size_t(f0) * plane;
+    S* im = (S*)f->im + size_t(f0) * plane;
+    S* xre = pl.n1 > 1 ? wre : re;
+    S* xim = pl.n1 > 1 ? wim : im;
+    if (!inverse) {
+      // rows (+ the column split's first steps), then the column blocks
+      DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "fft_row_fwd", op, re, im, xre,
+                                                             xim, nb, pl, AllSizes{})));
+      DOE_HIP((col_pass<T, S, 0, false, false>(ctx, "fft_col_fwd", op, xre, xim, re, im, nb, pl,
+                                               AllSizes{})));
+    } else {
+      DOE_HIP((col_pass<T, S, 0, true, false>(ctx, "fft_col_inv", op, re, im, xre, xim, nb, pl,
+                                              AllSizes{})));
+      DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "fft_row_inv", op, xre, xim, re,
+                                                            im, nb, pl, AllSizes{})));
+    }
   }
   return DOE_OK;
 }

@@ -20,19 +20,19 @@
 
 namespace doe {
 
-// Multiply (xr + i xi) by exp(-/+ 2*pi*i*m/16): minus for the forward
-// transform, plus for the inverse.
+// Multiply (xr + i xi) by exp(-/+ 2*pi*i*m/32): minus for the forward
+// transform, plus for the inverse.  Multiples of a quarter turn are exact.
 template <int M, bool INV, typename T>
-__device__ __forceinline__ void mul_w16(T& xr, T& xi) {
-  constexpr int m = M & 15;
+__device__ __forceinline__ void mul_w32(T& xr, T& xi) {
+  constexpr int m = M & 31;
   if constexpr (m == 0) {
     return;
-  } else if constexpr (m == 8) {
+  } else if constexpr (m == 16) {
     xr = -xr;
     xi = -xi;
-  } else if constexpr (m == 4 || m == 12) {
-    // exp(-i*pi/2) = -i (forward, m=4); the other three cases follow.
-    constexpr bool times_minus_i = (m == 4) != INV;
+  } else if constexpr (m == 8 || m == 24) {
+    // exp(-i*pi/2) = -i (forward, m=8); the other three cases follow.
+    constexpr bool times_minus_i = (m == 8) != INV;
     const T r = xr;
     if constexpr (times_minus_i) {
       xr = xi;
@@ -42,15 +42,15 @@ __device__ __forceinline__ void mul_w16(T& xr, T& xi) {
       xi = r;
     }
   } else {
-    constexpr T c = T(c16(m));
-    constexpr T s = INV ? T(s16(m)) : T(-s16(m));
+    constexpr T c = T(c32(m));
+    constexpr T s = INV ? T(s32(m)) : T(-s32(m));
     const T r = xr * c - xi * s;
     xi = xr * s + xi * c;
     xr = r;
   }
 }
 
-// In-register radix-R DFT (R <= 16), natural order in and out.
+// In-register radix-R DFT (R <= 32), natural order in and out.
 template <int R, bool INV, typename T>
 __device__ __forceinline__ void dft(T* vr, T* vi) {
   if constexpr (R > 1) {
@@ -70,7 +70,7 @@ __device__ __forceinline__ void dft(T* vr, T* vi) {
         constexpr int p = (b / half) * len + k;
         constexpr int q = p + half;
         T tr = ar[q], ti = ai[q];
-        mul_w16<k * (16 / len), INV>(tr, ti);
+        mul_w32<k * (32 / len), INV>(tr, ti);
         ar[q] = ar[p] - tr;
         ai[q] = ai[p] - ti;
         ar[p] = ar[p] + tr;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void dft(T* vr, T* vi) {
 template <int N, int P>
 struct LinePlan {
   static_assert((N & (N - 1)) == 0 && N >= 2, "N must be a power of two");
-  static_assert(P <= N && P <= 16 && (P & (P - 1)) == 0, "bad P");
+  static_assert(P <= N && P <= 32 && (P & (P - 1)) == 0, "bad P");
   static constexpr int T = N / P;
   static constexpr int LOGN = ilog2(N);
   static constexpr int LOGP = ilog2(P);
@@ -123,14 +123,21 @@ struct LinePlan {
   }
 };
 
-// One full line FFT.  `lds_re/lds_im` point at this line's exchange area and
-// `lidx(pos)` maps a line position to an LDS index.  Every thread of the
-// workgroup must call this (it contains barriers), active or not.
+// One full line FFT with a per-stage thread geometry.  `geo` says, for each
+// stage s, which line position index t the thread works on (geo.t<s>()),
+// where the exchange before stage s writes stage s-1's outputs
+// (geo.widx<s>(pos)) and reads stage s's inputs (geo.ridx<s>(pos)), and gets
+// a hook after that read (geo.after_read<s>(vr, vi)).  A thread may change
+// line or position between stages; every (line, position) must be written
+// and read exactly once per exchange.  Every thread of the workgroup must
+// call this (it contains barriers), active or not.
 // Twiddle table: tw[m] = exp(-2*pi*i*m/N), m in [0, N), planar re/im.
-template <typename T, int N, int P, bool INV, bool REV, class LIdx>
-__device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_re, T* lds_im,
-                                         const LIdx& lidx, const T* __restrict__ twr,
-                                         const T* __restrict__ twi) {
+// PLANES = 2: separate LDS buffers for re and im; PLANES = 1: one buffer,
+// the two planes exchanged one after the other (half the LDS, two more
+// barriers per exchange).
+template <typename T, int N, int P, bool INV, bool REV, int PLANES = 2, class Geo>
+__device__ __forceinline__ void fft_line_g(T (&vr)[P], T (&vi)[P], Geo& geo, T* lds_re, T* lds_im,
+                                           const T* __restrict__ twr, const T* __restrict__ twi) {
   using PL = LinePlan<N, P>;
   constexpr int TT = PL::T;
   static_for<0, PL::NST>([&](auto SI) {
@@ -138,33 +145,48 @@ __device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_r
     constexpr int R = PL::template radix<REV>(s);
     constexpr int NS = PL::template ns<REV>(s);
     constexpr int NB = P / R;
+    const int t = geo.template t<s>();
     if constexpr (s > 0) {
       constexpr int RP = PL::template radix<REV>(s - 1);
       constexpr int NSP = PL::template ns<REV>(s - 1);
       constexpr int NBP = P / RP;
-      __syncthreads();
-      static_for<0, NBP>([&](auto BI) {
-        constexpr int b = decltype(BI)::value;
-        const int j = t + b * TT;
-        const int base = (j / NSP) * NSP * RP + (j % NSP);
-        static_for<0, RP>([&](auto QI) {
-          constexpr int q = decltype(QI)::value;
-          const int li = lidx(base + q * NSP);
-          lds_re[li] = vr[b * RP + q];
-          lds_im[li] = vi[b * RP + q];
+      const int tp = geo.template t<s - 1>();
+      // one plane (or both) through LDS: write stage s-1's outputs, read
+      // stage s's inputs
+      auto exchange = [&](auto wr_c, auto wi_c) {
+        constexpr bool WRE = decltype(wr_c)::value;
+        constexpr bool WIM = decltype(wi_c)::value;
+        __syncthreads();
+        static_for<0, NBP>([&](auto BI) {
+          constexpr int b = decltype(BI)::value;
+          const int j = tp + b * TT;
+          const int base = (j / NSP) * NSP * RP + (j % NSP);
+          static_for<0, RP>([&](auto QI) {
+            constexpr int q = decltype(QI)::value;
+            const int li = geo.template widx<s>(base + q * NSP);
+            if constexpr (WRE) lds_re[li] = vr[b * RP + q];
+            if constexpr (WIM) (PLANES == 1 ? lds_re : lds_im)[li] = vi[b * RP + q];
+          });
         });
-      });
-      __syncthreads();
-      static_for<0, NB>([&](auto BI) {
-        constexpr int b = decltype(BI)::value;
-        const int j = t + b * TT;
-        static_for<0, R>([&](auto RI) {
-          constexpr int r = decltype(RI)::value;
-          const int li = lidx(j + r * (N / R));
-          vr[b * R + r] = lds_re[li];
-          vi[b * R + r] = lds_im[li];
+        __syncthreads();
+        static_for<0, NB>([&](auto BI) {
+          constexpr int b = decltype(BI)::value;
+          const int j = t + b * TT;
+          static_for<0, R>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
+            const int li = geo.template ridx<s>(j + r * (N / R));
+            if constexpr (WRE) vr[b * R + r] = lds_re[li];
+            if constexpr (WIM) vi[b * R + r] = (PLANES == 1 ? lds_re : lds_im)[li];
+          });
         });
-      });
+      };
+      if constexpr (PLANES == 2) {
+        exchange(std::true_type{}, std::true_type{});
+      } else {
+        exchange(std::true_type{}, std::false_type{});
+        exchange(std::false_type{}, std::true_type{});
+      }
+      geo.template after_read<s>(vr, vi);
     }
     static_for<0, NB>([&](auto BI) {
       constexpr int b = decltype(BI)::value;
@@ -193,6 +215,29 @@ __device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_r
   });
 }
 
+// Fixed geometry: one line, one position index for every stage.
+template <class LIdx>
+struct FixedGeo {
+  int tt;
+  LIdx lidx;
+  template <int S>
+  __device__ __forceinline__ int t() const { return tt; }
+  template <int S>
+  __device__ __forceinline__ int widx(int pos) const { return lidx(pos); }
+  template <int S>
+  __device__ __forceinline__ int ridx(int pos) const { return lidx(pos); }
+  template <int S, typename T, int P>
+  __device__ __forceinline__ void after_read(T (&)[P], T (&)[P]) const {}
+};
+
+template <typename T, int N, int P, bool INV, bool REV, int PLANES = 2, class LIdx>
+__device__ __forceinline__ void fft_line(T (&vr)[P], T (&vi)[P], int t, T* lds_re, T* lds_im,
+                                         const LIdx& lidx, const T* __restrict__ twr,
+                                         const T* __restrict__ twi) {
+  FixedGeo<LIdx> geo{t, lidx};
+  fft_line_g<T, N, P, INV, REV, PLANES>(vr, vi, geo, lds_re, lds_im, twr, twi);
+}
+
 // ---------------------------------------------------------------------------
 // Launch geometry (the row pass geometry lives with the row kernel).
 // Columns: C adjacent columns per workgroup, whole line resident; the tile
@@ -203,15 +248,24 @@ struct PChoice {
   static constexpr int P = N < 16 ? N : 16;
 };
 
-template <typename T, int N>
+template <typename T, int N, bool WIDE_OK = false>
 struct ColCfg {
-  static constexpr int P = PChoice<N>::P;
+  // FP32 lines of 512 points and more, for passes whose hooks are light
+  // (WIDE_OK): 32 points per thread and the planes exchanged one at a time,
+  // so a tile is 64 columns (256-byte row segments) at N = 512 and 32
+  // columns at N = 1024 within one 128 KB LDS buffer.  Heavier hooks would
+  // spill at 32 points per thread.
+  static constexpr bool WIDE = WIDE_OK && sizeof(T) == 4 && N >= 512;
+  static constexpr int P = WIDE ? 32 : PChoice<N>::P;
   static constexpr int T_LINE = N / P;
-  static constexpr int BUDGET = sizeof(T) == 4 ? 16384 : 8192;  // complex values in LDS
+  static constexpr int PLANES = WIDE ? 1 : 2;
+  // values per LDS buffer
+  static constexpr int BUDGET = sizeof(T) == 4 ? (PLANES == 1 ? 32768 : 16384) : 8192;
   static constexpr int C = cmax(1, cmin(64, cmin(BUDGET / N, 1024 / T_LINE)));
   static constexpr int WG = C * T_LINE;
   static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
   static constexpr int LDS_ELEMS = NEED_LDS ? (C * N + (C * N) / 32 + 1) : 1;
+  static constexpr int LDS_IM = PLANES == 2 ? LDS_ELEMS : 1;
 };
 
 // LDS index maps.

@@ -83,6 +83,7 @@ __device__ __forceinline__ T wgs_update(T w, T measured, T desired, T beta) {
 template <typename T>
 struct OpFFInit {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = true, HAS_POST = false;
   GsBuf<T> b;
   const T* phi0;
   template <typename U>
@@ -107,6 +108,7 @@ struct OpFFInit {
 template <typename T>
 struct OpFFB {
   static constexpr int NACC = R_N;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -182,6 +184,7 @@ struct OpFFB {
 template <typename T>
 struct OpFFA {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
   GsBuf<T> b;
   int write_phase;  // GS: last iteration; WGS: when the iterate improved
   template <typename U>
@@ -213,6 +216,7 @@ struct OpFFA {
 template <typename T>
 struct OpIPInit {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = true, HAS_POST = false;
   GsBuf<T> b;
   const T* phi0;
   template <typename U>
@@ -280,6 +284,7 @@ __device__ __forceinline__ void ip_finalize_measure(const GsBuf<T>& b, int it, c
 template <typename T>
 struct OpIPMP {
   static constexpr int NACC = R_N;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -299,6 +304,7 @@ struct OpIPMP {
 template <typename T>
 struct OpIPProj {
   static constexpr int NACC = R_N;
+  static constexpr bool HAS_PRE = true, HAS_POST = false;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -319,6 +325,7 @@ struct OpIPProj {
 template <typename T>
 struct OpIPMeas {
   static constexpr int NACC = R_N;
+  static constexpr bool HAS_PRE = false, HAS_POST = true;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -366,6 +373,7 @@ struct OpIPMeas {
 template <typename T>
 struct OpIPPhase {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
   GsBuf<T> b;
   template <typename U>
   __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}

@@ -42,6 +42,10 @@ __device__ __forceinline__ void cmul(T& xr, T& xi, T br, T bi) {
 
 struct OpNone {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
+  static constexpr bool COL_WIDE = true;
+  // the op for a chunk of the batch starting at field f0
+  OpNone at(int) const { return *this; }
   template <typename T>
   __device__ __forceinline__ void pre(int, int, int, T&, T&, double*) const {}
   template <typename T>
@@ -73,10 +77,16 @@ struct AsmField {
 // index, which is what is evaluated here.  `conj` gives H(-z) = conj(H(z)).
 struct OpAsmH {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = false, HAS_POST = false;
   const AsmField* fp;
   double dfx, dfy, inv_l2;
   int nx, ny, evanescent;
   int conj;  // 1: multiply by conj(H) = H(-z)
+  OpAsmH at(int f0) const {
+    OpAsmH o = *this;
+    o.fp += f0;
+    return o;
+  }
   template <typename T>
   __device__ __forceinline__ void pre(int, int, int, T&, T&, double*) const {}
   template <typename T>
@@ -149,8 +159,14 @@ enum : int {
 template <int PRE_MODE, int POST_MODE>
 struct OpChirp {
   static constexpr int NACC = 0;
+  static constexpr bool HAS_PRE = PRE_MODE != 0, HAS_POST = POST_MODE != 0;
   const ChirpField* fp;
   int nx, ny;
+  OpChirp at(int f0) const {
+    OpChirp o = *this;
+    o.fp += f0;
+    return o;
+  }
   template <typename T>
   __device__ __forceinline__ void apply(int mode, int f, int i, int j, T& re, T& im) const {
 #pragma clang fp contract(off)

@@ -100,22 +100,178 @@ struct RowCfg {
   static constexpr int G = cmax(1, 256 / (T_LINE * N1));
   static constexpr int LINES = G * N1;
   static constexpr int WG = LINES * T_LINE;
-  static constexpr int LSTRIDE = NX + NX / 16 + 1;  // padded line stride in LDS
+  // Padded line stride in LDS.  With N1 > 1 the cross-row stages put the N1
+  // rows of a group in adjacent lanes, so the stride is 32/N1 mod 32 banks:
+  // lanes (t, n1) of a half-wave then fall in distinct banks.
+  static constexpr int LBASE = NX + NX / 16;
+  static constexpr int LSTRIDE =
+      N1 == 1 ? LBASE + 1 : LBASE + ((32 / N1 - LBASE % 32) % 32 + 32) % 32;
   static constexpr bool NEED_LDS = LinePlan<NX, P>::NST > 1 || N1 > 1;
   static constexpr int LDS_ELEMS = NEED_LDS ? LINES * LSTRIDE : 1;
   static_assert(WG <= 1024, "row workgroup too large");
+  static_assert(N1 == 1 || N1 == 2 || N1 == 4, "cross-row DFT is a lane network of 2 or 4");
 };
+
+// Whether a column pass with this op may use the wide tile (ops say so with
+// COL_WIDE; the default is no).
+template <class Op, class = void>
+struct OpWide {
+  static constexpr bool value = false;
+};
+template <class Op>
+struct OpWide<Op, std::void_t<decltype(Op::COL_WIDE)>> {
+  static constexpr bool value = Op::COL_WIDE;
+};
+
+// Which elementwise hooks an op uses (ops that do not say use both).  The
+// cross-row step may move past a hook that is a no-op.
+template <class Op, class = void>
+struct OpHooks {
+  static constexpr bool pre = true, post = true;
+};
+template <class Op>
+struct OpHooks<Op, std::void_t<decltype(Op::HAS_PRE), decltype(Op::HAS_POST)>> {
+  static constexpr bool pre = Op::HAS_PRE, post = Op::HAS_POST;
+};
+
+template <int N1>
+__device__ __forceinline__ constexpr int brev_lane(int l) {
+  return N1 == 4 ? ((l & 1) << 1) | ((l >> 1) & 1) : l;
+}
+
+template <typename T>
+__device__ __forceinline__ T shx(T v, int mask) {
+  if constexpr (sizeof(T) == 4) {
+    return __shfl_xor(v, mask, 64);
+  } else {
+    return __shfl_xor(v, mask, 64);
+  }
+}
+
+// One radix-2 lane butterfly: lower lane (bit clear) u+v, upper lane v-u.
+template <typename T>
+__device__ __forceinline__ void lane_bfly(T& ur, T& ui, int mask, bool upper) {
+  const T vr = shx(ur, mask), vi = shx(ui, mask);
+  if (!upper) {
+    ur = ur + vr;
+    ui = ui + vi;
+  } else {
+    ur = vr - ur;
+    ui = vi - ui;
+  }
+}
+
+// multiply by W4^e (e in {0,1}) forward, or its conjugate for INV
+template <bool INV, typename T>
+__device__ __forceinline__ void mul_w4(T& ur, T& ui, int e) {
+  if (e) {
+    const T r = ur;
+    if (!INV) {  // * (-i)
+      ur = ui;
+      ui = -r;
+    } else {  // * (+i)
+      ur = -ui;
+      ui = r;
+    }
+  }
+}
+
+template <typename T, int N1, bool INV, bool DIT>
+__device__ __forceinline__ void lane_dft(T& ur, T& ui, int n1) {
+  if constexpr (N1 == 2) {
+    lane_bfly(ur, ui, 1, (n1 & 1) != 0);
+  } else if constexpr (N1 == 4) {
+    if constexpr (!DIT) {
+      lane_bfly(ur, ui, 2, (n1 & 2) != 0);
+      if (n1 & 2) mul_w4<INV>(ur, ui, n1 & 1);
+      lane_bfly(ur, ui, 1, (n1 & 1) != 0);
+    } else {
+      lane_bfly(ur, ui, 1, (n1 & 1) != 0);
+      if (n1 & 2) mul_w4<INV>(ur, ui, n1 & 1);
+      lane_bfly(ur, ui, 2, (n1 & 2) != 0);
+    }
+  }
+}
+
+// Cross-row step on the registers of a lane whose group rows sit in adjacent
+// lanes (lane n1 holds row slot n1 of its group, the same positions in every
+// lane).  Forward: DFT across the lanes (lane n1 then holds k1 = bitrev(n1)),
+// then the twiddle W_ny^(n2*k1).  Inverse: lane n1 holds k1 = n1; conjugate
+// twiddle, inverse DFT across the lanes scaled by 1/N1 (lane n1 then holds
+// the natural row slot bitrev(n1)).
+template <typename T, int P, int N1, bool INV>
+__device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, int n2,
+                                            const T* __restrict__ twyr,
+                                            const T* __restrict__ twyi) {
+  const int k1 = INV ? n1 : brev_lane<N1>(n1);
+  const T wr = twyr[n2 * k1];
+  const T wi = INV ? -twyi[n2 * k1] : twyi[n2 * k1];
+  constexpr T sc = T(1) / T(N1);
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    if constexpr (INV) {
+      const T a = vr[i] * wr - vi[i] * wi;
+      vi[i] = vr[i] * wi + vi[i] * wr;
+      vr[i] = a;
+      lane_dft<T, N1, true, false>(vr[i], vi[i], n1);
+      vr[i] *= sc;
+      vi[i] *= sc;
+    } else {
+      lane_dft<T, N1, false, false>(vr[i], vi[i], n1);
+      const T a = vr[i] * wr - vi[i] * wi;
+      vi[i] = vr[i] * wi + vi[i] * wr;
+      vr[i] = a;
+    }
+  }
+}
+
+// Row-pass geometry with the cross-row step folded into the line FFT's own
+// exchanges.  Stages 0 and NST-1 use the contiguous layout (the threads of
+// one row are adjacent lanes, so loads and stores are 256-byte runs); stages
+// 1..NST-2 use the interleaved layout (lane = t*N1 + n1), where the cross-row
+// step runs on registers right after the exchange into stage 1.  After that
+// step a lane holds row slot bitrev(n1) and writes it back to that line, so
+// an LDS line index always equals the row slot it holds.  Row FFTs and the
+// cross-row step commute (one acts along x, the other across rows with
+// x-independent weights), so the order is free.  CROSS: 0 none, 1 forward,
+// 2 inverse.
+template <int NX, int NST, int LSTRIDE, int CROSS, typename T, int N1>
+struct RowLaneGeo {
+  int tc, lc, ti, li, lb, n1, n2;
+  const T* twyr;
+  const T* twyi;
+  static constexpr int S_IN = 1, S_OUT = NST - 1;
+  static_assert(NST >= 3, "needs two exchanges");
+  static constexpr bool inter(int s) { return s >= S_IN && s < S_OUT; }
+  __device__ __forceinline__ static int lidx(int line, int pos) {
+    return line * LSTRIDE + pos + (pos >> 4);
+  }
+  template <int S>
+  __device__ __forceinline__ int t() const { return inter(S) ? ti : tc; }
+  template <int S>
+  __device__ __forceinline__ int ridx(int pos) const { return lidx(inter(S) ? li : lc, pos); }
+  template <int S>
+  __device__ __forceinline__ int widx(int pos) const {
+    if constexpr (inter(S - 1)) return lidx((S - 1 == S_IN && CROSS != 0) ? lb : li, pos);
+    else return lidx(lc, pos);
+  }
+  template <int S, typename U, int PP>
+  __device__ __forceinline__ void after_read(U (&vr)[PP], U (&vi)[PP]) const {
+    if constexpr (S == S_IN && CROSS == 1) cross_lanes<U, PP, N1, false>(vr, vi, n1, n2, twyr, twyi);
+    if constexpr (S == S_IN && CROSS == 2) cross_lanes<U, PP, N1, true>(vr, vi, n1, n2, twyr, twyi);
+  }
+};
+
 
 // Steps 1+2 of the four-step column factorisation across the N1 rows of each
 // group (forward: DFT then twiddle; inverse: conjugate twiddle then inverse
 // DFT scaled by 1/N1).  Registers hold positions posf(i) of line `l`.
-template <typename T, int NX, int P, int N1, bool INV, class PosF>
+template <typename T, int NX, int P, int N1, bool INV, int LS, class PosF>
 __device__ __forceinline__ void cross_rows(T (&vr)[P], T (&vi)[P], int l, int t, const PosF& posf,
                                            T* lre, T* lim, int n2, const T* __restrict__ twyr,
                                            const T* __restrict__ twyi) {
   if constexpr (N1 > 1) {
     constexpr int T_LINE = NX / P;
-    constexpr int LS = NX + NX / 16 + 1;
     const int g = l / N1;
     const int n1 = l - g * N1;
     __syncthreads();
@@ -201,11 +357,17 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
   using CF = RowCfg<T, NX, N1>;
   constexpr int P = CF::P;
   using PL = LinePlan<NX, P>;
+  // Cross-row steps on lanes inside the line FFT's exchanges when the FFT
+  // has two exchanges and the hooks they move past are no-ops; otherwise
+  // through an extra LDS exchange around the FFT.
+  static_assert(!(MODE == 0 && XIN && XOUT), "one cross-row step per line FFT");
+  constexpr bool LANES = N1 > 1 && PL::NST >= 3 && (!XIN || !OpHooks<Op>::pre) &&
+                         (!XOUT || !OpHooks<Op>::post);
   __shared__ T lre[CF::LDS_ELEMS];
   __shared__ T lim[CF::LDS_ELEMS];
   const int N2 = ny / N1;
   const int tid = threadIdx.x;
-  const int l = tid / CF::T_LINE;
+  const int l = tid / CF::T_LINE;  // contiguous layout: line and position index
   const int t = tid - l * CF::T_LINE;
   const int g = l / N1;
   const int n1 = l - g * N1;
@@ -231,16 +393,34 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
 #pragma unroll
     for (int i = 0; i < P; ++i) vr[i] = vi[i] = T(0);
   }
-  if constexpr (XIN)
-    cross_rows<T, NX, P, N1, true>(
+  if constexpr (XIN && !LANES)
+    cross_rows<T, NX, P, N1, true, CF::LSTRIDE>(
         vr, vi, l, t, [&](int i) { return PL::template in_pos<false>(t, i); }, lre, lim, n2, twyr,
         twyi);
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) op.pre(f, PL::template in_pos<false>(t, i), y, vr[i], vi[i], acc);
   }
-  const RowLIdx li{l * CF::LSTRIDE};
-  fft_line<T, NX, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
+  // the two line FFTs (MODE 1) or one (MODE 0)
+  constexpr int CROSS1 = !LANES ? 0 : XIN ? 2 : (MODE == 0 && XOUT) ? 1 : 0;
+  constexpr int CROSS2 = !LANES ? 0 : XOUT ? 1 : 0;
+  auto line_fft = [&](auto inv_c, auto rev_c, auto cross_c) {
+    constexpr bool INV = decltype(inv_c)::value;
+    constexpr bool REV = decltype(rev_c)::value;
+    constexpr int CROSS = decltype(cross_c)::value;
+    if constexpr (LANES) {
+      constexpr int GT = N1 * CF::T_LINE;  // threads of one group
+      const int gt = tid - g * GT;
+      const int ni = gt % N1;
+      RowLaneGeo<NX, PL::NST, CF::LSTRIDE, CROSS, T, N1> geo{
+          t, l, gt / N1, g * N1 + ni, g * N1 + brev_lane<N1>(ni), ni, n2, twyr, twyi};
+      fft_line_g<T, NX, P, INV, REV>(vr, vi, geo, lre, lim, twr, twi);
+    } else {
+      const RowLIdx li{l * CF::LSTRIDE};
+      fft_line<T, NX, P, INV, REV>(vr, vi, t, lre, lim, li, twr, twi);
+    }
+  };
+  line_fft(std::bool_constant<INV1>{}, std::false_type{}, std::integral_constant<int, CROSS1>{});
   constexpr T invN = T(1) / T(NX);
   constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1) {
@@ -254,7 +434,7 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
         op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
       }
     }
-    fft_line<T, NX, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
+    line_fft(std::bool_constant<INV2>{}, std::true_type{}, std::integral_constant<int, CROSS2>{});
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
   if (active) {
@@ -267,8 +447,8 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
       op.post(f, PL::template out_pos<REV_OUT>(t, i), y, vr[i], vi[i], acc);
     }
   }
-  if constexpr (XOUT)
-    cross_rows<T, NX, P, N1, false>(
+  if constexpr (XOUT && !LANES)
+    cross_rows<T, NX, P, N1, false, CF::LSTRIDE>(
         vr, vi, l, t, [&](int i) { return PL::template out_pos<REV_OUT>(t, i); }, lre, lim, n2,
         twyr, twyi);
   if (active) {
@@ -295,16 +475,16 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
 // Hooks see the natural row index y = k1 + N1*p.
 // ---------------------------------------------------------------------------
 template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op>
-__global__ void __launch_bounds__((ColCfg<T, N2>::WG))
+__global__ void __launch_bounds__((ColCfg<T, N2, OpWide<Op>::value>::WG))
     k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
                S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,
                const T* __restrict__ twi) {
-  using CF = ColCfg<T, N2>;
+  using CF = ColCfg<T, N2, OpWide<Op>::value>;
   constexpr int P = CF::P;
   constexpr int C = CF::C;
   using PL = LinePlan<N2, P>;
   __shared__ T lre[CF::LDS_ELEMS];
-  __shared__ T lim[CF::LDS_ELEMS];
+  __shared__ T lim[CF::LDS_IM];
   const int tid = threadIdx.x;
   const int col = tid % C;
   const int t = tid / C;
@@ -340,7 +520,7 @@ __global__ void __launch_bounds__((ColCfg<T, N2>::WG))
     for (int i = 0; i < P; ++i) vr[i] = vi[i] = T(0);
   }
   const ColLIdx<C> li{col};
-  fft_line<T, N2, P, INV1, false>(vr, vi, t, lre, lim, li, twr, twi);
+  fft_line<T, N2, P, INV1, false, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   constexpr T invN = T(1) / T(N2);
   constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1) {
@@ -355,7 +535,7 @@ __global__ void __launch_bounds__((ColCfg<T, N2>::WG))
         op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
       }
     }
-    fft_line<T, N2, P, INV2, true>(vr, vi, t, lre, lim, li, twr, twi);
+    fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
   if (active) {

@@ -1,6 +1,7 @@
 // Free-space propagators (ref: src/propagate.js): angular spectrum, single-FFT
 // Fresnel forward/inverse, lens focal-plane transform.  Each is two or three
 // passes with the transfer function / chirps / checkerboards fused in.
+#include <algorithm>
 #include <cmath>
 #include <limits>
 #include <vector>
@@ -41,18 +42,24 @@ int run_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, const OpAsmH& op)
   Plan2 pl;
   int rc = make_plan2(ctx, in->nx, in->ny, sizeof(T), col_split<T>(in->nx, in->ny), &pl);
   if (rc) return rc;
-  const S* ire = (const S*)in->re;
-  const S* iim = (const S*)in->im;
-  S* re = (S*)out->re;
-  S* im = (S*)out->im;
+  const size_t plane = size_t(in->nx) * size_t(in->ny);
+  const int cf = chunk_fields(ctx, plane * 2 * sizeof(S), in->batch);
   const OpNone none{};
-  // rows forward (+ column split) -> column blocks F . H . F^-1 -> rows inverse
-  DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "asm_row_fwd", none, ire, iim, re, im,
-                                                         in->batch, pl, AllSizes{})));
-  DOE_HIP((col_pass<T, S, 1, false, true>(ctx, "asm_col_h", op, re, im, re, im, in->batch, pl,
-                                          AllSizes{})));
-  DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "asm_row_inv", none, re, im, re, im,
-                                                        in->batch, pl, AllSizes{})));
+  for (int f0 = 0; f0 < in->batch; f0 += cf) {
+    const int nb = std::min(cf, in->batch - f0);
+    const S* ire = (const S*)in->re + size_t(f0) * plane;
+    const S* iim = (const S*)in->im + size_t(f0) * plane;
+    S* re = (S*)out->re + size_t(f0) * plane;
+    S* im = (S*)out->im + size_t(f0) * plane;
+    // rows forward (+ column split) -> column blocks F . H . F^-1 -> rows
+    // inverse; every pass after the first runs in place on the chunk
+    DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "asm_row_fwd", none, ire, iim, re,
+                                                           im, nb, pl, AllSizes{})));
+    DOE_HIP((col_pass<T, S, 1, false, true>(ctx, "asm_col_h", op.at(f0), re, im, re, im, nb, pl,
+                                            AllSizes{})));
+    DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "asm_row_inv", none, re, im, re, im,
+                                                          nb, pl, AllSizes{})));
+  }
   return DOE_OK;
 }
 
@@ -65,31 +72,38 @@ static int run_chirp2(doe_ctx* ctx, const doe_field* in, doe_field* out, const C
   Plan2 pl;
   int rc = make_plan2(ctx, in->nx, in->ny, sizeof(T), col_split<T>(in->nx, in->ny), &pl);
   if (rc) return rc;
-  const S* ire = (const S*)in->re;
-  const S* iim = (const S*)in->im;
-  S* re = (S*)out->re;
-  S* im = (S*)out->im;
-  // the split column pass permutes rows: route it through the work planes
-  S* wre = re;
-  S* wim = im;
+  const size_t plane = size_t(in->nx) * size_t(in->ny);
+  const int cf = chunk_fields(ctx, plane * 2 * sizeof(S), in->batch);
+  // the split column pass permutes rows: route it through work planes
+  S* wre = nullptr;
+  S* wim = nullptr;
   if (pl.n1 > 1) {
-    rc = ensure_work(ctx, in->count() * sizeof(S));
+    rc = ensure_work(ctx, size_t(cf) * plane * sizeof(S));
     if (rc) return rc;
     wre = (S*)ctx->work_re;
     wim = (S*)ctx->work_im;
   }
-  const OpChirp<PRE, 0> op_first{dfp, in->nx, in->ny};
-  const OpChirp<0, POST> op_last{dfp, in->nx, in->ny};
-  if constexpr (!INV) {
-    DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "chirp_row", op_first, ire, iim, wre,
-                                                           wim, in->batch, pl, AllSizes{})));
-    DOE_HIP((col_pass<T, S, 0, false, false>(ctx, "chirp_col", op_last, wre, wim, re, im, in->batch,
-                                             pl, AllSizes{})));
-  } else {
-    DOE_HIP((col_pass<T, S, 0, true, false>(ctx, "chirp_col", op_first, ire, iim, wre, wim,
-                                            in->batch, pl, AllSizes{})));
-    DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "chirp_row", op_last, wre, wim, re, im,
-                                                          in->batch, pl, AllSizes{})));
+  const OpChirp<PRE, 0> first{dfp, in->nx, in->ny};
+  const OpChirp<0, POST> last{dfp, in->nx, in->ny};
+  for (int f0 = 0; f0 < in->batch; f0 += cf) {
+    const int nb = std::min(cf, in->batch - f0);
+    const S* ire = (const S*)in->re + size_t(f0) * plane;
+    const S* iim = (const S*)in->im + size_t(f0) * plane;
+    S* re = (S*)out->re + size_t(f0) * plane;
+    S* im = (S*)out->im + size_t(f0) * plane;
+    S* xre = pl.n1 > 1 ? wre : re;
+    S* xim = pl.n1 > 1 ? wim : im;
+    if constexpr (!INV) {
+      DOE_HIP((row_pass<T, S, 0, false, false, false, true>(ctx, "chirp_row", first.at(f0), ire,
+                                                             iim, xre, xim, nb, pl, AllSizes{})));
+      DOE_HIP((col_pass<T, S, 0, false, false>(ctx, "chirp_col", last.at(f0), xre, xim, re, im, nb,
+                                               pl, AllSizes{})));
+    } else {
+      DOE_HIP((col_pass<T, S, 0, true, false>(ctx, "chirp_col", first.at(f0), ire, iim, xre, xim,
+                                              nb, pl, AllSizes{})));
+      DOE_HIP((row_pass<T, S, 0, true, false, true, false>(ctx, "chirp_row", last.at(f0), xre, xim,
+                                                            re, im, nb, pl, AllSizes{})));
+    }
   }
   return DOE_OK;
 }

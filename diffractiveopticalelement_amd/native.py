@@ -61,6 +61,8 @@ SIGNATURES = [
     ("doe_ctx_stream", C.c_int, [_VP, C.POINTER(_VP)]),
     ("doe_ctx_set_flags", C.c_int, [_VP, C.c_int]),
     ("doe_ctx_profile_report", C.c_int, [_VP, C.c_char_p, C.c_long]),
+    ("doe_ctx_set_chunk_bytes", C.c_int, [_VP, C.c_long]),
+    ("doe_ctx_get_chunk_bytes", C.c_int, [_VP, C.POINTER(C.c_long)]),
     ("doe_timer_start", C.c_int, [_VP]),
     ("doe_timer_stop", C.c_int, [_VP, c_double_p]),
     ("doe_field_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_VP)]),
@@ -160,9 +162,17 @@ class Context:
         check(self.lib.doe_ctx_profile_report(self.h, buf, len(buf)))
         out = {}
         for ln in buf.value.decode().splitlines():
-            tag, cnt, ms = ln.split()
-            out[tag] = (int(cnt), float(ms))
+            tag, cnt, ms, pts = ln.split()
+            out[tag] = (int(cnt), float(ms), int(pts))
         return out
+
+    def set_chunk_bytes(self, nbytes: int) -> None:
+        check(self.lib.doe_ctx_set_chunk_bytes(self.h, int(nbytes)))
+
+    def chunk_bytes(self) -> int:
+        v = C.c_long()
+        check(self.lib.doe_ctx_get_chunk_bytes(self.h, C.byref(v)))
+        return v.value
 
     def timer_start(self) -> None:
         check(self.lib.doe_timer_start(self.h))

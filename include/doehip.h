@@ -59,12 +59,22 @@ int doe_ctx_stream(doe_ctx* ctx, void** stream);
  * device-resident fields return without synchronising the stream.
  * DOE_CTX_PROFILE: every kernel launch is bracketed by HIP events on the
  * context stream; doe_ctx_profile_report synchronises and writes one line
- * per kernel tag, "tag count total_ms", then clears the records.
+ * per kernel tag, "tag count total_ms total_points" (grid points summed over
+ * the launches), then clears the records.
  * doe_timer_start/stop bracket a region with two events on the stream. */
 #define DOE_CTX_ASYNC 1
 #define DOE_CTX_PROFILE 2
 int doe_ctx_set_flags(doe_ctx* ctx, int flags);
 int doe_ctx_profile_report(doe_ctx* ctx, char* buf, long cap);
+
+/* Batch chunking (no counterpart in the reference, whose transforms take one
+ * field at a time).  Batched FFT / propagation calls run chunk by chunk, a
+ * chunk's planes (re + im) being at most `bytes`, so the passes of one chunk
+ * can re-read it from the GPU's memory-side cache.  Default 0 = whole batch
+ * in one launch per pass (environment DOE_CHUNK_MIB overrides it at context
+ * creation). */
+int doe_ctx_set_chunk_bytes(doe_ctx* ctx, long bytes);
+int doe_ctx_get_chunk_bytes(doe_ctx* ctx, long* bytes);
 int doe_timer_start(doe_ctx* ctx);
 int doe_timer_stop(doe_ctx* ctx, double* ms);
 

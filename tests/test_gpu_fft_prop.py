@@ -185,6 +185,54 @@ def test_asm_batch_per_field_distances(ctx):
     g.free()
 
 
+def test_chunked_batches_match_whole_batch(ctx):
+    # a batch run chunk by chunk (per-field parameters offset per chunk) is
+    # bit-identical to the whole batch in one launch per pass
+    print("chunked batch 5x512^2", flush=True)
+    B, N, dx, lam = 5, 512, 8e-6, 633e-9
+    rng = np.random.default_rng(12)
+    zs = rng.uniform(0.01, 0.1, B)
+    u = rng.standard_normal((B, N, N)) + 1j * rng.standard_normal((B, N, N))
+    old = ctx.chunk_bytes()
+    outs = {}
+    try:
+        for chunk in (0, 2 * N * N * 8):  # whole batch; chunks of 2 fields (2, 2, 1)
+            ctx.set_chunk_bytes(chunk)
+            f = ctx.upload(u.real, u.imag, "f32")
+            g = ctx.field(N, N, B, "f32")
+            res = []
+            ctx.fft2d(f)
+            res.append(f.download())
+            ctx.fft2d(f, inverse=True)
+            res.append(f.download())
+            ctx.asm(f, g, dx, dx, lam, zs)
+            res.append(g.download())
+            ctx.asm(f, g, dx, dx, lam, zs, band_limit=False)
+            res.append(g.download())
+            ctx.fresnel(f, g, dx, dx, lam, zs)
+            res.append(g.download())
+            ctx.fresnel(g, g, dx, dx, lam, zs, inverse=True, in_dx=lam * zs[0] / (N * dx),
+                        in_dy=lam * zs[0] / (N * dx))
+            res.append(g.download())
+            outs[chunk] = res
+            f.free()
+            g.free()
+    finally:
+        ctx.set_chunk_bytes(old)
+    a, b = outs.values()
+    for (ra, ia), (rb, ib) in zip(a, b):
+        assert np.array_equal(ra, rb) and np.array_equal(ia, ib)
+    # the last field (last chunk) used its own distance: numpy reference
+    f1 = (np.arange(N) - N / 2) / (N * dx)
+    FX, FY = np.meshgrid(f1, f1)
+    arg = 1 / lam ** 2 - FX ** 2 - FY ** 2
+    H = np.where(arg >= 0, np.exp(2j * np.pi * zs[-1] * np.sqrt(np.abs(arg))), 0)
+    ref = np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(
+        np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(u[-1]))) * H)))
+    re, im = a[3]
+    assert rel(re[-1] + 1j * im[-1], ref) < 1e-5
+
+
 # ------------------------------------------------------------- Fresnel ----
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 @pytest.mark.parametrize("n,dx,z", [(8, 20e-6, 50e-3), (128, 8e-6, 30e-3), (256, 4e-6, 0.01)])

@@ -105,15 +105,22 @@ def test_fp32_first_ten_iterations(ctx, mode, alg, fb, esc):
 
 
 def test_fp32_longer_runs_stay_close(ctx):
-    # 60 iterations: FP32 rounding lets the trajectory drift from the FP64
-    # one, but the design quality does not: the best-iterate rmse (the WGS
-    # error history) stays within 1e-3 and the per-iteration efficiency
-    # within 5e-3 (measured drift 1.4e-3 for far-field WGS at 64^2).
+    # 60 iterations.  Image-plane WGS is well conditioned: the FP32 iterates
+    # stay within 1e-4 of the FP64 oracle's (measured <= 3e-6).  Far-field WGS
+    # is not: past ~20 iterations its per-iteration efficiency amplifies
+    # rounding differences (even the FP64 GPU path, which tracks the oracle to
+    # 1e-9 for 12 iterations, departs by 1.3e-2..2.3e-2 by iteration 60, seeds
+    # 1-3), so there the design quality is compared: the best-iterate rmse
+    # within 1e-3 (measured <= 2e-8) and the best efficiency reached within
+    # 1e-2 (measured <= 3.6e-3).
     n = 64
-    for mode, alg in (("far-field", "wgs"), ("image-plane", "wgs")):
-        ra, r, g = run_both(ctx, mode, alg, n, 60, "f32")
-        assert abs(r["error"][-1] - g["error"][-1]) < 1e-3, (mode, alg)
-        assert abs(r["efficiency"][-1] - g["efficiency"][-1]) < 5e-3, (mode, alg)
+    ra, r, g = run_both(ctx, "image-plane", "wgs", n, 60, "f32")
+    assert np.max(np.abs(np.array(r["error"]) - g["error"])) < 1e-4
+    assert np.max(np.abs(np.array(r["efficiency"]) - g["efficiency"])) < 1e-4
+    for seed in (1, 2):
+        ra, r, g = run_both(ctx, "far-field", "wgs", n, 60, "f32", seed=seed)
+        assert abs(r["error"][-1] - g["error"][-1]) < 1e-3, seed
+        assert abs(max(r["efficiency"]) - np.max(g["efficiency"])) < 1e-2, seed
 
 
 PROJ = [
