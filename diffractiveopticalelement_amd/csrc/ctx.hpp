@@ -86,6 +86,8 @@ struct doe_ctx {
   // at most this many bytes; 0 = whole batch at once (the default: on the
   // 16 x 2048^2 batch, 128 MiB chunks measured 8% slower than one launch)
   long chunk_bytes = 0;
+  // traversal direction of the next pass; flipped by every row/column launch
+  int dir = 0;
 };
 
 struct doe_field {
@@ -162,6 +164,15 @@ inline hipError_t finish(doe_ctx* ctx) {
   return hipStreamSynchronize(ctx->stream);
 }
 
+// Consecutive passes walk their grids in opposite directions, so each pass
+// starts on the data the previous one touched last (still in the GPU's
+// memory-side cache when the passes run in place).
+inline int next_dir(doe_ctx* ctx) {
+  const int d = ctx->dir;
+  ctx->dir ^= 1;
+  return d;
+}
+
 // Fields per chunk of a batched transform.  MI355X has a 256 MB memory-side
 // cache in front of HBM; keeping a chunk's planes (and the work planes that a
 // permuting pass writes) to about half of it lets the second and third passes
@@ -223,7 +234,7 @@ inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const 
     hipLaunchKernelGGL((k_row_pass<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op>),
                        dim3(unsigned(grid)), dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim,
                        pl.ny, groups, (const T*)pl.tx.re, (const T*)pl.tx.im, (const T*)pl.ty.re,
-                       (const T*)pl.ty.im);
+                       (const T*)pl.ty.im, next_dir(ctx));
     prof_end(ctx);
     return hipGetLastError();
   }
@@ -238,7 +249,7 @@ inline hipError_t launch_col(doe_ctx* ctx, const char* tag, const Op& op, const 
   prof_begin(ctx, tag, long(pl.nx) * pl.ny * batch);
   hipLaunchKernelGGL((k_col_pass<T, S, N2, MODE, INV1, INV2, Op>), dim3(unsigned(grid)),
                      dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, pl.nx, tiles, pl.n1,
-                     (const T*)pl.tc.re, (const T*)pl.tc.im);
+                     (const T*)pl.tc.re, (const T*)pl.tc.im, next_dir(ctx));
   prof_end(ctx);
   return hipGetLastError();
 }

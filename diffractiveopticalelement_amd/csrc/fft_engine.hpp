@@ -248,25 +248,37 @@ struct PChoice {
   static constexpr int P = N < 16 ? N : 16;
 };
 
-template <typename T, int N, bool WIDE_OK = false>
-struct ColCfg {
-  // FP32 lines of 512 points and more, for passes whose hooks are light
-  // (WIDE_OK): 32 points per thread and the planes exchanged one at a time,
-  // so a tile is 64 columns (256-byte row segments) at N = 512 and 32
-  // columns at N = 1024 within one 128 KB LDS buffer.  Heavier hooks would
-  // spill at 32 points per thread.
-  static constexpr bool WIDE = WIDE_OK && sizeof(T) == 4 && N >= 512;
-  static constexpr int P = WIDE ? 32 : PChoice<N>::P;
+// Column tile geometry: P points per thread, up to CMAX adjacent columns,
+// PLANES LDS buffers (1: re and im exchanged one after the other).  The tile
+// holds as many columns as fit a 128 KB LDS budget and 1024 threads.
+template <typename T, int N, int P_, int CMAX, int PLANES_>
+struct ColGeom {
+  static constexpr int P = P_ < N ? P_ : N;
   static constexpr int T_LINE = N / P;
-  static constexpr int PLANES = WIDE ? 1 : 2;
-  // values per LDS buffer
-  static constexpr int BUDGET = sizeof(T) == 4 ? (PLANES == 1 ? 32768 : 16384) : 8192;
-  static constexpr int C = cmax(1, cmin(64, cmin(BUDGET / N, 1024 / T_LINE)));
+  static constexpr int PLANES = PLANES_;
+  static constexpr int BUDGET = int(131072 / sizeof(T)) / PLANES;  // values per LDS buffer
+  static constexpr int C = cmax(1, cmin(CMAX, cmin(BUDGET / N, 1024 / T_LINE)));
   static constexpr int WG = C * T_LINE;
   static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
   static constexpr int LDS_ELEMS = NEED_LDS ? (C * N + (C * N) / 32 + 1) : 1;
   static constexpr int LDS_IM = PLANES == 2 ? LDS_ELEMS : 1;
 };
+
+// Default column geometry.  FP32 lines of 512 points and more, for passes
+// whose hooks are light (WIDE_OK): 32 points per thread and one LDS plane at
+// a time, 32 columns (128-byte row segments) per 512-thread workgroup, two
+// workgroups per CU (measured on 16 x 2048^2 at N = 512: 225 us against
+// 231 us for 64-column tiles at one workgroup per CU and 240 us for 16
+// points per thread with both planes).  Heavier hooks would spill at 32
+// points per thread.
+template <typename T, int N, bool WIDE_OK = false>
+struct ColCfgSel {
+  static constexpr bool WIDE = WIDE_OK && sizeof(T) == 4 && N >= 512;
+  using type = typename std::conditional<WIDE, ColGeom<T, N, 32, 32, 1>,
+                                         ColGeom<T, N, PChoice<N>::P, 64, 2>>::type;
+};
+template <typename T, int N, bool WIDE_OK = false>
+using ColCfg = typename ColCfgSel<T, N, WIDE_OK>::type;
 
 // LDS index maps.
 struct RowLIdx {

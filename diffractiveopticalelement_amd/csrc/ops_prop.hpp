@@ -3,16 +3,18 @@
 // Phase arguments that reach large magnitudes (kz*z ~ 1e5 rad, Fresnel chirps
 // ~ 1e4 rad) are formed in FP64 with the same operation order as the
 // reference and without FMA contraction, so the FP64 check mode reproduces
-// the reference's arguments bit for bit.  The FP32 path reduces the FP64
-// argument modulo 2*pi before its FP32 sincos.
+// the reference's arguments bit for bit.  The FP32 path converts the FP64
+// argument to revolutions, keeps the fractional part (exact in FP64) and
+// evaluates the hardware sine/cosine, whose argument is in revolutions (max
+// abs error 1.2e-7 on [-0.5, 0.5] revolutions, measured on gfx950 against
+// FP64; FP32 sincosf measures 7e-8 on the same grid, at ~4x the instructions).
 #pragma once
 #include "passes.hpp"
 
 namespace doe {
 
 constexpr double kPi = 3.141592653589793;
-constexpr double kTwoPi = 6.283185307179586;
-constexpr double kTwoPiLo = 2.4492935982947064e-16;  // 2*pi - kTwoPi
+constexpr double kInvTwoPi = 0.15915494309189535;    // 1 / (2*pi)
 
 // exp(i*phi) for an FP64 phase, evaluated in precision T.
 template <typename T>
@@ -23,13 +25,10 @@ __device__ __forceinline__ void cis(double phi, T& c, T& s) {
     c = cd;
     s = sd;
   } else {
-    const double n = rint(phi * (1.0 / kTwoPi));
-    double r = fma(-n, kTwoPi, phi);
-    r = fma(-n, kTwoPiLo, r);
-    float sf, cf;
-    sincosf(float(r), &sf, &cf);
-    c = cf;
-    s = sf;
+    const double rev = phi * kInvTwoPi;
+    const float fr = float(rev - rint(rev));
+    c = __builtin_amdgcn_cosf(fr);
+    s = __builtin_amdgcn_sinf(fr);
   }
 }
 

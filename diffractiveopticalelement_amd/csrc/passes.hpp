@@ -93,8 +93,9 @@ __device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::
 // ---------------------------------------------------------------------------
 // Row pass geometry: G groups of N1 rows, T_LINE = NX/P threads per row.
 // ---------------------------------------------------------------------------
-template <typename T, int NX, int N1>
+template <typename T, int NX, int N1, int PLANES_ = 2>
 struct RowCfg {
+  static constexpr int PLANES = PLANES_;
   static constexpr int P = PChoice<NX>::P;
   static constexpr int T_LINE = NX / P;
   static constexpr int G = cmax(1, 256 / (T_LINE * N1));
@@ -108,6 +109,7 @@ struct RowCfg {
       N1 == 1 ? LBASE + 1 : LBASE + ((32 / N1 - LBASE % 32) % 32 + 32) % 32;
   static constexpr bool NEED_LDS = LinePlan<NX, P>::NST > 1 || N1 > 1;
   static constexpr int LDS_ELEMS = NEED_LDS ? LINES * LSTRIDE : 1;
+  static constexpr int LDS_IM = PLANES == 2 ? LDS_ELEMS : 1;
   static_assert(WG <= 1024, "row workgroup too large");
   static_assert(N1 == 1 || N1 == 2 || N1 == 4, "cross-row DFT is a lane network of 2 or 4");
 };
@@ -349,12 +351,13 @@ __device__ __forceinline__ void cross_rows(T (&vr)[P], T (&vi)[P], int l, int t,
 // cross-row step first); XOUT: leave the output in four-step order.
 // ---------------------------------------------------------------------------
 template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
-          bool XOUT, class Op>
-__global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
+          bool XOUT, class Op, int RPLANES = 2>
+__global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
     k_row_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
                S* __restrict__ oim, int ny, long total_groups, const T* __restrict__ twr,
-               const T* __restrict__ twi, const T* __restrict__ twyr, const T* __restrict__ twyi) {
-  using CF = RowCfg<T, NX, N1>;
+               const T* __restrict__ twi, const T* __restrict__ twyr, const T* __restrict__ twyi,
+               int rev) {
+  using CF = RowCfg<T, NX, N1, RPLANES>;
   constexpr int P = CF::P;
   using PL = LinePlan<NX, P>;
   // Cross-row steps on lanes inside the line FFT's exchanges when the FFT
@@ -363,15 +366,19 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
   static_assert(!(MODE == 0 && XIN && XOUT), "one cross-row step per line FFT");
   constexpr bool LANES = N1 > 1 && PL::NST >= 3 && (!XIN || !OpHooks<Op>::pre) &&
                          (!XOUT || !OpHooks<Op>::post);
+  static_assert(CF::PLANES == 2 || LANES || N1 == 1, "the LDS cross-row step needs two planes");
   __shared__ T lre[CF::LDS_ELEMS];
-  __shared__ T lim[CF::LDS_ELEMS];
+  __shared__ T lim[CF::LDS_IM];
   const int N2 = ny / N1;
   const int tid = threadIdx.x;
   const int l = tid / CF::T_LINE;  // contiguous layout: line and position index
   const int t = tid - l * CF::T_LINE;
   const int g = l / N1;
   const int n1 = l - g * N1;
-  const long gi = (long)blockIdx.x * CF::G + g;
+  // rev: walk the grid from its far end (consecutive passes alternate, so a
+  // pass starts on the rows the previous one touched last, still in cache)
+  const int wb = rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x);
+  const long gi = (long)wb * CF::G + g;
   const bool active = gi < total_groups;
   const int f = active ? int(gi / N2) : 0;
   const int n2 = active ? int(gi - (long)f * N2) : 0;
@@ -414,10 +421,10 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
       const int ni = gt % N1;
       RowLaneGeo<NX, PL::NST, CF::LSTRIDE, CROSS, T, N1> geo{
           t, l, gt / N1, g * N1 + ni, g * N1 + brev_lane<N1>(ni), ni, n2, twyr, twyi};
-      fft_line_g<T, NX, P, INV, REV>(vr, vi, geo, lre, lim, twr, twi);
+      fft_line_g<T, NX, P, INV, REV, CF::PLANES>(vr, vi, geo, lre, lim, twr, twi);
     } else {
       const RowLIdx li{l * CF::LSTRIDE};
-      fft_line<T, NX, P, INV, REV>(vr, vi, t, lre, lim, li, twr, twi);
+      fft_line<T, NX, P, INV, REV, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
     }
   };
   line_fft(std::bool_constant<INV1>{}, std::false_type{}, std::integral_constant<int, CROSS1>{});
@@ -462,7 +469,7 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
   if constexpr (Op::NACC > 0) {
     // a workgroup's groups never span two fields (the launcher checks G | N2)
     const int wg_in_field = N2 / CF::G;
-    reduce_finalize<Op, CF::WG>(op, acc, tid, f, blockIdx.x, wg_in_field, (long)f * wg_in_field);
+    reduce_finalize<Op, CF::WG>(op, acc, tid, f, wb, wg_in_field, (long)f * wg_in_field);
   }
 }
 
@@ -474,12 +481,12 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1>::WG))
 //   fused   (MODE 1):        load and store N2*k1 + p, natural index in mid
 // Hooks see the natural row index y = k1 + N1*p.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op>
-__global__ void __launch_bounds__((ColCfg<T, N2, OpWide<Op>::value>::WG))
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op,
+          class CF = ColCfg<T, N2, OpWide<Op>::value>>
+__global__ void __launch_bounds__((CF::WG))
     k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
                S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,
-               const T* __restrict__ twi) {
-  using CF = ColCfg<T, N2, OpWide<Op>::value>;
+               const T* __restrict__ twi, int rev) {
   constexpr int P = CF::P;
   constexpr int C = CF::C;
   using PL = LinePlan<N2, P>;
@@ -488,7 +495,7 @@ __global__ void __launch_bounds__((ColCfg<T, N2, OpWide<Op>::value>::WG))
   const int tid = threadIdx.x;
   const int col = tid % C;
   const int t = tid / C;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int lb = xcd_remap(rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x), gridDim.x);
   const int per_field = tiles * n1cnt;
   const int f = lb / per_field;
   const int rem = lb - f * per_field;
