@@ -34,6 +34,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0
+# per-dispatch HBM byte counters of the same kernels (rocprofv3 FETCH_SIZE /
+# WRITE_SIZE runs of this script), scaled to bytes by the copy calibration
+COUNTERS = ROOT / "profiles" / "fft_passes_16x2048sq_f32_counters.json"
 B, N = 16, 2048
 DX, LAM = 4e-6, 633e-9
 
@@ -58,6 +61,17 @@ def cpu_baseline(n: int = 2048, reps: int = 8) -> dict | None:
                    f"{ms:.1f} ms per transform; GB/s counts the same 32 B/pt as the GPU value"),
         "host": platform.processor() or platform.machine(),
     }
+
+
+def traffic_of(tag: str) -> int | None:
+    """HBM bytes per launch of kernel `tag` from the committed counter run, or None."""
+    if not COUNTERS.exists():
+        return None
+    d = json.loads(COUNTERS.read_text())
+    k = d["kernels"].get(tag)
+    if k is None:
+        return None
+    return int(round((k["fetch_kib"] * d["scale"]["fetch"] + k["write_kib"] * d["scale"]["write"]) * 1024))
 
 
 def main() -> int:
@@ -173,7 +187,7 @@ def main() -> int:
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": round(dom["gbs_alg"] / PEAK_HBM_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_of(dom_tag),
             "alg_bytes_per_launch": 16 * dom["pts_per_launch"],
         },
         "chunk_bytes": ctx.chunk_bytes(),
