@@ -42,6 +42,7 @@ SOURCES = [
     ("fft.hip", [1, 2, 3, 4]),
     ("prop.hip", [0, 1, 2, 3, 4]),
     ("gs.hip", [0, 1, 2]),
+    ("gs_batch.hip", [None]),
 ]
 
 

@@ -22,192 +22,11 @@
 #include <limits>
 #include <vector>
 
-#include "ctx.hpp"
-#include "ops_prop.hpp"
+#include "gs_ops.hpp"
 
 namespace doe {
 
 int check_field(const doe_ctx* ctx, const doe_field* f);
-
-// Per-design device state (one per field of a batch).
-struct GsState {
-  double best_rmse;
-  double nrm;  // weight normalisation: w = w' * nrm
-  int best_phase_slot;
-  int best_amp_slot;
-  int improved;
-  int pad;
-};
-
-template <typename T>
-struct GsBuf {
-  const T* illum;
-  const T* tgt;
-  const T* roi;  // nullable: whole plane
-  T* w;          // unnormalised weights w' (nullable for GS)
-  T* phase[2];
-  T* amp[2];
-  GsState* state;
-  double* err_hist;
-  double* eff_hist;
-  int nx, ny;
-  double dxdy_x, dxdy_y;  // dx, dy for the image-plane efficiency expression
-  double norm;            // far field: 1/sqrt(nx*ny)
-  double gsign;           // far field: (-1)^(nx/2+ny/2)
-  double beta;
-  double feedback;
-  int escape;
-  int weighted;
-};
-
-// reduction slots
-enum { R_ERRNUM, R_ERRDEN, R_INROI, R_TOTAL, R_RNUM, R_RDEN, R_WSUM, R_TSUM, R_N };
-
-template <typename T>
-__device__ __forceinline__ T clampv(T v, T lo, T hi) {
-  return v < lo ? lo : (v > hi ? hi : v);
-}
-
-// WGS weight update of doe.js:514-521 for one ROI pixel; returns w'.
-template <typename T>
-__device__ __forceinline__ T wgs_update(T w, T measured, T desired, T beta) {
-  const T ratio = measured > T(1e-30) ? desired / measured : T(2);
-  T wn = w * pow(clampv(ratio, T(0.25), T(4)), beta);
-  return clampv(wn, T(1e-4), T(1e4));
-}
-
-// ---------------------------------------------------------------------------
-// Far field
-// ---------------------------------------------------------------------------
-// Initial row pass: x~ = c .* illum .* exp(i*phi0), then row FFT.
-template <typename T>
-struct OpFFInit {
-  static constexpr int NACC = 0;
-  static constexpr bool HAS_PRE = true, HAS_POST = false;
-  GsBuf<T> b;
-  const T* phi0;
-  template <typename U>
-  __device__ __forceinline__ void pre(int, int x, int y, U& re, U& im, double*) const {
-    const long k = (long)y * b.nx + x;
-    const T il = b.illum[k];
-    T s, c;
-    if constexpr (sizeof(T) == 8) sincos(phi0[k], &s, &c);
-    else sincosf(phi0[k], &s, &c);
-    const T cb = ((x + y) & 1) ? T(-1) : T(1);
-    re = cb * il * c;
-    im = cb * il * s;
-  }
-  template <typename U>
-  __device__ __forceinline__ void mid(int, int, int, U&, U&, double*) const {}
-  template <typename U>
-  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
-};
-
-// Column pass B: measurement, projection and WGS update (doe.js:675-712 and
-// 490-521) on the centred-index spectrum.
-template <typename T>
-struct OpFFB {
-  static constexpr int NACC = R_N;
-  static constexpr bool HAS_PRE = false, HAS_POST = false;
-  GsBuf<T> b;
-  double* partials;
-  unsigned* counters;
-  int it;
-  int write_amp;  // GS: last iteration writes |Y| to amp[0]
-  template <typename U>
-  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
-  template <typename U>
-  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
-  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double* acc) const {
-    const long k = (long)y * b.nx + x;
-    const T a = sqrt(re * re + im * im) * T(b.norm);
-    const T m = b.roi ? b.roi[k] : T(1);
-    const T t0 = b.tgt[k];
-    T w = T(1);
-    if (b.weighted) w = b.w[k] * T(b.state->nrm);
-    const T t = t0 * w;
-    acc[R_TOTAL] += double(a) * double(a);
-    if (m > T(0)) {
-      acc[R_INROI] += double(a) * double(a) * double(m);
-      const double d = (double(a) - double(t)) * double(m);
-      acc[R_ERRNUM] += d * d;
-      acc[R_ERRDEN] += double(t) * double(t);
-      if (b.weighted) {
-        const double d0 = double(a) - double(t0);
-        acc[R_RNUM] += d0 * d0;
-        acc[R_RDEN] += double(t0) * double(t0);
-        const T wn = wgs_update<T>(w, a, t0, T(b.beta));
-        b.w[k] = wn;
-        acc[R_WSUM] += double(wn) * double(wn) * double(t0) * double(t0);
-        acc[R_TSUM] += double(t0) * double(t0);
-      }
-    }
-    if (b.weighted) {
-      b.amp[1 - b.state->best_amp_slot][k] = a;
-    } else if (write_amp) {
-      b.amp[0][k] = a;
-    }
-    const T tp = m > T(0) ? t : a;
-    if (a > T(0)) {
-      const T s = tp / a;
-      re *= s;
-      im *= s;
-    } else {
-      const T cb = ((x + y) & 1) ? T(-1) : T(1);
-      re = T(b.gsign) * cb * tp / T(b.norm);
-      im = T(0);
-    }
-  }
-  __device__ __forceinline__ void finalize(int, const double* s) const {
-    GsState* st = b.state;
-    const double err = s[R_ERRDEN] > 0 ? sqrt(s[R_ERRNUM] / s[R_ERRDEN]) : 0.0;
-    const double eff = s[R_TOTAL] > 0 ? s[R_INROI] / s[R_TOTAL] : 0.0;
-    if (b.weighted) {
-      const double rmse = s[R_RDEN] > 0 ? sqrt(s[R_RNUM] / s[R_RDEN]) : 0.0;
-      if (rmse < st->best_rmse) {
-        st->best_rmse = rmse;
-        st->improved = 1;
-        st->best_amp_slot = 1 - st->best_amp_slot;
-      } else {
-        st->improved = 0;
-      }
-      st->nrm = s[R_WSUM] > 0 ? sqrt(s[R_TSUM] / s[R_WSUM]) : 1.0;
-      b.err_hist[it] = st->best_rmse;
-    } else {
-      b.err_hist[it] = err;
-    }
-    b.eff_hist[it] = eff;
-  }
-};
-
-// Row pass A: phase of the inverse transform, illumination re-imposed.
-template <typename T>
-struct OpFFA {
-  static constexpr int NACC = 0;
-  static constexpr bool HAS_PRE = false, HAS_POST = false;
-  GsBuf<T> b;
-  int write_phase;  // GS: last iteration; WGS: when the iterate improved
-  template <typename U>
-  __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
-  template <typename U>
-  __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
-  __device__ __forceinline__ void mid(int, int x, int y, T& re, T& im, double*) const {
-    const long k = (long)y * b.nx + x;
-    const T cb = ((x + y) & 1) ? T(-1) : T(1);
-    const bool wp = b.weighted ? (b.state->improved != 0) : (write_phase != 0);
-    if (wp) b.phase[0][k] = atan2(cb * im, cb * re);
-    const T r = sqrt(re * re + im * im);
-    const T il = b.illum[k];
-    if (r > T(0)) {
-      const T s = il / r;
-      re *= s;
-      im *= s;
-    } else {
-      re = cb * il;
-      im = T(0);
-    }
-  }
-};
 
 // ---------------------------------------------------------------------------
 // Image plane
@@ -396,42 +215,6 @@ struct OpIPPhase {
   }
 };
 
-template <typename T>
-__global__ void k_gs_init_state(GsState* st, T* w, const T* roi, long n, int weighted) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) {
-    st->best_rmse = __builtin_inf();
-    st->nrm = 1.0;
-    st->best_phase_slot = 0;
-    st->best_amp_slot = 0;
-    st->improved = 0;
-    st->pad = 0;
-  }
-  if (weighted && i < n) w[i] = (roi ? (roi[i] > T(0) ? T(1) : T(0)) : T(1));
-}
-
-template <typename T>
-__global__ void k_from_f64_t(const double* __restrict__ a, T* __restrict__ b, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = T(a[i]);
-}
-template <typename T>
-__global__ void k_to_f64_t(const T* __restrict__ a, double* __restrict__ b, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = double(a[i]);
-}
-
-// xorshift32 random phase of doe.js:65-86: seed>>>0 || 1; phi = (u*2-1)*pi.
-static void random_phase(std::vector<double>& p, unsigned seed) {
-  uint32_t s = seed ? seed : 1u;
-  for (double& v : p) {
-    s ^= s << 13;
-    s ^= s >> 17;
-    s ^= s << 5;
-    v = (double(s) / 4294967296.0 * 2 - 1) * kPi;
-  }
-}
-
 // Device workspace of one design.
 template <typename T>
 struct GsWork {
@@ -455,32 +238,6 @@ struct GsWork {
     return DOE_OK;
   }
 };
-
-template <typename T>
-static int upload_t(doe_ctx* ctx, const double* host, T* dst, long n) {
-  int rc = ensure_stage(ctx, size_t(n) * sizeof(double));
-  if (rc) return rc;
-  DOE_HIP(hipMemcpyAsync(ctx->stage, host, size_t(n) * sizeof(double), hipMemcpyHostToDevice,
-                         ctx->stream));
-  hipLaunchKernelGGL(k_from_f64_t<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     (const double*)ctx->stage, dst, n);
-  DOE_HIP(hipGetLastError());
-  DOE_HIP(hipStreamSynchronize(ctx->stream));
-  return DOE_OK;
-}
-
-template <typename T>
-static int download_t(doe_ctx* ctx, const T* src, double* host, long n) {
-  int rc = ensure_stage(ctx, size_t(n) * sizeof(double));
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_to_f64_t<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     src, (double*)ctx->stage, n);
-  DOE_HIP(hipGetLastError());
-  DOE_HIP(hipMemcpyAsync(host, ctx->stage, size_t(n) * sizeof(double), hipMemcpyDeviceToHost,
-                         ctx->stream));
-  DOE_HIP(hipStreamSynchronize(ctx->stream));
-  return DOE_OK;
-}
 
 template <typename T>
 int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const double* tgt,
@@ -533,7 +290,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   DOE_HIP(hipMemsetAsync(ph[1], 0, tb, ctx->stream));
   DOE_HIP(hipMemsetAsync(W.hist, 0, sizeof(double) * 2 * size_t(K > 0 ? K : 1), ctx->stream));
   hipLaunchKernelGGL(k_gs_init_state<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                     st_d, w_d, (const T*)W.roi_d, n, weighted ? 1 : 0);
+                     st_d, 1, w_d, (const T*)W.roi_d, n, weighted ? 1 : 0);
   DOE_HIP(hipGetLastError());
 
   GsBuf<T>& b = W.b;
@@ -558,6 +315,8 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   b.feedback = plan->feedback;
   b.escape = plan->escape;
   b.weighted = weighted ? 1 : 0;
+  b.fstride = n;
+  b.hstride = 0;
 
   Plan2 pl;
   rc = make_plan2(ctx, nx, ny, sizeof(T), col_split<T>(nx, ny), &pl);

@@ -55,19 +55,32 @@ __device__ __forceinline__ void block_sum(double (&acc)[NACC], double* sh, int t
   }
 }
 
-// Per-workgroup partials + last-arriving-workgroup finalisation for one field.
-// `wg_in_field` workgroups contribute to field f; `slot` indexes partials.
-template <class Op, int NTHR>
-__device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::NACC], int tid,
-                                                int f, long slot, int wg_in_field,
-                                                long first_slot) {
-  constexpr int NA = Op::NACC;
+// Per-workgroup partials + last-arriving-workgroup finalisation for one field,
+// in two phases so a pass can sum its accumulators as soon as its hooks are
+// done (freeing their registers) and arrive later.
+// Phase 1: block-wide sum of the per-thread accumulators into `out` (valid in
+// thread 0).
+template <int NA, int NTHR>
+__device__ __forceinline__ void wg_sum(double (&acc)[NA], double* out, int tid) {
   __shared__ double sh[16 * NA];
-  __shared__ int last;
   block_sum<NA>(acc, sh, tid, NTHR);
   if (tid == 0) {
 #pragma unroll
-    for (int k = 0; k < NA; ++k) op.partials[slot * NA + k] = acc[k];
+    for (int k = 0; k < NA; ++k) out[k] = acc[k];
+  }
+}
+// Phase 2: thread 0 publishes the workgroup's sums to `partials[slot]` and
+// arrives on the field's counter; the last of `wg_in_field` workgroups folds
+// the field's partials in slot order and calls op.finalize.
+template <class Op, int NTHR>
+__device__ __forceinline__ void arrive_finalize(const Op& op, const double* wg_sums, int tid, int f,
+                                                long slot, int wg_in_field, long first_slot) {
+  constexpr int NA = Op::NACC;
+  __shared__ double sh[16 * NA];
+  __shared__ int last;
+  if (tid == 0) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) op.partials[slot * NA + k] = wg_sums[k];
     __threadfence();
     const unsigned prev = atomicAdd(op.counters + f, 1u);
     last = (prev == unsigned(wg_in_field - 1));
@@ -89,16 +102,24 @@ __device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::
     }
   }
 }
+template <class Op, int NTHR>
+__device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::NACC], int tid,
+                                                int f, long slot, int wg_in_field,
+                                                long first_slot) {
+  __shared__ double sums[Op::NACC];
+  wg_sum<Op::NACC, NTHR>(acc, sums, tid);
+  arrive_finalize<Op, NTHR>(op, sums, tid, f, slot, wg_in_field, first_slot);
+}
 
 // ---------------------------------------------------------------------------
 // Row pass geometry: G groups of N1 rows, T_LINE = NX/P threads per row.
 // ---------------------------------------------------------------------------
-template <typename T, int NX, int N1, int PLANES_ = 2>
+template <typename T, int NX, int N1, int PLANES_ = 2, int WGT = 256>
 struct RowCfg {
   static constexpr int PLANES = PLANES_;
   static constexpr int P = PChoice<NX>::P;
   static constexpr int T_LINE = NX / P;
-  static constexpr int G = cmax(1, 256 / (T_LINE * N1));
+  static constexpr int G = cmax(1, WGT / (T_LINE * N1));  // groups per workgroup
   static constexpr int LINES = G * N1;
   static constexpr int WG = LINES * T_LINE;
   // Padded line stride in LDS.  With N1 > 1 the cross-row stages put the N1
@@ -350,14 +371,17 @@ __device__ __forceinline__ void cross_rows(T (&vr)[P], T (&vi)[P], int l, int t,
 // XIN: the input's column axis is in four-step order (apply the inverse
 // cross-row step first); XOUT: leave the output in four-step order.
 // ---------------------------------------------------------------------------
+// The body of a row pass for logical workgroup `wb`, on the LDS buffers the
+// caller provides (CF::LDS_ELEMS and CF::LDS_IM elements).  Shared by the
+// one-batch kernel below and the table-driven batched kernels.
 template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
-          bool XOUT, class Op, int RPLANES = 2>
-__global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
-    k_row_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
-               S* __restrict__ oim, int ny, long total_groups, const T* __restrict__ twr,
-               const T* __restrict__ twi, const T* __restrict__ twyr, const T* __restrict__ twyi,
-               int rev) {
-  using CF = RowCfg<T, NX, N1, RPLANES>;
+          bool XOUT, class Op, class CF>
+__device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire,
+                                         const S* __restrict__ iim, S* __restrict__ ore,
+                                         S* __restrict__ oim, int ny, long total_groups,
+                                         const T* __restrict__ twr, const T* __restrict__ twi,
+                                         const T* __restrict__ twyr, const T* __restrict__ twyi,
+                                         int wb, T* lre, T* lim) {
   constexpr int P = CF::P;
   using PL = LinePlan<NX, P>;
   // Cross-row steps on lanes inside the line FFT's exchanges when the FFT
@@ -367,17 +391,12 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
   constexpr bool LANES = N1 > 1 && PL::NST >= 3 && (!XIN || !OpHooks<Op>::pre) &&
                          (!XOUT || !OpHooks<Op>::post);
   static_assert(CF::PLANES == 2 || LANES || N1 == 1, "the LDS cross-row step needs two planes");
-  __shared__ T lre[CF::LDS_ELEMS];
-  __shared__ T lim[CF::LDS_IM];
   const int N2 = ny / N1;
   const int tid = threadIdx.x;
   const int l = tid / CF::T_LINE;  // contiguous layout: line and position index
   const int t = tid - l * CF::T_LINE;
   const int g = l / N1;
   const int n1 = l - g * N1;
-  // rev: walk the grid from its far end (consecutive passes alternate, so a
-  // pass starts on the rows the previous one touched last, still in cache)
-  const int wb = rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x);
   const long gi = (long)wb * CF::G + g;
   const bool active = gi < total_groups;
   const int f = active ? int(gi / N2) : 0;
@@ -385,6 +404,8 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
   const int y = n2 + N2 * n1;
   const long base = ((long)f * ny + y) * (long)NX;
   constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
+  constexpr bool EARLY_SUM = false;  // summing before the second FFT measured worse (spills)
+  __shared__ double wg_sums[NA];
   T vr[P], vi[P];
   double acc[NA];
 #pragma unroll
@@ -441,6 +462,8 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
         op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
       }
     }
+    // sums complete after the mid hook: sum them before the second FFT
+    if constexpr (EARLY_SUM) wg_sum<NA, CF::WG>(acc, wg_sums, tid);
     line_fft(std::bool_constant<INV2>{}, std::true_type{}, std::integral_constant<int, CROSS2>{});
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
@@ -469,8 +492,29 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
   if constexpr (Op::NACC > 0) {
     // a workgroup's groups never span two fields (the launcher checks G | N2)
     const int wg_in_field = N2 / CF::G;
-    reduce_finalize<Op, CF::WG>(op, acc, tid, f, wb, wg_in_field, (long)f * wg_in_field);
+    if constexpr (EARLY_SUM)
+      arrive_finalize<Op, CF::WG>(op, wg_sums, tid, f, wb, wg_in_field, (long)f * wg_in_field);
+    else
+      reduce_finalize<Op, CF::WG>(op, acc, tid, f, wb, wg_in_field, (long)f * wg_in_field);
   }
+}
+
+template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
+          bool XOUT, class Op, int RPLANES = 2>
+__global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
+    k_row_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
+               S* __restrict__ oim, int ny, long total_groups, const T* __restrict__ twr,
+               const T* __restrict__ twi, const T* __restrict__ twyr, const T* __restrict__ twyi,
+               int rev) {
+  using CF = RowCfg<T, NX, N1, RPLANES>;
+  __shared__ T lre[CF::LDS_ELEMS];
+  __shared__ T lim[CF::LDS_IM];
+  // rev: walk the grid from its far end (consecutive passes alternate, so a
+  // pass starts on the rows the previous one touched last, still in cache)
+  const int wb = rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x);
+  row_body<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op, CF>(op, ire, iim, ore, oim, ny,
+                                                             total_groups, twr, twi, twyr, twyi,
+                                                             wb, lre, lim);
 }
 
 // ---------------------------------------------------------------------------
@@ -481,21 +525,20 @@ __global__ void __launch_bounds__((RowCfg<T, NX, N1, RPLANES>::WG))
 //   fused   (MODE 1):        load and store N2*k1 + p, natural index in mid
 // Hooks see the natural row index y = k1 + N1*p.
 // ---------------------------------------------------------------------------
-template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op,
-          class CF = ColCfg<T, N2, OpWide<Op>::value>>
-__global__ void __launch_bounds__((CF::WG))
-    k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
-               S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,
-               const T* __restrict__ twi, int rev) {
+// The body of a column pass for logical workgroup `lb` on caller-provided
+// LDS (CF::LDS_ELEMS and CF::LDS_IM elements).
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op, class CF>
+__device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire,
+                                         const S* __restrict__ iim, S* __restrict__ ore,
+                                         S* __restrict__ oim, int nx, int tiles, int n1cnt,
+                                         const T* __restrict__ twr, const T* __restrict__ twi,
+                                         int lb, T* lre, T* lim) {
   constexpr int P = CF::P;
   constexpr int C = CF::C;
   using PL = LinePlan<N2, P>;
-  __shared__ T lre[CF::LDS_ELEMS];
-  __shared__ T lim[CF::LDS_IM];
   const int tid = threadIdx.x;
   const int col = tid % C;
   const int t = tid / C;
-  const int lb = xcd_remap(rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x), gridDim.x);
   const int per_field = tiles * n1cnt;
   const int f = lb / per_field;
   const int rem = lb - f * per_field;
@@ -508,6 +551,8 @@ __global__ void __launch_bounds__((CF::WG))
   constexpr bool NAT_IN = MODE == 0 && INV1;
   constexpr bool NAT_OUT = MODE == 0 && !INV1;
   constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
+  constexpr bool EARLY_SUM = false;  // summing before the second FFT measured worse (spills)
+  __shared__ double wg_sums[NA];
   T vr[P], vi[P];
   double acc[NA];
 #pragma unroll
@@ -542,6 +587,9 @@ __global__ void __launch_bounds__((CF::WG))
         op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
       }
     }
+    // sums complete after the mid hook: sum them across the workgroup before
+    // the second FFT, so the FP64 accumulators are not live across it
+    if constexpr (EARLY_SUM) wg_sum<NA, CF::WG>(acc, wg_sums, tid);
     fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
@@ -560,9 +608,24 @@ __global__ void __launch_bounds__((CF::WG))
       st(oim + a, vi[i]);
     }
   }
-  if constexpr (Op::NACC > 0) {
+  if constexpr (EARLY_SUM) {
+    arrive_finalize<Op, CF::WG>(op, wg_sums, tid, f, lb, per_field, (long)f * per_field);
+  } else if constexpr (Op::NACC > 0) {
     reduce_finalize<Op, CF::WG>(op, acc, tid, f, lb, per_field, (long)f * per_field);
   }
+}
+
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op,
+          class CF = ColCfg<T, N2, OpWide<Op>::value>>
+__global__ void __launch_bounds__((CF::WG))
+    k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
+               S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,
+               const T* __restrict__ twi, int rev) {
+  __shared__ T lre[CF::LDS_ELEMS];
+  __shared__ T lim[CF::LDS_IM];
+  const int lb = xcd_remap(rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x), gridDim.x);
+  col_body<T, S, N2, MODE, INV1, INV2, Op, CF>(op, ire, iim, ore, oim, nx, tiles, n1cnt, twr, twi,
+                                               lb, lre, lim);
 }
 
 // Elementwise pass over a batch (no FFT): op.apply(field, x, y, k).

@@ -89,7 +89,17 @@ SIGNATURES = [
     ("doe_gs_run", C.c_int, [_VP, C.POINTER(GsPlan), c_double_p, c_double_p, c_double_p,
                              c_double_p, C.c_uint, c_double_p, c_double_p, c_double_p,
                              c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("doe_gs_batch_create", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_double,
+                                      C.c_int, C.POINTER(_VP)]),
+    ("doe_gs_batch_destroy", C.c_int, [_VP]),
+    ("doe_gs_batch_set", C.c_int, [_VP, C.c_int, c_double_p, c_double_p, c_double_p, c_double_p]),
+    ("doe_gs_batch_random_init", C.c_int, [_VP, C.c_ulonglong]),
+    ("doe_gs_batch_run", C.c_int, [_VP, C.c_int, C.c_int]),
+    ("doe_gs_batch_get", C.c_int, [_VP, C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
+                                   c_double_p, c_double_p]),
 ]
+
+GSB_GRAPH = 1
 
 _lib: C.CDLL | None = None
 
@@ -273,6 +283,60 @@ class Context:
             _dptr(out["rmse"])))
         out["rmse"] = float(out["rmse"][0])
         return out
+
+
+class GsBatch:
+    """Batched far-field GS/WGS over designs of uneven (square) sides."""
+
+    def __init__(self, ctx: Context, sides, algorithm: int, beta: float = 0.9,
+                 with_roi: bool = False):
+        self.ctx = ctx
+        self.sides = [int(v) for v in sides]
+        arr = (C.c_int * len(self.sides))(*self.sides)
+        h = _VP()
+        check(ctx.lib.doe_gs_batch_create(ctx.h, len(self.sides), arr, int(algorithm), float(beta),
+                                          int(with_roi), C.byref(h)))
+        self.h = h
+        self.iterations = 0
+
+    def set(self, index: int, illumination, target_amp, roi=None, init_phase=None) -> None:
+        ill = _darr(illumination).ravel()
+        tgt = _darr(target_amp).ravel()
+        roi_a = None if roi is None else _darr(roi).ravel()
+        ip = None if init_phase is None else _darr(init_phase).ravel()
+        check(self.ctx.lib.doe_gs_batch_set(self.h, int(index), _dptr(ill), _dptr(tgt),
+                                            _dptr(roi_a) if roi_a is not None else None,
+                                            _dptr(ip) if ip is not None else None))
+
+    def random_init(self, seed: int) -> None:
+        check(self.ctx.lib.doe_gs_batch_random_init(self.h, C.c_ulonglong(seed)))
+
+    def run(self, iterations: int, graph: bool = True) -> None:
+        check(self.ctx.lib.doe_gs_batch_run(self.h, int(iterations), GSB_GRAPH if graph else 0))
+        self.iterations = int(iterations)
+
+    def get(self, index: int) -> dict:
+        n = self.sides[index] ** 2
+        k = self.iterations
+        out = {"phase": np.zeros(n), "error": np.zeros(k), "efficiency": np.zeros(k),
+               "amplitude": np.zeros(n), "weights": np.zeros(n), "rmse": np.zeros(1)}
+        check(self.ctx.lib.doe_gs_batch_get(self.h, int(index), _dptr(out["phase"]),
+                                            _dptr(out["error"]), _dptr(out["efficiency"]),
+                                            _dptr(out["amplitude"]), _dptr(out["weights"]),
+                                            _dptr(out["rmse"])))
+        out["rmse"] = float(out["rmse"][0])
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.doe_gs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Field:

@@ -164,6 +164,36 @@ int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* illumination
                double* amplitude_out, double* intensity_out, double* weights_out,
                double* best_rmse_out);
 
+/* Batched far-field phase retrieval.  No single counterpart in the
+ * reference, which designs one element per call: each design of a batch runs
+ * fourierCgh (src/doe.js:639-730), under DOE_ALG_WGS with the weight update
+ * and best-iterate tracking of weightedGerchbergSaxton (doe.js:455-546), with
+ * the same results as doe_gs_run in DOE_MODE_FAR_FIELD / DOE_F32 on that
+ * design alone.  A batch holds `count` square designs of sides 256, 512,
+ * 1024 or 2048, mixed freely; every iteration of the whole batch is two
+ * persistent kernel launches driven by a per-design size table, and with
+ * DOE_GSB_GRAPH the iteration loop is captured once as a hipGraph and
+ * replayed.  Inputs per design (doe_gs_batch_set): illumination, target
+ * amplitude, ROI (only if the batch was created with_roi; then required),
+ * initial phase (NULL keeps the current one; doe_gs_batch_random_init draws
+ * every design's phase on the device from a counter-based splitmix64 stream,
+ * phi = (u*2 - 1)*pi in FP64).  doe_gs_batch_run restarts every design from
+ * its initial phase.  Outputs per design as doe_gs_run: phase, error and
+ * efficiency histories (iterations of the last run), amplitude of the
+ * reported iterate, weights, best rmse (NaN for GS). */
+typedef struct doe_gs_batch doe_gs_batch;
+#define DOE_GSB_GRAPH 1
+int doe_gs_batch_create(doe_ctx* ctx, int count, const int* sides, int algorithm, double beta,
+                        int with_roi, doe_gs_batch** out);
+int doe_gs_batch_destroy(doe_gs_batch* batch);
+int doe_gs_batch_set(doe_gs_batch* batch, int index, const double* illumination,
+                     const double* target_amp, const double* roi, const double* init_phase);
+int doe_gs_batch_random_init(doe_gs_batch* batch, unsigned long long seed);
+int doe_gs_batch_run(doe_gs_batch* batch, int iterations, int flags);
+int doe_gs_batch_get(doe_gs_batch* batch, int index, double* phase_out, double* error_out,
+                     double* efficiency_out, double* amplitude_out, double* weights_out,
+                     double* best_rmse_out);
+
 #ifdef __cplusplus
 }
 #endif
