@@ -82,6 +82,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--chunk-mib", type=int, default=None,
                     help="batch chunk size in MiB of field planes (default: the library's, 0 = whole batch)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the fft2d autotuner (built-in configuration)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy the 512 MiB batch once (device-to-device), so a profiler's "
                          "byte counters can be scaled against a copy of known size")
@@ -114,6 +116,12 @@ def main() -> int:
         ctx.lib.doe_field_copy(ctx.h, field.h, scratch.h)
         ctx.sync()
 
+    tuning = None
+    if not args.no_tune:
+        # pick the fastest configuration for this shape (outside the timed region)
+        rep_t = ctx.autotune_fft2d(N, N, B, "f32", reps=2)
+        tuning = ctx.tune_get(N, N, B, "f32")
+        tuning["candidates"] = len(rep_t.strip().splitlines()) - 1
     ctx.set_flags(asynchronous=True, profile=False)
     for _ in range(args.warmup):
         ctx.fft2d(field)
@@ -191,6 +199,7 @@ def main() -> int:
             "alg_bytes_per_launch": 16 * dom["pts_per_launch"],
         },
         "chunk_bytes": ctx.chunk_bytes(),
+        "tuning": tuning,
         "kernels": kernels,
         "pipeline": pipeline,
         "cpu_baseline": cpu,

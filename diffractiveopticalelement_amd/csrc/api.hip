@@ -1,8 +1,10 @@
 // C-ABI entry points: context, fields, conversions, fft2d.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ctx.hpp"
 
@@ -560,6 +562,122 @@ int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
   rc = dispatch_fft2d(ctx, f, inverse);
   if (rc) return rc;
   DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int colvar,
+                 int rowplanes, long chunk_bytes) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  DOE_REQUIRE(n1 == 0 || n1 == 1 || n1 == 2 || n1 == 4, DOE_ERR_INVALID, "tune: n1 must be 0, 1, 2 or 4");
+  DOE_REQUIRE(colvar >= 0 && colvar <= 3, DOE_ERR_INVALID, "tune: colvar must be 0..3");
+  DOE_REQUIRE(rowplanes == 1 || rowplanes == 2, DOE_ERR_INVALID, "tune: rowplanes must be 1 or 2");
+  doe_tune t;
+  t.n1 = n1;
+  t.colvar = colvar;
+  t.rowplanes = rowplanes;
+  t.chunk = chunk_bytes;
+  ctx->tune[std::make_tuple(nx, ny, batch, dtype)] = t;
+  return DOE_OK;
+}
+
+int doe_tune_get(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int* n1, int* colvar,
+                 int* rowplanes, long* chunk_bytes) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  const doe_tune t = tune_for(ctx, nx, ny, batch, dtype);
+  const int builtin_n1 = dtype == DOE_F64 ? col_split<double>(nx, ny) : col_split<float>(nx, ny);
+  if (n1) *n1 = t.n1 > 0 ? t.n1 : builtin_n1;
+  if (colvar) *colvar = t.colvar;
+  if (rowplanes) *rowplanes = t.rowplanes;
+  if (chunk_bytes) *chunk_bytes = t.chunk >= 0 ? t.chunk : ctx->chunk_bytes;
+  return DOE_OK;
+}
+
+int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int reps, char* report,
+                       long cap) {
+  DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
+  DOE_REQUIRE(reps > 0, DOE_ERR_INVALID, "autotune: reps must be positive");
+  doe_field* f = nullptr;
+  int rc = doe_field_create(ctx, nx, ny, batch, dtype, &f);
+  if (rc) return rc;
+  const size_t bytes = f->count() * dtype_size(dtype);
+  if (hipMemsetAsync(f->re, 0, bytes, ctx->stream) != hipSuccess ||
+      hipMemsetAsync(f->im, 0, bytes, ctx->stream) != hipSuccess) {
+    doe_field_free(f);
+    set_error("autotune: hipMemsetAsync failed");
+    return DOE_ERR_HIP;
+  }
+  const auto key = std::make_tuple(nx, ny, batch, dtype);
+  const bool had = ctx->tune.count(key) > 0;
+  const doe_tune saved_tune = had ? ctx->tune[key] : doe_tune{};
+  const int saved_flags = ctx->flags;
+  ctx->flags = DOE_CTX_ASYNC;
+  const bool f32 = dtype == DOE_F32;
+  const bool split_dtype = dtype != DOE_F64;
+  const long fb = long(nx) * ny * 2 * long(dtype_size(dtype));
+  std::vector<long> chunks = {0};
+  if (batch > 1 && fb * batch > (128L << 20)) chunks.push_back(128L << 20);
+  std::string rep;
+  double best_ms = 1e30;
+  doe_tune best{};
+  for (int n1 : {1, 2, 4}) {
+    if (n1 > 1 && (!split_dtype || ny % n1 || ny / n1 < 2 || nx < 256 || long(nx) * n1 > 16384))
+      continue;
+    for (int rp : {2, 1}) {
+      if (rp == 1 && !f32) continue;
+      for (int cv = 0; cv <= (f32 ? 3 : 0); ++cv) {
+        for (long ch : chunks) {
+          doe_tune t;
+          t.n1 = n1;
+          t.colvar = cv;
+          t.rowplanes = rp;
+          t.chunk = ch;
+          ctx->tune[key] = t;
+          // one forward + inverse pair untimed (plans, work planes), then `reps`
+          int r = dispatch_fft2d(ctx, f, 0);
+          if (!r) r = dispatch_fft2d(ctx, f, 1);
+          if (r || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;  // a variant that does not exist for this shape
+          }
+          double ms = 0;
+          r = doe_timer_start(ctx);
+          for (int k = 0; !r && k < reps; ++k) {
+            r = dispatch_fft2d(ctx, f, 0);
+            if (!r) r = dispatch_fft2d(ctx, f, 1);
+          }
+          if (!r) r = doe_timer_stop(ctx, &ms);
+          if (r) continue;
+          ms /= 2.0 * reps;  // per transform
+          char line[160];
+          snprintf(line, sizeof line, "n1=%d colvar=%d rowplanes=%d chunk=%ld ms=%.4f\n", n1, cv,
+                   rp, ch, ms);
+          rep += line;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+          }
+        }
+      }
+    }
+  }
+  ctx->flags = saved_flags;
+  doe_field_free(f);
+  if (best_ms >= 1e30) {
+    if (had) ctx->tune[key] = saved_tune;
+    else ctx->tune.erase(key);
+    set_error("autotune: no configuration ran");
+    return DOE_ERR_UNSUPPORTED;
+  }
+  ctx->tune[key] = best;
+  char line[160];
+  snprintf(line, sizeof line, "best n1=%d colvar=%d rowplanes=%d chunk=%ld ms=%.4f\n", best.n1,
+           best.colvar, best.rowplanes, best.chunk, best_ms);
+  rep += line;
+  if (report && cap > 0) {
+    const size_t n = std::min(rep.size(), size_t(cap - 1));
+    std::memcpy(report, rep.data(), n);
+    report[n] = 0;
+  }
   return DOE_OK;
 }
 

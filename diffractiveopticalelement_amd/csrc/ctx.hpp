@@ -6,6 +6,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -51,6 +52,14 @@ struct Twiddle {
 
 }  // namespace doe
 
+// One tuned configuration of a batched fft2d shape (see doe_autotune_fft2d).
+struct doe_tune {
+  int n1 = 0;         // column split (0: the built-in choice)
+  int colvar = 0;     // column tile variant (FP32; 0: the built-in choice)
+  int rowplanes = 2;  // LDS planes of the row pass (FP32 split passes)
+  long chunk = -1;    // chunk bytes (-1: the context's)
+};
+
 struct doe_prof_rec {
   const char* tag;
   hipEvent_t a, b;
@@ -88,6 +97,8 @@ struct doe_ctx {
   long chunk_bytes = 0;
   // traversal direction of the next pass; flipped by every row/column launch
   int dir = 0;
+  // tuned fft2d configurations, key (nx, ny, batch, dtype)
+  std::map<std::tuple<int, int, int, int>, doe_tune> tune;
 };
 
 struct doe_field {
@@ -178,13 +189,16 @@ inline int next_dir(doe_ctx* ctx) {
 // permuting pass writes) to about half of it lets the second and third passes
 // of a transform re-read the chunk from that cache instead of HBM.  Chunks
 // are balanced so the last one is not a short tail.
-inline int chunk_fields(const doe_ctx* ctx, size_t field_bytes, int batch) {
-  if (ctx->chunk_bytes <= 0 || batch <= 1) return batch;
-  long c = long(size_t(ctx->chunk_bytes) / field_bytes);
+inline int chunk_fields(long chunk_bytes, size_t field_bytes, int batch) {
+  if (chunk_bytes <= 0 || batch <= 1) return batch;
+  long c = long(size_t(chunk_bytes) / field_bytes);
   if (c < 1) c = 1;
   if (c >= batch) return batch;
   const long nchunks = (batch + c - 1) / c;
   return int((batch + nchunks - 1) / nchunks);
+}
+inline int chunk_fields(const doe_ctx* ctx, size_t field_bytes, int batch) {
+  return chunk_fields(ctx->chunk_bytes, field_bytes, batch);
 }
 
 // Column split N1 of the four-step column factorisation for an nx-by-ny grid
@@ -218,20 +232,22 @@ struct Plan2 {
 int make_plan2(doe_ctx* ctx, int nx, int ny, int tsize, int n1, Plan2* p);
 
 template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
-          bool XOUT, class Op>
+          bool XOUT, class Op, int RPL = 2>
 inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const S* ire,
                              const S* iim, S* ore, S* oim, int batch, const Plan2& pl) {
-  if constexpr (!split_ok<T, NX, N1>()) {
-    return hipErrorInvalidValue;  // never selected by col_split
+  constexpr bool LANES_POSSIBLE = N1 > 1 && LinePlan<NX, PChoice<NX>::P>::NST >= 3 &&
+                                  (!XIN || !OpHooks<Op>::pre) && (!XOUT || !OpHooks<Op>::post);
+  if constexpr (!split_ok<T, NX, N1>() || (RPL == 1 && !(LANES_POSSIBLE || N1 == 1))) {
+    return hipErrorInvalidValue;  // never selected by col_split / the tuner
   } else {
-    using CF = RowCfg<T, NX, N1>;
+    using CF = RowCfg<T, NX, N1, RPL>;
     const long groups = (long)(pl.ny / N1) * batch;
     const long grid = (groups + CF::G - 1) / CF::G;
     if constexpr (Op::NACC > 0) {
       if ((pl.ny / N1) % CF::G != 0) return hipErrorInvalidValue;
     }
     prof_begin(ctx, tag, long(pl.nx) * pl.ny * batch);
-    hipLaunchKernelGGL((k_row_pass<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op>),
+    hipLaunchKernelGGL((k_row_pass<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op, RPL>),
                        dim3(unsigned(grid)), dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim,
                        pl.ny, groups, (const T*)pl.tx.re, (const T*)pl.tx.im, (const T*)pl.ty.re,
                        (const T*)pl.ty.im, next_dir(ctx));
@@ -240,14 +256,27 @@ inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const 
   }
 }
 
-template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op>
+// Column tile variants the tuner chooses among (FP32): 0 the op's default,
+// 1: 16 points x up to 64 columns, both planes; 2: 32 points x up to 64
+// columns, one plane; 3: 32 points x up to 32 columns, one plane.
+template <typename T, int N2, class Op, int CV>
+struct ColVar {
+  using type = typename std::conditional<
+      CV == 1, ColGeom<T, N2, 16, 64, 2>,
+      typename std::conditional<
+          CV == 2, ColGeom<T, N2, 32, 64, 1>,
+          typename std::conditional<CV == 3, ColGeom<T, N2, 32, 32, 1>,
+                                    typename OpColCfg<T, N2, Op>::type>::type>::type>::type;
+};
+
+template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op, int CV = 0>
 inline hipError_t launch_col(doe_ctx* ctx, const char* tag, const Op& op, const S* ire,
                              const S* iim, S* ore, S* oim, int batch, const Plan2& pl) {
-  using CF = ColCfg<T, N2, OpWide<Op>::value>;
+  using CF = typename ColVar<T, N2, Op, CV>::type;
   const int tiles = (pl.nx + CF::C - 1) / CF::C;
   const long grid = (long)tiles * pl.n1 * batch;
   prof_begin(ctx, tag, long(pl.nx) * pl.ny * batch);
-  hipLaunchKernelGGL((k_col_pass<T, S, N2, MODE, INV1, INV2, Op>), dim3(unsigned(grid)),
+  hipLaunchKernelGGL((k_col_pass<T, S, N2, MODE, INV1, INV2, Op, CF>), dim3(unsigned(grid)),
                      dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, pl.nx, tiles, pl.n1,
                      (const T*)pl.tc.re, (const T*)pl.tc.im, next_dir(ctx));
   prof_end(ctx);
@@ -256,7 +285,7 @@ inline hipError_t launch_col(doe_ctx* ctx, const char* tag, const Op& op, const 
 
 // Row pass at the plan's split (instantiates only the valid (NX, N1) pairs).
 template <typename T, typename S, int MODE, bool INV1, bool INV2, bool XIN, bool XOUT, class Op,
-          class NF>
+          int RPL = 2, class NF>
 inline hipError_t row_pass(doe_ctx* ctx, const char* tag, const Op& op, const S* ire, const S* iim,
                            S* ore, S* oim, int batch, const Plan2& pl, NF&& with_sizes) {
   hipError_t e = hipErrorInvalidValue;
@@ -264,22 +293,28 @@ inline hipError_t row_pass(doe_ctx* ctx, const char* tag, const Op& op, const S*
     constexpr int NX = decltype(NN)::value;
     with_n1(pl.n1, [&](auto MM) {
       constexpr int N1 = decltype(MM)::value;
-      e = launch_row<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op>(ctx, tag, op, ire, iim, ore,
-                                                                    oim, batch, pl);
+      e = launch_row<T, S, NX, N1, MODE, INV1, INV2, XIN, XOUT, Op, RPL>(ctx, tag, op, ire, iim,
+                                                                         ore, oim, batch, pl);
     });
   });
   return e;
 }
 
-template <typename T, typename S, int MODE, bool INV1, bool INV2, class Op, class NF>
+template <typename T, typename S, int MODE, bool INV1, bool INV2, class Op, int CV = 0, class NF>
 inline hipError_t col_pass(doe_ctx* ctx, const char* tag, const Op& op, const S* ire, const S* iim,
                            S* ore, S* oim, int batch, const Plan2& pl, NF&& with_sizes) {
   hipError_t e = hipErrorInvalidValue;
   with_sizes(pl.n2, [&](auto NN) {
-    e = launch_col<T, S, decltype(NN)::value, MODE, INV1, INV2, Op>(ctx, tag, op, ire, iim, ore,
-                                                                     oim, batch, pl);
+    e = launch_col<T, S, decltype(NN)::value, MODE, INV1, INV2, Op, CV>(ctx, tag, op, ire, iim,
+                                                                         ore, oim, batch, pl);
   });
   return e;
+}
+
+// The tuned configuration for an fft2d shape, or the built-in one.
+inline doe_tune tune_for(const doe_ctx* ctx, int nx, int ny, int batch, int dtype) {
+  auto it = ctx->tune.find(std::make_tuple(nx, ny, batch, dtype));
+  return it == ctx->tune.end() ? doe_tune{} : it->second;
 }
 
 // size dispatchers usable as the `with_sizes` argument
@@ -290,6 +325,12 @@ struct AllSizes {
 struct GsSizes {
   template <class F>
   bool operator()(int n, F&& f) const { return with_n_gs(n, f); }
+};
+// sides 256..8192 only (the tuner's variants; smaller sides keep the built-in
+// configuration)
+struct LargeSizes {
+  template <class F>
+  bool operator()(int n, F&& f) const { return n >= 256 && with_n(n, f); }
 };
 
 // Family entry points (defined in their own translation units; each

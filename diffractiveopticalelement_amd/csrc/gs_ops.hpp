@@ -57,10 +57,19 @@ __device__ __forceinline__ T clampv(T v, T lo, T hi) {
 }
 
 // WGS weight update of doe.js:514-521 for one ROI pixel; returns w'.
+// FP32: the power as exp2(beta * log2(r)) on the hardware log2/exp2 (r is
+// clamped to [1/4, 4], so log2(r) is in [-2, 2]).
 template <typename T>
 __device__ __forceinline__ T wgs_update(T w, T measured, T desired, T beta) {
   const T ratio = measured > T(1e-30) ? desired / measured : T(2);
-  T wn = w * pow(clampv(ratio, T(0.25), T(4)), beta);
+  const T r = clampv(ratio, T(0.25), T(4));
+  T f;
+  if constexpr (sizeof(T) == 8) {
+    f = pow(r, beta);
+  } else {
+    f = __builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(r));
+  }
+  const T wn = w * f;
   return clampv(wn, T(1e-4), T(1e4));
 }
 
@@ -97,6 +106,9 @@ template <typename T>
 struct OpFFB {
   static constexpr int NACC = R_N;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
+  // 512-thread tiles of 32 points per thread with both LDS planes: one
+  // workgroup per CU, but a 256-register budget instead of spilling at 128
+  static constexpr int COL_P = 32, COL_PLANES = 2;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;

@@ -77,6 +77,9 @@ struct AsmField {
 struct OpAsmH {
   static constexpr int NACC = 0;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
+  // column tiles of 32 points per thread, both LDS planes, 512 threads: the
+  // FP64 transfer-function argument fits the 256-register budget unspilled
+  static constexpr int COL_P = 32, COL_PLANES = 2;
   const AsmField* fp;
   double dfx, dfy, inv_l2;
   int nx, ny, evanescent;

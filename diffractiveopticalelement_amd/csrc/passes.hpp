@@ -146,6 +146,19 @@ struct OpWide<Op, std::void_t<decltype(Op::COL_WIDE)>> {
   static constexpr bool value = Op::COL_WIDE;
 };
 
+// Column geometry an op asks for (COL_P points per thread, COL_PLANES LDS
+// planes) for FP32 lines of 512+ points; default: the generic choice.
+template <typename T, int N, class Op, class = void>
+struct OpColCfg {
+  using type = ColCfg<T, N, OpWide<Op>::value>;
+};
+template <typename T, int N, class Op>
+struct OpColCfg<T, N, Op, std::void_t<decltype(Op::COL_P), decltype(Op::COL_PLANES)>> {
+  using type = typename std::conditional<(sizeof(T) == 4 && N >= 512),
+                                         ColGeom<T, N, Op::COL_P, 64, Op::COL_PLANES>,
+                                         ColCfg<T, N, OpWide<Op>::value>>::type;
+};
+
 // Which elementwise hooks an op uses (ops that do not say use both).  The
 // cross-row step may move past a hook that is a no-op.
 template <class Op, class = void>
@@ -616,7 +629,7 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
 }
 
 template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op,
-          class CF = ColCfg<T, N2, OpWide<Op>::value>>
+          class CF = typename OpColCfg<T, N2, Op>::type>
 __global__ void __launch_bounds__((CF::WG))
     k_col_pass(Op op, const S* __restrict__ ire, const S* __restrict__ iim, S* __restrict__ ore,
                S* __restrict__ oim, int nx, int tiles, int n1cnt, const T* __restrict__ twr,

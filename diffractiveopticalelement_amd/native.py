@@ -62,6 +62,12 @@ SIGNATURES = [
     ("doe_ctx_set_flags", C.c_int, [_VP, C.c_int]),
     ("doe_ctx_profile_report", C.c_int, [_VP, C.c_char_p, C.c_long]),
     ("doe_ctx_set_chunk_bytes", C.c_int, [_VP, C.c_long]),
+    ("doe_tune_set", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_long]),
+    ("doe_tune_get", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                               C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long)]),
+    ("doe_autotune_fft2d", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                                     C.c_long]),
     ("doe_ctx_get_chunk_bytes", C.c_int, [_VP, C.POINTER(C.c_long)]),
     ("doe_timer_start", C.c_int, [_VP]),
     ("doe_timer_stop", C.c_int, [_VP, c_double_p]),
@@ -175,6 +181,22 @@ class Context:
             tag, cnt, ms, pts = ln.split()
             out[tag] = (int(cnt), float(ms), int(pts))
         return out
+
+    def tune_set(self, nx, ny, batch, dtype, n1=0, colvar=0, rowplanes=2, chunk=-1) -> None:
+        check(self.lib.doe_tune_set(self.h, nx, ny, batch, DTYPES[dtype], n1, colvar, rowplanes,
+                                    chunk))
+
+    def tune_get(self, nx, ny, batch, dtype) -> dict:
+        v = [C.c_int(), C.c_int(), C.c_int(), C.c_long()]
+        check(self.lib.doe_tune_get(self.h, nx, ny, batch, DTYPES[dtype], *[C.byref(x) for x in v]))
+        return {"n1": v[0].value, "colvar": v[1].value, "rowplanes": v[2].value,
+                "chunk": v[3].value}
+
+    def autotune_fft2d(self, nx, ny, batch, dtype="f32", reps=3) -> str:
+        buf = C.create_string_buffer(1 << 14)
+        check(self.lib.doe_autotune_fft2d(self.h, nx, ny, batch, DTYPES[dtype], reps, buf,
+                                          len(buf)))
+        return buf.value.decode()
 
     def set_chunk_bytes(self, nbytes: int) -> None:
         check(self.lib.doe_ctx_set_chunk_bytes(self.h, int(nbytes)))

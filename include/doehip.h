@@ -95,6 +95,25 @@ int doe_field_copy(doe_ctx* ctx, const doe_field* src, doe_field* dst);
  * In place on every grid of the batch.  Forward is the unscaled DFT with
  * exp(-2*pi*i*n*k/N); inverse uses exp(+...) and scales by 1/(nx*ny). */
 int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
+
+/* fft2d tuning (no counterpart in the reference).  A context keeps one
+ * configuration per (nx, ny, batch, dtype): the column split n1 (1, 2 or 4;
+ * 0 = built-in), the FP32 column tile variant colvar (0 built-in; 1: 16
+ * points x 64 columns, both LDS planes; 2: 32 points x 64 columns, one plane;
+ * 3: 32 points x 32 columns, one plane), the LDS planes of FP32 row passes
+ * (1 or 2) and the chunk size in bytes (-1 = the context's).  Variants are
+ * built for sides 256..8192.  doe_autotune_fft2d times every configuration
+ * that exists for the shape (forward + inverse, `reps` times, on a scratch
+ * field), keeps the fastest and writes one line per candidate
+ * ("n1=.. colvar=.. rowplanes=.. chunk=.. ms=..", ms per transform) and a
+ * final "best ..." line into `report`. */
+int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int colvar,
+                 int rowplanes, long chunk_bytes);
+int doe_tune_get(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int* n1, int* colvar,
+                 int* rowplanes, long* chunk_bytes);
+int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int reps, char* report,
+                       long cap);
+
 /* Host drop-in of fft2d(re, im, nx, ny, inverse) (ref: src/fft.js:148):
  * uploads the Float64 planes, transforms in `dtype`, downloads in place.
  * Returns DOE_ERR_SHAPE if len != nx*ny (the reference's length error). */

@@ -1,0 +1,93 @@
+"""fft2d tuning: every configuration the tuner can pick (column split, column
+tile variant, row-pass LDS planes, chunking) computes the same transform.
+
+Each variant is checked against numpy's FFT (FP32 gate 1e-5 relative L2) and
+by its inverse (round trip 1e-6), and the autotuner's pick is checked the
+same way.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from diffractiveopticalelement_amd import native as nat
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = nat.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def valid(nx, ny, n1, colvar, rowplanes):
+    """Which configurations exist (FP32)."""
+    if n1 > 1 and (ny % n1 or nx < 256 or nx * n1 > 16384):
+        return False
+    variant = colvar != 0 or rowplanes != 2
+    if variant and (nx < 256 or ny // n1 < 256):
+        return False
+    if rowplanes == 1 and n1 > 1 and nx < 512:
+        return False
+    return True
+
+
+@pytest.mark.parametrize("nx,ny,batch", [(2048, 2048, 2), (1024, 1024, 3), (512, 4096, 1),
+                                         (4096, 1024, 1), (256, 2048, 2)])
+def test_every_variant_computes_fft2d(ctx, nx, ny, batch):
+    print(f"tune variants {nx}x{ny}x{batch}", flush=True)
+    rng = np.random.default_rng(nx + ny + batch)
+    u = rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))
+    ref = np.fft.fft2(u)
+    f = ctx.field(nx, ny, batch, "f32")
+    ran = 0
+    for n1, cv, rp, ch in itertools.product((1, 2, 4), (0, 1, 2, 3), (2, 1), (0, 8 << 20)):
+        if not valid(nx, ny, n1, cv, rp):
+            continue
+        ctx.tune_set(nx, ny, batch, "f32", n1, cv, rp, ch)
+        f.upload(u.real, u.imag)
+        ctx.fft2d(f)
+        re, im = f.download()
+        assert rel(re + 1j * im, ref) < 1e-5, (n1, cv, rp, ch)
+        ctx.fft2d(f, inverse=True)
+        re, im = f.download()
+        assert rel(re + 1j * im, u) < 1e-6, (n1, cv, rp, ch)
+        ran += 1
+    assert ran >= 8
+    ctx.tune_set(nx, ny, batch, "f32")  # back to the built-in configuration
+    f.free()
+
+
+def test_autotune_picks_a_working_configuration(ctx):
+    nx, ny, batch = 1024, 1024, 2
+    rep = ctx.autotune_fft2d(nx, ny, batch, "f32", reps=2)
+    lines = rep.strip().splitlines()
+    assert lines[-1].startswith("best ")
+    assert len(lines) >= 9
+    t = ctx.tune_get(nx, ny, batch, "f32")
+    best = dict(kv.split("=") for kv in lines[-1].split()[1:])
+    assert int(best["n1"]) == t["n1"] and int(best["colvar"]) == t["colvar"]
+    assert int(best["rowplanes"]) == t["rowplanes"] and int(best["chunk"]) == t["chunk"]
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))
+    f = ctx.upload(u.real, u.imag, "f32")
+    ctx.fft2d(f)
+    re, im = f.download()
+    assert rel(re + 1j * im, np.fft.fft2(u)) < 1e-5
+    f.free()
+
+
+def test_tune_argument_errors(ctx):
+    with pytest.raises(nat.DoeError, match="n1"):
+        ctx.tune_set(512, 512, 1, "f32", n1=3)
+    with pytest.raises(nat.DoeError, match="colvar"):
+        ctx.tune_set(512, 512, 1, "f32", colvar=7)
+    # an untuned shape reports the built-in choice
+    t = ctx.tune_get(2048, 2048, 16, "f32")
+    assert t["n1"] == 4 and t["colvar"] == 0 and t["rowplanes"] == 2
