@@ -43,6 +43,8 @@ SOURCES = [
     ("prop.hip", [0, 1, 2, 3, 4]),
     ("gs.hip", [0, 1, 2]),
     ("gs_batch.hip", [None]),
+    ("nn.hip", [None]),
+    ("warmstart.hip", [None]),
 ]
 
 

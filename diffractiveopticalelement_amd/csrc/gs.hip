@@ -323,7 +323,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   if (rc) return rc;
   // reduction scratch: partials per workgroup (row or column pass), one
   // arrival counter per field
-  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_N, 1);
+  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF, 1);
   if (rc) return rc;
   T* re = W.fre;
   T* im = W.fim;

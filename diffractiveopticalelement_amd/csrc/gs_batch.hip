@@ -22,7 +22,7 @@
 #include <numeric>
 #include <vector>
 
-#include "gs_ops.hpp"
+#include "gs_batch.hpp"
 
 namespace doe {
 
@@ -62,58 +62,6 @@ inline const char* gsb_tag(int n, int pass) {
 
 using namespace doe;
 
-struct doe_gs_batch {
-  struct Class {
-    int n = 0;
-    std::vector<int> users;  // user index of each design of the class
-    size_t off = 0;          // element offset of the class's first design
-    GsState* state = nullptr;
-    double* partials = nullptr;
-    unsigned* counters = nullptr;
-    size_t hist_off = 0;     // design d's histories at hist + hist_off + d*2K
-    Plan2 pl;
-    GsBuf<float> b{};
-  };
-  doe_ctx* ctx = nullptr;
-  int count = 0;
-  int algorithm = DOE_ALG_GS;
-  double beta = 0.9;
-  bool has_roi = false;
-  std::vector<int> side;                    // per user index
-  std::vector<std::pair<int, int>> where;   // user index -> (class, position)
-  std::vector<Class> cls;
-  size_t total = 0;
-  std::vector<void*> allocs;
-  float* re = nullptr;
-  float* im = nullptr;
-  float* illum = nullptr;
-  float* tgt = nullptr;
-  float* roi = nullptr;
-  float* w = nullptr;
-  float* phi0 = nullptr;
-  float* ph[2] = {nullptr, nullptr};
-  float* am[2] = {nullptr, nullptr};
-  double* hist = nullptr;
-  int hist_k = 0;  // history capacity per design
-  int last_k = 0;  // iterations of the last run
-  hipGraphExec_t graph = nullptr;
-  int graph_k = -1;
-
-  ~doe_gs_batch() {
-    if (graph) hipGraphExecDestroy(graph);
-    for (void* p : allocs) hipFree(p);
-  }
-  int alloc(void** p, size_t bytes) {
-    if (hipMalloc(p, bytes) != hipSuccess) {
-      (void)hipGetLastError();
-      set_error("doe_gs_batch: device allocation failed");
-      return DOE_ERR_OOM;
-    }
-    allocs.push_back(*p);
-    return DOE_OK;
-  }
-};
-
 namespace {
 
 // Per-class operation buffers for the current history capacity.
@@ -121,7 +69,7 @@ int gsb_prepare(doe_gs_batch* B, int K) {
   doe_ctx* ctx = B->ctx;
   if (K > B->hist_k) {
     void* p = nullptr;
-    int rc = B->alloc(&p, sizeof(double) * 2 * size_t(K) * B->count);
+    int rc = B->alloc(&p, sizeof(double) * 3 * size_t(K) * B->count);
     if (rc) return rc;
     B->hist = (double*)p;
     B->hist_k = K;
@@ -131,7 +79,7 @@ int gsb_prepare(doe_gs_batch* B, int K) {
       B->graph_k = -1;
     }
   }
-  DOE_HIP(hipMemsetAsync(B->hist, 0, sizeof(double) * 2 * size_t(B->hist_k) * B->count,
+  DOE_HIP(hipMemsetAsync(B->hist, 0, sizeof(double) * 3 * size_t(B->hist_k) * B->count,
                          ctx->stream));
   size_t h = 0;
   for (auto& c : B->cls) {
@@ -149,8 +97,9 @@ int gsb_prepare(doe_gs_batch* B, int K) {
     c.hist_off = h;
     b.err_hist = B->hist + h;
     b.eff_hist = B->hist + h + B->hist_k;
-    b.hstride = 2L * B->hist_k;
-    h += size_t(2) * B->hist_k * c.users.size();
+    b.scr_hist = B->hist + h + 2 * size_t(B->hist_k);
+    b.hstride = 3L * B->hist_k;
+    h += size_t(3) * B->hist_k * c.users.size();
     b.nx = c.n;
     b.ny = c.n;
     b.fstride = long(c.n) * c.n;
@@ -269,7 +218,7 @@ int doe_gs_batch_create(doe_ctx* ctx, int count, const int* sides, int algorithm
     const size_t slots = size_t(std::max(c.n, 4 * c.n)) * nd;
     if (!rc) rc = B->alloc(&p, sizeof(GsState) * nd);
     if (!rc) c.state = (GsState*)p;
-    if (!rc) rc = B->alloc(&p, sizeof(double) * R_N * slots);
+    if (!rc) rc = B->alloc(&p, sizeof(double) * R_NFF * slots);
     if (!rc) c.partials = (double*)p;
     if (!rc) rc = B->alloc(&p, sizeof(unsigned) * nd);
     if (!rc) {
@@ -409,13 +358,28 @@ int doe_gs_batch_get(doe_gs_batch* B, int index, double* phase_out, double* erro
   if (rc) return rc;
   const int K = B->last_k;
   if ((error_out || efficiency_out) && K > 0) {
-    std::vector<double> h(2 * size_t(B->hist_k));
-    DOE_HIP(hipMemcpy(h.data(), B->hist + c.hist_off + size_t(d) * 2 * B->hist_k,
+    std::vector<double> h(3 * size_t(B->hist_k));
+    DOE_HIP(hipMemcpy(h.data(), B->hist + c.hist_off + size_t(d) * 3 * B->hist_k,
                       sizeof(double) * h.size(), hipMemcpyDeviceToHost));
     if (error_out) std::copy(h.begin(), h.begin() + K, error_out);
     if (efficiency_out) std::copy(h.begin() + B->hist_k, h.begin() + B->hist_k + K, efficiency_out);
   }
   if (best_rmse_out) *best_rmse_out = weighted ? st.best_rmse : std::nan("");
+  return DOE_OK;
+}
+
+int doe_gs_batch_get_screen(doe_gs_batch* B, int index, double* screen_rmse_out) {
+  DOE_REQUIRE(B && screen_rmse_out, DOE_ERR_INVALID, "doe_gs_batch_get_screen: null argument");
+  DOE_REQUIRE(index >= 0 && index < B->count, DOE_ERR_INVALID,
+              "doe_gs_batch_get_screen: index out of range");
+  DOE_HIP(hipSetDevice(B->ctx->device));
+  DOE_HIP(hipStreamSynchronize(B->ctx->stream));
+  const auto& c = B->cls[B->where[index].first];
+  const int d = B->where[index].second;
+  if (B->last_k > 0)
+    DOE_HIP(hipMemcpy(screen_rmse_out,
+                      B->hist + c.hist_off + size_t(d) * 3 * B->hist_k + 2 * size_t(B->hist_k),
+                      sizeof(double) * B->last_k, hipMemcpyDeviceToHost));
   return DOE_OK;
 }
 

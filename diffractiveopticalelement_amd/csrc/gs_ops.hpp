@@ -34,6 +34,7 @@ struct GsBuf {
   GsState* state;
   double* err_hist;
   double* eff_hist;
+  double* scr_hist;  // far field: screen rmse per iteration (nullable)
   int nx, ny;
   double dxdy_x, dxdy_y;  // dx, dy for the image-plane efficiency expression
   double norm;            // far field: 1/sqrt(nx*ny)
@@ -50,6 +51,12 @@ struct GsBuf {
 
 // reduction slots
 enum { R_ERRNUM, R_ERRDEN, R_INROI, R_TOTAL, R_RNUM, R_RDEN, R_WSUM, R_TSUM, R_N };
+// far field also accumulates the screen metric of the measured plane
+// (projector.js:554-614 screenMetrics rmse: target irradiance T power-matched
+// to the plane's ROI power, intensity RMS over the ROI): with I = a^2,
+// T = t0^2 and s = sum(m*I) / sum(T), rmse^2 = (sum I^2 - 2 s sum I T +
+// s^2 sum T^2) / (s^2 sum T^2) over m > 0.
+enum { R_SI2 = R_N, R_SIT, R_ST2, R_TALL, R_NFF };
 
 template <typename T>
 __device__ __forceinline__ T clampv(T v, T lo, T hi) {
@@ -104,7 +111,7 @@ struct OpFFInit {
 // 490-521) on the centred-index spectrum.
 template <typename T>
 struct OpFFB {
-  static constexpr int NACC = R_N;
+  static constexpr int NACC = R_NFF;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
   // 512-thread tiles of 32 points per thread with both LDS planes: one
   // workgroup per CU, but a 256-register budget instead of spilling at 128
@@ -127,9 +134,14 @@ struct OpFFB {
     T w = T(1);
     if (b.weighted) w = b.w[k] * T(st->nrm);
     const T t = t0 * w;
-    acc[R_TOTAL] += double(a) * double(a);
+    const double I = double(a) * double(a), Tt = double(t0) * double(t0);
+    acc[R_TOTAL] += I;
+    acc[R_TALL] += Tt;
     if (m > T(0)) {
-      acc[R_INROI] += double(a) * double(a) * double(m);
+      acc[R_INROI] += I * double(m);
+      acc[R_SI2] += I * I;
+      acc[R_SIT] += I * Tt;
+      acc[R_ST2] += Tt * Tt;
       const double d = (double(a) - double(t)) * double(m);
       acc[R_ERRNUM] += d * d;
       acc[R_ERRDEN] += double(t) * double(t);
@@ -163,6 +175,12 @@ struct OpFFB {
     GsState* st = b.state + f;
     double* err_hist = b.err_hist + f * b.hstride;
     double* eff_hist = b.eff_hist + f * b.hstride;
+    if (b.scr_hist) {
+      const double sc = s[R_TALL] > 0 ? s[R_INROI] / s[R_TALL] : 0.0;
+      const double den = sc * sc * s[R_ST2];
+      const double num = s[R_SI2] - 2 * sc * s[R_SIT] + sc * sc * s[R_ST2];
+      b.scr_hist[f * b.hstride + it] = den > 0 ? sqrt(fmax(num, 0.0) / den) : 0.0;
+    }
     const double err = s[R_ERRDEN] > 0 ? sqrt(s[R_ERRNUM] / s[R_ERRDEN]) : 0.0;
     const double eff = s[R_TOTAL] > 0 ? s[R_INROI] / s[R_TOTAL] : 0.0;
     if (b.weighted) {

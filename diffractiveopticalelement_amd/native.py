@@ -103,6 +103,20 @@ SIGNATURES = [
     ("doe_gs_batch_run", C.c_int, [_VP, C.c_int, C.c_int]),
     ("doe_gs_batch_get", C.c_int, [_VP, C.c_int, c_double_p, c_double_p, c_double_p, c_double_p,
                                    c_double_p, c_double_p]),
+    ("doe_gs_batch_get_screen", C.c_int, [_VP, C.c_int, c_double_p]),
+    ("doe_wsnet_create", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(_VP)]),
+    ("doe_wsnet_destroy", C.c_int, [_VP]),
+    ("doe_wsnet_info", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    ("doe_wsnet_param_count", C.c_int, [_VP, C.POINTER(C.c_long)]),
+    ("doe_wsnet_get_params", C.c_int, [_VP, _VP]),
+    ("doe_wsnet_set_params", C.c_int, [_VP, _VP]),
+    ("doe_wsnet_forward", C.c_int, [_VP, C.c_int, _VP, _VP]),
+    ("doe_wsnet_loss_grad", C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, c_double_p, _VP]),
+    ("doe_wsnet_adam_step", C.c_int, [_VP, C.c_float]),
+    ("doe_gs_batch_design_order", C.c_int, [_VP, C.POINTER(C.c_int)]),
+    ("doe_warmstart_inputs", C.c_int, [_VP, C.c_int, _VP]),
+    ("doe_warmstart_labels", C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
 ]
 
 GSB_GRAPH = 1
@@ -349,9 +363,79 @@ class GsBatch:
         out["rmse"] = float(out["rmse"][0])
         return out
 
+    def design_order(self):
+        arr = (C.c_int * len(self.sides))()
+        check(self.ctx.lib.doe_gs_batch_design_order(self.h, arr))
+        return list(arr)
+
+    def warm_inputs(self, res: int, input_ptr: int) -> None:
+        check(self.ctx.lib.doe_warmstart_inputs(self.h, res, input_ptr))
+
+    def warm_labels(self, res: int, lre_ptr: int, lim_ptr: int, w_ptr: int) -> None:
+        check(self.ctx.lib.doe_warmstart_labels(self.h, res, lre_ptr, lim_ptr, w_ptr))
+
+    def warm_apply(self, net: "WsNet", scratch_ptr: int) -> None:
+        check(self.ctx.lib.doe_warmstart_apply(self.h, net.h, scratch_ptr))
+
+    def get_screen(self, index: int):
+        out = np.zeros(self.iterations)
+        check(self.ctx.lib.doe_gs_batch_get_screen(self.h, int(index), _dptr(out)))
+        return out
+
     def close(self) -> None:
         if self.h:
             self.ctx.lib.doe_gs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WsNet:
+    """The warm-start network (device-pointer interface of doe_wsnet_*)."""
+
+    def __init__(self, ctx: Context, res: int = 256, base: int = 32, max_batch: int = 8,
+                 seed: int = 0):
+        self.ctx = ctx
+        h = _VP()
+        check(ctx.lib.doe_wsnet_create(ctx.h, res, base, max_batch, C.c_ulonglong(seed),
+                                       C.byref(h)))
+        self.h = h
+        self.res, self.base, self.max_batch = res, base, max_batch
+        n = C.c_long()
+        check(ctx.lib.doe_wsnet_param_count(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.nparams, dtype=np.float32)
+        check(self.ctx.lib.doe_wsnet_get_params(self.h, out.ctypes.data))
+        return out
+
+    def set_params(self, p) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == self.nparams
+        check(self.ctx.lib.doe_wsnet_set_params(self.h, a.ctypes.data))
+
+    def forward(self, batch: int, input_ptr: int, output_ptr: int) -> None:
+        check(self.ctx.lib.doe_wsnet_forward(self.h, batch, input_ptr, output_ptr))
+
+    def loss_grad(self, batch, input_ptr, lre_ptr, lim_ptr, w_ptr, want_grad=False):
+        loss = C.c_double()
+        g = np.zeros(self.nparams, dtype=np.float32) if want_grad else None
+        check(self.ctx.lib.doe_wsnet_loss_grad(self.h, batch, input_ptr, lre_ptr, lim_ptr, w_ptr,
+                                               C.byref(loss),
+                                               g.ctypes.data if g is not None else None))
+        return loss.value, g
+
+    def adam_step(self, lr: float) -> None:
+        check(self.ctx.lib.doe_wsnet_adam_step(self.h, C.c_float(lr)))
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx.lib.doe_wsnet_destroy(self.h)
             self.h = None
 
     def __del__(self):

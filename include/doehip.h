@@ -212,6 +212,50 @@ int doe_gs_batch_run(doe_gs_batch* batch, int iterations, int flags);
 int doe_gs_batch_get(doe_gs_batch* batch, int index, double* phase_out, double* error_out,
                      double* efficiency_out, double* amplitude_out, double* weights_out,
                      double* best_rmse_out);
+/* Per-iteration screen rmse of a design (iterations of the last run): the
+ * measured far-field irradiance I of each iteration judged as
+ * ProjectorSystem.screenMetrics does (projector.js:554-614): the target
+ * irradiance T = t^2 power-matched to the plane's ROI power, s = sum(I*roi) /
+ * sum(T), rmse = sqrt(sum_{roi>0} (I - sT)^2 / sum_{roi>0} (sT)^2). */
+int doe_gs_batch_get_screen(doe_gs_batch* batch, int index, double* screen_rmse_out);
+
+/* Learned warm start (no counterpart in the reference, whose designs start
+ * from a random or flat phase: doe.js:651, 850-870).  A convolutional
+ * encoder-decoder (3x3 convolutions, additive skips, ReLU; BF16 weights and
+ * activations, FP32 accumulation on the matrix cores; 1.75 M parameters at
+ * base width 32) maps a target irradiance pooled to res x res to a
+ * target-plane phasor field (2 channels, NHWC).  All pointers below are
+ * device pointers: input batch*res*res floats, output/label planes
+ * batch*res*res (output interleaved re, im).  doe_wsnet_loss_grad runs the
+ * forward pass, the phase-aligned, conjugate-aware weighted loss and the
+ * backward pass (gradients kept for doe_wsnet_adam_step; copied to grad_host
+ * if non-NULL); parameters are FP32 masters with BF16 copies. */
+typedef struct doe_wsnet doe_wsnet;
+int doe_wsnet_create(doe_ctx* ctx, int res, int base, int max_batch, unsigned long long seed,
+                     doe_wsnet** out);
+int doe_wsnet_destroy(doe_wsnet* net);
+int doe_wsnet_info(doe_wsnet* net, int* res, int* max_batch);
+int doe_wsnet_param_count(doe_wsnet* net, long* n);
+int doe_wsnet_get_params(doe_wsnet* net, float* host);
+int doe_wsnet_set_params(doe_wsnet* net, const float* host);
+int doe_wsnet_forward(doe_wsnet* net, int batch, const float* input, float* output);
+int doe_wsnet_loss_grad(doe_wsnet* net, int batch, const float* input, const float* label_re,
+                        const float* label_im, const float* weight, double* loss,
+                        float* grad_host);
+int doe_wsnet_adam_step(doe_wsnet* net, float lr);
+
+/* Hand-off between a far-field batch and the network (designs in schedule
+ * order, see doe_gs_batch_design_order; every side a multiple of res):
+ * network inputs from the targets; training labels (pooled target-plane
+ * phasor of each design's current best phase, and the pooled target
+ * amplitude as weight); and doe_warmstart_apply, which replaces every
+ * design's initial phase by arg(F^-1(t * prediction)) (scratch: device
+ * buffer of count*res*res*3 floats). */
+int doe_gs_batch_design_order(doe_gs_batch* batch, int* order);
+int doe_warmstart_inputs(doe_gs_batch* batch, int res, float* input);
+int doe_warmstart_labels(doe_gs_batch* batch, int res, float* label_re, float* label_im,
+                         float* weight);
+int doe_warmstart_apply(doe_gs_batch* batch, doe_wsnet* net, float* scratch);
 
 #ifdef __cplusplus
 }

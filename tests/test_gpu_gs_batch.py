@@ -139,3 +139,29 @@ def test_batch_argument_errors(ctx):
     with pytest.raises(nat.DoeError, match="out of range"):
         b.set(1, ill, t, roi)
     b.close()
+
+
+def test_screen_rmse_history_matches_numpy(ctx):
+    # iteration 0 measures the far field of illum * exp(i phi0): judge it as
+    # screenMetrics does (target power-matched to the ROI power)
+    sides = [128, 256]
+    b = nat.GsBatch(ctx, sides, nat.ALG_WGS, 0.9, with_roi=True)
+    probs = []
+    for i, n in enumerate(sides):
+        ill, t, roi, phi0 = problem(n, 8e-6, 40 + i, soft_roi=False)
+        b.set(i, ill, t, roi, phi0)
+        probs.append((ill, t, roi, phi0))
+    b.run(3, graph=False)
+    for i, n in enumerate(sides):
+        ill, t, roi, phi0 = probs[i]
+        x = ill * np.exp(1j * phi0)
+        Y = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(x))) / n
+        irr = np.abs(Y) ** 2
+        T = t ** 2
+        s = np.sum(irr * roi) / np.sum(T)
+        m = roi > 0
+        ref = np.sqrt(np.sum((irr[m] - s * T[m]) ** 2) / np.sum((s * T[m]) ** 2))
+        scr = b.get_screen(i)
+        assert abs(scr[0] - ref) < 1e-5 * max(ref, 1.0), (scr[0], ref)
+        assert np.all(np.isfinite(scr))
+    b.close()
