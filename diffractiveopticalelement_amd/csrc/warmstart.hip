@@ -225,28 +225,15 @@ int doe_warmstart_labels(doe_gs_batch* B, int res, float* label_re, float* label
   return DOE_OK;
 }
 
-// Initial phases of every design from the network's prediction (replaces the
-// designs' initial phases; the next doe_gs_batch_run starts from them).
-// `scratch` holds count * res^2 * 3 floats (inputs and predictions).
-int doe_warmstart_apply(doe_gs_batch* B, doe_wsnet* net, float* scratch) {
-  DOE_REQUIRE(B && net && scratch, DOE_ERR_INVALID, "null argument");
-  int res = 0, mb = 0;
-  int rc = doe_wsnet_info(net, &res, &mb);
-  if (rc) return rc;
-  rc = check_sides(B, res);
-  if (rc) return rc;
-  DOE_HIP(hipSetDevice(B->ctx->device));
+}  // extern "C"
+
+namespace doe {
+namespace {
+// initial phases from target-plane phasors z (count * res^2 * 2, schedule order)
+int ws_apply_phasor(doe_gs_batch* B, int res, const float* z) {
   hipStream_t st = B->ctx->stream;
   const long per = long(res) * res;
-  float* in = scratch;
-  float* z = scratch + per * B->count;
-  rc = ws_inputs(B, res, in);
-  if (rc) return rc;
-  for (int d0 = 0; d0 < B->count; d0 += mb) {
-    const int nb = std::min(mb, B->count - d0);
-    rc = doe_wsnet_forward(net, nb, in + per * d0, z + per * 2 * d0);
-    if (rc) return rc;
-  }
+  int rc = DOE_OK;
   long o = 0;
   for (const auto& c : B->cls) {
     const int nd = int(c.users.size());
@@ -261,6 +248,50 @@ int doe_warmstart_apply(doe_gs_batch* B, doe_wsnet* net, float* scratch) {
     o += nd;
   }
   DOE_HIP(hipGetLastError());
+  return DOE_OK;
+}
+}  // namespace
+}  // namespace doe
+
+extern "C" {
+
+// Initial phases of every design from the network's prediction (replaces the
+// designs' initial phases; the next doe_gs_batch_run starts from them).
+// `scratch` holds count * res^2 * 3 floats (inputs and predictions).
+int doe_warmstart_apply(doe_gs_batch* B, doe_wsnet* net, float* scratch) {
+  DOE_REQUIRE(B && net && scratch, DOE_ERR_INVALID, "null argument");
+  int res = 0, mb = 0;
+  int rc = doe_wsnet_info(net, &res, &mb);
+  if (rc) return rc;
+  rc = check_sides(B, res);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(B->ctx->device));
+  const long per = long(res) * res;
+  float* in = scratch;
+  float* z = scratch + per * B->count;
+  rc = ws_inputs(B, res, in);
+  if (rc) return rc;
+  for (int d0 = 0; d0 < B->count; d0 += mb) {
+    const int nb = std::min(mb, B->count - d0);
+    rc = doe_wsnet_forward(net, nb, in + per * d0, z + per * 2 * d0);
+    if (rc) return rc;
+  }
+  rc = ws_apply_phasor(B, res, z);
+  if (rc) return rc;
+  DOE_HIP(finish(B->ctx));
+  return DOE_OK;
+}
+
+// Initial phases from given target-plane phasors (device, count*res^2*2
+// floats interleaved re, im, schedule order): the same hand-off without a
+// network (e.g. a quadratic target-plane phase).
+int doe_warmstart_apply_phasor(doe_gs_batch* B, int res, const float* z) {
+  DOE_REQUIRE(B && z, DOE_ERR_INVALID, "null argument");
+  int rc = check_sides(B, res);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(B->ctx->device));
+  rc = ws_apply_phasor(B, res, z);
+  if (rc) return rc;
   DOE_HIP(finish(B->ctx));
   return DOE_OK;
 }

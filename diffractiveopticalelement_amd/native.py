@@ -117,6 +117,7 @@ SIGNATURES = [
     ("doe_warmstart_inputs", C.c_int, [_VP, C.c_int, _VP]),
     ("doe_warmstart_labels", C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
+    ("doe_warmstart_apply_phasor", C.c_int, [_VP, C.c_int, _VP]),
 ]
 
 GSB_GRAPH = 1
@@ -376,6 +377,9 @@ class GsBatch:
 
     def warm_apply(self, net: "WsNet", scratch_ptr: int) -> None:
         check(self.ctx.lib.doe_warmstart_apply(self.h, net.h, scratch_ptr))
+
+    def warm_apply_phasor(self, res: int, phasor_ptr: int) -> None:
+        check(self.ctx.lib.doe_warmstart_apply_phasor(self.h, res, phasor_ptr))
 
     def get_screen(self, index: int):
         out = np.zeros(self.iterations)

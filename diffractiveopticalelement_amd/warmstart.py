@@ -1,0 +1,294 @@
+"""Learned warm start for batched far-field WGS: target families, training
+pairs from the batched engine, training, and the warm-vs-plain comparison.
+
+Target families (seeded, square side n, no counterpart in the reference; the
+reference's own text/disc/spot targets live in js/projector.js):
+  spots   4..64 Gaussian spots (sigma 1.5 px), uniform weights
+  blobs   2..8 Gaussian blobs, sigma 0.01..0.04 n
+  lines   1..6 line segments, 3 px wide
+  glyphs  2..5 characters of the 5x7 glyph table, block-scaled
+Each is an irradiance I >= 0 inside the central half of the plane; the
+design target is t = sqrt(I) power-matched to the illumination (a top-hat
+aperture of radius 0.45 n with a 2 px raised-cosine edge), and the ROI is
+the support I > 0.05 max, dilated by 2 px.
+
+Usage (GPU):
+    python -m diffractiveopticalelement_amd.warmstart --train 512 --steps 1500 --eval 32
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import native as nat
+
+# 5x7 glyphs as row bit masks (bit 4 = leftmost column), for the glyph family
+_GLYPHS = {
+    "A": [14, 17, 17, 31, 17, 17, 17], "B": [30, 17, 17, 30, 17, 17, 30],
+    "C": [14, 17, 16, 16, 16, 17, 14], "D": [30, 17, 17, 17, 17, 17, 30],
+    "E": [31, 16, 16, 30, 16, 16, 31], "F": [31, 16, 16, 30, 16, 16, 16],
+    "H": [17, 17, 17, 31, 17, 17, 17], "L": [16, 16, 16, 16, 16, 16, 31],
+    "O": [14, 17, 17, 17, 17, 17, 14], "P": [30, 17, 17, 30, 16, 16, 16],
+    "T": [31, 4, 4, 4, 4, 4, 4], "X": [17, 17, 10, 4, 10, 17, 17],
+}
+FAMILIES = ("spots", "blobs", "lines", "glyphs")
+
+
+def illumination(n: int) -> np.ndarray:
+    c = np.arange(n) - n / 2 + 0.5
+    r = np.hypot(c[None, :], c[:, None])
+    rad, soft = 0.45 * n, 2.0
+    v = 0.5 * (1 + np.cos(np.pi * (r - rad + soft) / soft))
+    return np.where(r <= rad - soft, 1.0, np.where(r >= rad, 0.0, v))
+
+
+def make_target(kind: str, n: int, rng: np.random.Generator) -> np.ndarray:
+    """irradiance of one target of family `kind` (float64, n x n)"""
+    I = np.zeros((n, n))
+    lo, hi = n // 4, 3 * n // 4
+    if kind == "spots":
+        k = int(rng.integers(4, 65))
+        ys = rng.integers(lo + 4, hi - 4, k)
+        xs = rng.integers(lo + 4, hi - 4, k)
+        d = np.arange(-4, 5)
+        g = np.exp(-(d[None, :] ** 2 + d[:, None] ** 2) / (2 * 1.5 ** 2))
+        for y, x in zip(ys, xs):
+            I[y - 4:y + 5, x - 4:x + 5] += g
+    elif kind == "blobs":
+        k = int(rng.integers(2, 9))
+        yy, xx = np.mgrid[0:n, 0:n]
+        for _ in range(k):
+            s = rng.uniform(0.01, 0.04) * n
+            cy, cx = rng.uniform(lo + 2 * s, hi - 2 * s, 2)
+            r0, r1 = int(max(0, cy - 4 * s)), int(min(n, cy + 4 * s + 1))
+            c0, c1 = int(max(0, cx - 4 * s)), int(min(n, cx + 4 * s + 1))
+            I[r0:r1, c0:c1] += rng.uniform(0.5, 1.0) * np.exp(
+                -((yy[r0:r1, c0:c1] - cy) ** 2 + (xx[r0:r1, c0:c1] - cx) ** 2) / (2 * s * s))
+    elif kind == "lines":
+        k = int(rng.integers(1, 7))
+        yy, xx = np.mgrid[lo:hi, lo:hi].astype(float)
+        sub = np.zeros((hi - lo, hi - lo))
+        for _ in range(k):
+            p = rng.uniform(lo + 8, hi - 8, 2)
+            q = rng.uniform(lo + 8, hi - 8, 2)
+            d = q - p
+            t = np.clip(((yy - p[0]) * d[0] + (xx - p[1]) * d[1]) / max(d @ d, 1e-9), 0, 1)
+            dist = np.hypot(yy - (p[0] + t * d[0]), xx - (p[1] + t * d[1]))
+            sub = np.maximum(sub, (dist <= 1.5).astype(float))
+        I[lo:hi, lo:hi] = sub
+    elif kind == "glyphs":
+        text = "".join(rng.choice(sorted(_GLYPHS), int(rng.integers(2, 6))))
+        raster = np.zeros((7, 6 * len(text) - 1))
+        for i, ch in enumerate(text):
+            for r, bits in enumerate(_GLYPHS[ch]):
+                for c in range(5):
+                    if bits >> (4 - c) & 1:
+                        raster[r, 6 * i + c] = 1.0
+        s = max(1, int((hi - lo) * 0.9 / raster.shape[1]))
+        big = np.kron(raster, np.ones((s, s)))
+        y0 = (n - big.shape[0]) // 2
+        x0 = (n - big.shape[1]) // 2
+        I[y0:y0 + big.shape[0], x0:x0 + big.shape[1]] = big
+    else:
+        raise ValueError(kind)
+    return I
+
+
+def design_inputs(I: np.ndarray, ill: np.ndarray):
+    """target amplitude (power-matched to the illumination) and binary ROI"""
+    t = np.sqrt(I)
+    t *= np.sqrt(np.sum(ill ** 2) / np.sum(t ** 2))
+    roi = I > 0.05 * I.max()
+    for _ in range(2):
+        p = np.pad(roi, 1)
+        roi = (p[1:-1, 1:-1] | p[:-2, 1:-1] | p[2:, 1:-1] | p[1:-1, :-2] | p[1:-1, 2:])
+    return t, roi.astype(float)
+
+
+def fill_batch(ctx, count: int, n: int, seed: int, iters_unused=None):
+    rng = np.random.default_rng(seed)
+    b = nat.GsBatch(ctx, [n] * count, nat.ALG_WGS, 0.9, with_roi=True)
+    ill = illumination(n)
+    kinds = []
+    for i in range(count):
+        kind = FAMILIES[i % len(FAMILIES)]
+        t, roi = design_inputs(make_target(kind, n, rng), ill)
+        b.set(i, ill, t, roi)
+        kinds.append(kind)
+    return b, kinds
+
+
+def quadratic_phasors(x: np.ndarray, n: int, spread: float = 0.25) -> np.ndarray:
+    """Target-plane phasors exp(i a |u - u0|^2) per pooled input x[d] (res x res),
+    u0 the irradiance centroid, a chosen so the target's RMS radius r maps to
+    spread * n in the element plane (a chirp of curvature a moves a target
+    point at radius r to element radius a r n / pi).  Returns
+    (count, res, res, 2) float32."""
+    count, res, _ = x.shape
+    f = n / res
+    v, u = np.mgrid[0:res, 0:res].astype(float)
+    out = np.zeros((count, res, res, 2), np.float32)
+    for d in range(count):
+        w = x[d] / max(x[d].sum(), 1e-30)
+        cu, cv = (w * u).sum(), (w * v).sum()
+        r2 = (w * ((u - cu) ** 2 + (v - cv) ** 2)).sum()
+        r = max(np.sqrt(r2) * f, 4.0)  # full-resolution pixels
+        a = spread * np.pi / r  # rad per full-resolution pixel^2
+        ph = a * f * f * ((u - cu) ** 2 + (v - cv) ** 2)
+        out[d, ..., 0] = np.cos(ph)
+        out[d, ..., 1] = np.sin(ph)
+    return out
+
+
+def start_quadratic(ctx, b, count: int, n: int, res: int):
+    """replace every design's initial phase by the quadratic target-plane start"""
+    inp = ctx.field(res, res, count, "f32")
+    b.warm_inputs(res, inp.ptrs()[0])
+    x, _ = inp.download()
+    z = quadratic_phasors(x, n)
+    zf = ctx.field(2 * res, res, count, "f32")
+    zf.upload(z.reshape(count, res, 2 * res), np.zeros((count, res, 2 * res)))
+    b.warm_apply_phasor(res, zf.ptrs()[0])
+    inp.free()
+    zf.free()
+
+
+def make_pairs(ctx, count: int, n: int, res: int, iters: int, seed: int, chunk: int = 64):
+    """(inputs, label_re, label_im, weights) of `count` designs, res x res each;
+    labels are the converged target-plane phases of WGS started from the
+    quadratic target-plane phase (smooth, vortex-free starts give smooth,
+    learnable solutions)"""
+    X, LR, LI, W = [], [], [], []
+    for c0 in range(0, count, chunk):
+        m = min(chunk, count - c0)
+        b, _ = fill_batch(ctx, m, n, seed + c0)
+        start_quadratic(ctx, b, m, n, res)
+        b.run(iters, graph=True)
+        inp = ctx.field(res, res, m, "f32")
+        lab = ctx.field(res, res, m, "f32")
+        wts = ctx.field(res, res, m, "f32")
+        b.warm_inputs(res, inp.ptrs()[0])
+        b.warm_labels(res, lab.ptrs()[0], lab.ptrs()[1], wts.ptrs()[0])
+        x, _ = inp.download()
+        lr, li = lab.download()
+        w, _ = wts.download()
+        X.append(x)
+        LR.append(lr)
+        LI.append(li)
+        W.append(w)
+        for f in (inp, lab, wts):
+            f.free()
+        b.close()
+    return np.concatenate(X), np.concatenate(LR), np.concatenate(LI), np.concatenate(W)
+
+
+def train(ctx, data, res: int, base: int = 32, batch: int = 8, steps: int = 1000,
+          lr: float = 1e-3, seed: int = 0, log=print):
+    X, LR, LI, W = data
+    net = nat.WsNet(ctx, res, base, batch, seed)
+    inp = ctx.field(res, res, batch, "f32")
+    lab = ctx.field(res, res, batch, "f32")
+    wts = ctx.field(res, res, batch, "f32")
+    rng = np.random.default_rng(seed)
+    z = np.zeros((batch, res, res))
+    hist = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        idx = rng.choice(len(X), batch, replace=False)
+        inp.upload(X[idx], z)
+        lab.upload(LR[idx], LI[idx])
+        wts.upload(W[idx], z)
+        loss, _ = net.loss_grad(batch, inp.ptrs()[0], lab.ptrs()[0], lab.ptrs()[1], wts.ptrs()[0])
+        step_lr = lr * min(1.0, (s + 1) / 50) * (0.5 * (1 + np.cos(np.pi * s / steps)))
+        net.adam_step(step_lr)
+        hist.append(loss)
+        if s % max(1, steps // 10) == 0 or s == steps - 1:
+            log(f"  step {s}: loss {np.mean(hist[-20:]):.4f} ({time.perf_counter() - t0:.1f} s)")
+    for f in (inp, lab, wts):
+        f.free()
+    return net, hist
+
+
+def first_below(h, thr):
+    k = np.nonzero(np.asarray(h) <= thr)[0]
+    return int(k[0]) + 1 if k.size else None
+
+
+def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0.01, log=print):
+    """plain (random start) vs warm (network start) WGS on the same targets"""
+    out = {}
+    for mode in ("plain", "quadratic", "warm"):
+        b, kinds = fill_batch(ctx, count, n, seed)
+        t_inf = 0.0
+        if mode == "plain":
+            b.random_init(seed)
+        elif mode == "quadratic":
+            start_quadratic(ctx, b, count, n, net.res)
+        else:
+            scratch = ctx.field(net.res, net.res, 4 * count, "f32")  # >= 3 count res^2 floats
+            ctx.sync()
+            ctx.timer_start()
+            b.warm_apply(net, scratch.ptrs()[0])
+            t_inf = ctx.timer_stop()
+            scratch.free()
+        ctx.timer_start()
+        b.run(iters, graph=True)
+        ms = ctx.timer_stop()
+        scr = np.array([b.get_screen(i) for i in range(count)])
+        its = [first_below(h, thr) for h in scr]
+        per_iter = ms / iters / count
+        out[mode] = {
+            "ms_batch": ms, "ms_per_design_iteration": per_iter, "inference_ms_batch": t_inf,
+            "iterations_to_thr": its, "final_screen_rmse": scr[:, -1].tolist(),
+            "best_screen_rmse": scr.min(axis=1).tolist(), "kinds": kinds,
+            "screen_rmse_median": np.median(scr, axis=0)[[0, 9, 49, 99, iters - 1]].tolist()
+            if iters >= 100 else None,
+        }
+        b.close()
+        reached = [k for k in its if k is not None]
+        log(f"  {mode}: {len(reached)}/{count} designs reach screen rmse <= {thr}; median "
+            f"iterations {np.median(reached) if reached else None}; final rmse median "
+            f"{np.median(scr[:, -1]):.4f}; {per_iter * 1e3:.2f} us per design-iteration")
+        for kind in FAMILIES:
+            sel = [i for i, k in enumerate(kinds) if k == kind]
+            log(f"    {kind}: iterations {[its[i] for i in sel]}, screen rmse at 10/50/final "
+                f"{np.round(np.median(scr[sel][:, [9, 49, -1]], axis=0), 4).tolist()}")
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--res", type=int, default=256)
+    ap.add_argument("--train", type=int, default=256, help="training designs")
+    ap.add_argument("--label-iters", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--eval", type=int, default=32)
+    ap.add_argument("--eval-iters", type=int, default=300)
+    ap.add_argument("--threshold", type=float, default=0.01)
+    ap.add_argument("--out", default=None, help="write the result JSON here")
+    args = ap.parse_args(argv)
+    ctx = nat.Context(0)
+    t0 = time.perf_counter()
+    print(f"pairs: {args.train} designs at {args.n}^2, {args.label_iters} WGS iterations", flush=True)
+    data = make_pairs(ctx, args.train, args.n, args.res, args.label_iters, seed=1000)
+    print(f"  {time.perf_counter() - t0:.1f} s", flush=True)
+    print(f"training {args.steps} steps", flush=True)
+    net, hist = train(ctx, data, args.res, steps=args.steps,
+                      log=lambda s: print(s, flush=True))
+    print("evaluation", flush=True)
+    res = evaluate(ctx, net, args.eval, args.n, args.eval_iters, seed=99999, thr=args.threshold,
+                   log=lambda s: print(s, flush=True))
+    res["train_loss_first_last"] = [float(np.mean(hist[:20])), float(np.mean(hist[-20:]))]
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

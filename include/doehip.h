@@ -256,6 +256,10 @@ int doe_warmstart_inputs(doe_gs_batch* batch, int res, float* input);
 int doe_warmstart_labels(doe_gs_batch* batch, int res, float* label_re, float* label_im,
                          float* weight);
 int doe_warmstart_apply(doe_gs_batch* batch, doe_wsnet* net, float* scratch);
+/* The same hand-off from given target-plane phasors (device, count*res*res*2
+ * floats interleaved re, im, schedule order) instead of a network, e.g. a
+ * quadratic target-plane phase. */
+int doe_warmstart_apply_phasor(doe_gs_batch* batch, int res, const float* phasor);
 
 #ifdef __cplusplus
 }
