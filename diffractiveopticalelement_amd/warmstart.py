@@ -7,7 +7,8 @@ reference's own text/disc/spot targets live in js/projector.js):
   blobs   2..8 Gaussian blobs, sigma 0.01..0.04 n
   lines   1..6 line segments, 3 px wide
   glyphs  2..5 characters of the 5x7 glyph table, block-scaled
-Each is an irradiance I >= 0 inside the central half of the plane; the
+Each is an irradiance I >= 0 inside the central half of the plane (blob
+tails reach up to 2 sigma past it); the
 design target is t = sqrt(I) power-matched to the illumination (a top-hat
 aperture of radius 0.45 n with a 2 px raised-cosine edge), and the ROI is
 the support I > 0.05 max, dilated by 2 px.
@@ -144,14 +145,20 @@ def quadratic_phasors(x: np.ndarray, n: int, spread: float = 0.25) -> np.ndarray
     return out
 
 
+def _phasor_field(ctx, z: np.ndarray):
+    """device copy of (count, res, res, 2) interleaved phasors"""
+    count, res = z.shape[0], z.shape[1]
+    zf = ctx.field(2 * res, res, count, "f32")
+    zf.upload(z.reshape(count, res, 2 * res), np.zeros((count, res, 2 * res)))
+    return zf
+
+
 def start_quadratic(ctx, b, count: int, n: int, res: int):
     """replace every design's initial phase by the quadratic target-plane start"""
     inp = ctx.field(res, res, count, "f32")
     b.warm_inputs(res, inp.ptrs()[0])
     x, _ = inp.download()
-    z = quadratic_phasors(x, n)
-    zf = ctx.field(2 * res, res, count, "f32")
-    zf.upload(z.reshape(count, res, 2 * res), np.zeros((count, res, 2 * res)))
+    zf = _phasor_field(ctx, quadratic_phasors(x, n))
     b.warm_apply_phasor(res, zf.ptrs()[0])
     inp.free()
     zf.free()
@@ -159,9 +166,9 @@ def start_quadratic(ctx, b, count: int, n: int, res: int):
 
 def make_pairs(ctx, count: int, n: int, res: int, iters: int, seed: int, chunk: int = 64):
     """(inputs, label_re, label_im, weights) of `count` designs, res x res each;
-    labels are the converged target-plane phases of WGS started from the
-    quadratic target-plane phase (smooth, vortex-free starts give smooth,
-    learnable solutions)"""
+    labels are the best target-plane phases of `iters` WGS iterations started
+    from the quadratic target-plane phase, a deterministic function of the
+    target (a random start would give every target many unrelated labels)"""
     X, LR, LI, W = [], [], [], []
     for c0 in range(0, count, chunk):
         m = min(chunk, count - c0)
@@ -218,37 +225,76 @@ def first_below(h, thr):
     return int(k[0]) + 1 if k.size else None
 
 
-def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0.01, log=print):
-    """plain (random start) vs warm (network start) WGS on the same targets"""
+def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0.01, log=print,
+             modes=("plain", "quadratic", "warm"), label_res=()):
+    """WGS from several starts on the same `count` targets.
+
+    plain      random element phase (the method without the network)
+    quadratic  the quadratic target-plane phase of start_quadratic
+    warm       the network's target-plane phase
+    labels@r   each design's own best plain-run far-field phase, pooled to r^2
+               and handed off like the network's output: what a network that
+               reproduced its training labels exactly at resolution r would
+               give (an upper bound for the learned start at that resolution)
+
+    Per design: the screen rmse history, the first iteration at or below
+    `thr`, the best rmse, and for every mode but plain the iterations to the
+    common threshold max(thr, best plain, best of the mode) in both runs."""
     out = {}
-    for mode in ("plain", "quadratic", "warm"):
+    labels = {}
+    scr_plain = None
+    order = ["plain"] + [m for m in modes if m != "plain"]
+    for r in label_res:
+        order.append(f"labels@{r}")
+    for mode in order:
         b, kinds = fill_batch(ctx, count, n, seed)
         t_inf = 0.0
         if mode == "plain":
             b.random_init(seed)
         elif mode == "quadratic":
             start_quadratic(ctx, b, count, n, net.res)
-        else:
+        elif mode == "warm":
             scratch = ctx.field(net.res, net.res, 4 * count, "f32")  # >= 3 count res^2 floats
             ctx.sync()
             ctx.timer_start()
             b.warm_apply(net, scratch.ptrs()[0])
             t_inf = ctx.timer_stop()
             scratch.free()
+        else:
+            r = int(mode.split("@")[1])
+            zf = _phasor_field(ctx, labels[r])
+            b.warm_apply_phasor(r, zf.ptrs()[0])
+            zf.free()
         ctx.timer_start()
         b.run(iters, graph=True)
         ms = ctx.timer_stop()
         scr = np.array([b.get_screen(i) for i in range(count)])
+        if mode == "plain":
+            scr_plain = scr
+            for r in label_res:
+                lab = ctx.field(r, r, count, "f32")
+                wts = ctx.field(r, r, count, "f32")
+                b.warm_labels(r, lab.ptrs()[0], lab.ptrs()[1], wts.ptrs()[0])
+                lr_, li_ = lab.download()
+                labels[r] = np.stack([lr_, li_], axis=-1).astype(np.float32)
+                lab.free()
+                wts.free()
+        b.close()
         its = [first_below(h, thr) for h in scr]
         per_iter = ms / iters / count
-        out[mode] = {
+        rec = {
             "ms_batch": ms, "ms_per_design_iteration": per_iter, "inference_ms_batch": t_inf,
             "iterations_to_thr": its, "final_screen_rmse": scr[:, -1].tolist(),
             "best_screen_rmse": scr.min(axis=1).tolist(), "kinds": kinds,
             "screen_rmse_median": np.median(scr, axis=0)[[0, 9, 49, 99, iters - 1]].tolist()
             if iters >= 100 else None,
         }
-        b.close()
+        if mode != "plain":
+            common = np.maximum(np.maximum(scr_plain.min(axis=1), scr.min(axis=1)), thr)
+            rec["common_threshold"] = common.tolist()
+            rec["iterations_to_common"] = [first_below(h, c) for h, c in zip(scr, common)]
+            rec["plain_iterations_to_common"] = [first_below(h, c) for h, c in zip(scr_plain, common)]
+        out[mode] = rec
         reached = [k for k in its if k is not None]
         log(f"  {mode}: {len(reached)}/{count} designs reach screen rmse <= {thr}; median "
             f"iterations {np.median(reached) if reached else None}; final rmse median "
@@ -256,7 +302,44 @@ def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0
         for kind in FAMILIES:
             sel = [i for i, k in enumerate(kinds) if k == kind]
             log(f"    {kind}: iterations {[its[i] for i in sel]}, screen rmse at 10/50/final "
-                f"{np.round(np.median(scr[sel][:, [9, 49, -1]], axis=0), 4).tolist()}")
+                f"{np.round(np.median(scr[sel][:, [min(9, iters - 1), min(49, iters - 1), -1]], axis=0), 4).tolist()}")
+    return out
+
+
+def summarize(res: dict, thr: float = 0.01) -> dict:
+    """per mode and target family: designs reaching `thr`, median iterations and
+    GPU ms per design to `thr`, median best rmse, and the median ratio of plain's
+    GPU time to the mode's GPU time at the common threshold (> 1: the start helps)"""
+    plain = res["plain"]
+    out = {}
+    for mode, r in res.items():
+        if not isinstance(r, dict) or "kinds" not in r:
+            continue
+        kinds = r["kinds"]
+        count = len(kinds)
+        inf_ms = r["inference_ms_batch"] / count
+        fam = {}
+        for kind in FAMILIES:
+            sel = [i for i, k in enumerate(kinds) if k == kind]
+            its = [r["iterations_to_thr"][i] for i in sel if r["iterations_to_thr"][i] is not None]
+            e = {"designs": len(sel), "reach_thr": len(its),
+                 "median_iterations_to_thr": float(np.median(its)) if its else None,
+                 "median_gpu_ms_to_thr": round(float(np.median(its)) * r["ms_per_design_iteration"]
+                                               + inf_ms, 4) if its else None,
+                 "median_best_rmse": round(float(np.median([r["best_screen_rmse"][i] for i in sel])), 5)}
+            if mode != "plain":
+                ratios = []
+                for i in sel:
+                    a, b_ = r["plain_iterations_to_common"][i], r["iterations_to_common"][i]
+                    if a is None or b_ is None:
+                        continue
+                    ratios.append(a * plain["ms_per_design_iteration"]
+                                  / (b_ * r["ms_per_design_iteration"] + inf_ms))
+                e["median_common_thr"] = round(float(np.median([r["common_threshold"][i] for i in sel])), 5)
+                e["median_plain_over_mode_gpu_time"] = round(float(np.median(ratios)), 3) if ratios else None
+            fam[kind] = e
+        out[mode] = {"us_per_design_iteration": round(r["ms_per_design_iteration"] * 1e3, 2),
+                     "inference_ms_per_design": round(inf_ms, 4), "families": fam}
     return out
 
 
@@ -270,8 +353,11 @@ def main(argv=None) -> int:
     ap.add_argument("--eval", type=int, default=32)
     ap.add_argument("--eval-iters", type=int, default=300)
     ap.add_argument("--threshold", type=float, default=0.01)
+    ap.add_argument("--label-res", type=int, nargs="*", default=None,
+                    help="resolutions of the label-bound starts (default: the network's and n)")
     ap.add_argument("--out", default=None, help="write the result JSON here")
     args = ap.parse_args(argv)
+    label_res = args.label_res if args.label_res is not None else [args.res, args.n]
     ctx = nat.Context(0)
     t0 = time.perf_counter()
     print(f"pairs: {args.train} designs at {args.n}^2, {args.label_iters} WGS iterations", flush=True)
@@ -282,8 +368,10 @@ def main(argv=None) -> int:
                       log=lambda s: print(s, flush=True))
     print("evaluation", flush=True)
     res = evaluate(ctx, net, args.eval, args.n, args.eval_iters, seed=99999, thr=args.threshold,
-                   log=lambda s: print(s, flush=True))
+                   log=lambda s: print(s, flush=True), label_res=label_res)
     res["train_loss_first_last"] = [float(np.mean(hist[:20])), float(np.mean(hist[-20:]))]
+    res["summary"] = summarize(res, args.threshold)
+    print(json.dumps(res["summary"], indent=1), flush=True)
     if args.out:
         with open(args.out, "w") as f:
             json.dump(res, f)
