@@ -45,6 +45,7 @@ SOURCES = [
     ("gs_batch.hip", [None]),
     ("nn.hip", [None]),
     ("warmstart.hip", [None]),
+    ("metrics.hip", [None]),
 ]
 
 

@@ -1,8 +1,10 @@
 // Drop-in for src/projector.js: the ProjectorSystem pipeline.  Targets and
 // ROIs are built on the host (they are uploaded once per design); design()
 // runs the whole phase-retrieval loop on the GPU; simulate() propagates on
-// the GPU; screenMetrics() judges in FP64 on the host.
+// the GPU; screenMetrics() judges the device-resident screen on the GPU in
+// FP64 (the screen is never downloaded).
 import { ComplexField } from './field.js';
+import { native } from './native.js';
 import { propagateAngularSpectrumInPlace, propagateFresnelFFT, focalPlaneSpectrum, lensPhase } from './propagate.js';
 import { designDOE, illuminationProfile } from './doe.js';
 import {
@@ -403,7 +405,44 @@ export class ProjectorSystem {
   }
 
   // FP64 judge over the downloaded screen irradiance (ref: projector.js:554-614).
+  // Device copy of a target's irradiance (re) and ROI (im), FP64, cached per
+  // (irradiance, roi) pair.
+  _targetDevice(irradiance, mask) {
+    const c = this._tgtDev;
+    if (c && c.irr === irradiance && c.mask === mask) return c.dev;
+    const dev = native.fieldCreate(this.nx, this.ny, 1, 1);
+    native.fieldUpload(dev, irradiance, mask || new Float64Array(irradiance.length).fill(1));
+    this._tgtDev = { irr: irradiance, mask, dev };
+    return dev;
+  }
+
+  // ref: projector.js:554-614.  Every figure is reduced on the GPU in FP64
+  // (doe_screen_metrics); the incident power is the element's sum |E| dx dy,
+  // i.e. that of the design's illumination.
   screenMetrics(field, { target = null, roi = null, phase = null } = {}) {
+    const tgt = target || this.target;
+    const mask = roi || tgt.roi;
+    const element = new ComplexField(this.nx, this.ny, this.dx, this.dy);
+    element.setAmplitudePhase(this.result.illumination, phase || this.result.phase);
+    const grid = tgt.grid || null;
+    return native.screenMetrics(
+      field._device(),
+      this._targetDevice(tgt.irradiance, mask),
+      element._device(1), // FP64: the element sums keep the host illumination's precision
+      field.dx,
+      field.dy,
+      this.dx,
+      this.dy,
+      grid ? (grid.cols === undefined ? 8 : grid.cols) : 0,
+      grid ? (grid.rows === undefined ? 8 : grid.rows) : 0,
+      grid && grid.minPeak !== undefined ? grid.minPeak : 0.05,
+      true,
+    );
+  }
+
+  // The same figures on the host from the downloaded irradiance (FP64 loops
+  // of metrics.js); kept for callers that hold host-only fields.
+  screenMetricsHost(field, { target = null, roi = null, phase = null } = {}) {
     const tgt = target || this.target;
     const irr = field.intensity();
     const mask = roi || tgt.roi;

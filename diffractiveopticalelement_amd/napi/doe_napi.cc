@@ -339,6 +339,62 @@ double prop(napi_env env, napi_value obj, const char* k, double dflt) {
 }
 
 // gsRun(plan, illumination, targetAmp, roi|null, initPhase|null, seed)
+// screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, minPeak, weightTarget)
+// -> the reference's screenMetrics object (projector.js:597-613); target is
+// an FP64 field handle with re = irradiance, im = ROI.
+napi_value ScreenMetrics(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 7) {
+    napi_throw_type_error(env, nullptr,
+                          "screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, "
+                          "minPeak, weightTarget)");
+    return nullptr;
+  }
+  doe_field* s = field_of(env, a.v[0]);
+  doe_field* t = field_of(env, a.v[1]);
+  doe_field* e = field_of(env, a.v[2]);
+  if (!s || !t || !e) {
+    napi_throw_type_error(env, nullptr, "screenMetrics: screen, target and element must be field handles");
+    return nullptr;
+  }
+  doe_screen_opts o;
+  o.sdx = num(env, a.v[3]);
+  o.sdy = num(env, a.v[4]);
+  o.edx = num(env, a.v[5]);
+  o.edy = num(env, a.v[6]);
+  o.grid_cols = a.n > 7 ? (int)num(env, a.v[7], 0) : 0;
+  o.grid_rows = a.n > 8 ? (int)num(env, a.v[8], 0) : 0;
+  o.grid_min_peak = a.n > 9 ? num(env, a.v[9], 0.05) : 0.05;
+  o.ssim_weight_target = a.n > 10 ? (boolean(env, a.v[10], true) ? 1 : 0) : 1;
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  doe_metrics m;
+  if (throw_status(env, doe_screen_metrics(c, s, t, e, &o, &m))) return nullptr;
+  napi_value obj;
+  CHECK_NAPI(napi_create_object(env, &obj));
+  napi_value nul;
+  napi_get_null(env, &nul);
+  set(env, obj, "efficiency", make_num(env, m.efficiency));
+  set(env, obj, "fluxFraction", make_num(env, m.flux_fraction));
+  set(env, obj, "uniformity", make_num(env, m.uniformity));
+  set(env, obj, "spotUniformity", m.has_spots ? make_num(env, m.spot_uniformity) : nul);
+  set(env, obj, "flatness", make_num(env, m.flatness));
+  set(env, obj, "contrast", make_num(env, m.contrast));
+  set(env, obj, "rmse", make_num(env, m.rmse));
+  set(env, obj, "fidelity", make_num(env, m.fidelity));
+  set(env, obj, "correlation", make_num(env, m.correlation));
+  set(env, obj, "ssim", make_num(env, m.ssim));
+  set(env, obj, "snr", make_num(env, m.snr));
+  set(env, obj, "zeroOrder", make_num(env, m.zero_order));
+  set(env, obj, "rmsRadius", make_num(env, m.rms_radius));
+  set(env, obj, "meanIntensity", make_num(env, m.mean_intensity));
+  set(env, obj, "peakIntensity", make_num(env, m.peak_intensity));
+  set(env, obj, "powerInSignal", make_num(env, m.power_in_signal));
+  set(env, obj, "powerAtScreen", make_num(env, m.power_at_screen));
+  set(env, obj, "powerIncident", make_num(env, m.power_incident));
+  return obj;
+}
+
 napi_value GsRun(napi_env env, napi_callback_info info) {
   Args a;
   get_args(env, info, a);
@@ -408,6 +464,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"fresnel", nullptr, Fresnel, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"lens", nullptr, Lens, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gsRun", nullptr, GsRun, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"screenMetrics", nullptr, ScreenMetrics, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

@@ -48,6 +48,22 @@ class GsPlan(C.Structure):
     ]
 
 
+class ScreenOpts(C.Structure):
+    _fields_ = [("sdx", C.c_double), ("sdy", C.c_double), ("edx", C.c_double), ("edy", C.c_double),
+                ("grid_cols", C.c_int), ("grid_rows", C.c_int), ("grid_min_peak", C.c_double),
+                ("ssim_weight_target", C.c_int)]
+
+
+METRIC_NAMES = ["efficiency", "flux_fraction", "uniformity", "spot_uniformity", "flatness", "contrast",
+                "rmse", "fidelity", "correlation", "ssim", "snr", "zero_order", "rms_radius",
+                "mean_intensity", "peak_intensity", "power_in_signal", "power_at_screen",
+                "power_incident"]
+
+
+class Metrics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in METRIC_NAMES] + [("has_spots", C.c_int)]
+
+
 # Every symbol the header declares: (name, restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = [
@@ -118,6 +134,7 @@ SIGNATURES = [
     ("doe_warmstart_labels", C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
     ("doe_warmstart_apply_phasor", C.c_int, [_VP, C.c_int, _VP]),
+    ("doe_screen_metrics", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts), C.POINTER(Metrics)]),
 ]
 
 GSB_GRAPH = 1
@@ -298,6 +315,22 @@ class Context:
         check(self.lib.doe_lens(self.h, fin.h, fout.h, dx, dy, lam, _dptr(ff), _dptr(odx),
                                 _dptr(ody)))
         return odx, ody
+
+    def screen_metrics(self, screen: "Field", target: "Field", element: "Field", sdx: float,
+                       sdy: float, edx: float, edy: float, grid=None, min_peak: float = 0.05,
+                       weight_target: bool = True) -> dict:
+        """screenMetrics of a device-resident screen field (target: FP64 field with
+        re = target irradiance, im = ROI; element: illumination * e^{i phase});
+        grid = (cols, rows) of a spot array or None"""
+        o = ScreenOpts(sdx, sdy, edx, edy, grid[0] if grid else 0, grid[1] if grid else 0, min_peak,
+                       1 if weight_target else 0)
+        m = Metrics()
+        check(self.lib.doe_screen_metrics(self.h, screen.h, target.h, element.h, C.byref(o),
+                                          C.byref(m)))
+        out = {n: getattr(m, n) for n in METRIC_NAMES}
+        if not m.has_spots:
+            out["spot_uniformity"] = None
+        return out
 
     def gs_run(self, plan: GsPlan, illumination, target_amp, roi=None, init_phase=None,
                seed: int = 12345):

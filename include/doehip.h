@@ -261,6 +261,36 @@ int doe_warmstart_apply(doe_gs_batch* batch, doe_wsnet* net, float* scratch);
  * quadratic target-plane phase. */
 int doe_warmstart_apply_phasor(doe_gs_batch* batch, int res, const float* phasor);
 
+/* Projection-quality metrics of a screen field on the device, in FP64.
+ * Replaces ProjectorSystem#screenMetrics (projector.js:554-614) and the
+ * metrics.js helpers it calls (maskedSum, regionStats, rmsError,
+ * correlation, ssim, spotUniformity, zeroOrderFraction, rmsRadius:
+ * metrics.js:10-321) for a device-resident screen.
+ *   screen   the simulated screen field (any dtype)
+ *   target   FP64 field on the screen's grid: re = target irradiance,
+ *            im = ROI (the mask values, as the reference's maskedSum uses)
+ *   element  the element-plane field illumination * e^{i phase} (any dtype):
+ *            zero-order fraction, and the incident power sum|E| * edx * edy,
+ *            which equals the reference's maskedSum(illumination()) * dx * dy
+ *            for that field */
+typedef struct doe_screen_opts {
+  double sdx, sdy;          /* screen pitch (power cell, rmsRadius) */
+  double edx, edy;          /* element pitch (incident power) */
+  int grid_cols, grid_rows; /* spot-array cells (target.grid); 0, 0: none */
+  double grid_min_peak;     /* cells count pixels >= this * plane peak (reference default 0.05) */
+  int ssim_weight_target;   /* 1: windows weighted by target energy, as screenMetrics does */
+} doe_screen_opts;
+
+typedef struct doe_metrics {
+  double efficiency, flux_fraction, uniformity, spot_uniformity, flatness, contrast, rmse,
+      fidelity, correlation, ssim, snr, zero_order, rms_radius, mean_intensity, peak_intensity,
+      power_in_signal, power_at_screen, power_incident;
+  int has_spots; /* 0: spot_uniformity is NaN (the reference's null) */
+} doe_metrics;
+
+int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+                       const doe_field* element, const doe_screen_opts* opts, doe_metrics* out);
+
 #ifdef __cplusplus
 }
 #endif

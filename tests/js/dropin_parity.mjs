@@ -134,10 +134,17 @@ for (const prec of ['f32', 'f64']) {
     const os = new OProjector(cfg);
     const rg = gs.design();
     const ro = os.design();
-    const mg = gs.screenMetrics(gs.simulate());
+    const screen = gs.simulate();
+    const mg = gs.screenMetrics(screen);
     const mo = os.screenMetrics(os.simulate());
     const c = metricsClose(mg, mo, KEYS, 1e-3);
     check(`projector ${mode} ${prec} screenMetrics`, c.ok, c.bad);
+    // device reductions vs the host FP64 loops over the same downloaded screen
+    const mh = gs.screenMetricsHost(screen);
+    const rk = Object.keys(mh).filter((k) => mh[k] !== null && Number.isFinite(mh[k]));
+    const bad = rk.filter((k) => !(Math.abs(mg[k] - mh[k]) <= 1e-9 * Math.abs(mh[k]) + 1e-15));
+    check(`projector ${mode} ${prec} screenMetrics device vs host`, bad.length === 0 && rk.length >= 17,
+      bad.map((k) => [k, mg[k], mh[k]]));
     let maxe = 0;
     for (let i = 0; i < 10; i++) maxe = Math.max(maxe, Math.abs(rg.error[i] - ro.error[i]), Math.abs(rg.efficiency[i] - ro.efficiency[i]));
     check(`projector ${mode} ${prec} first 10 iterates`, maxe < 1e-4, maxe);
