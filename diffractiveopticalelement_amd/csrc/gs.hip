@@ -428,10 +428,166 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   return DOE_OK;
 }
 
+// Multi-plane GS (gerchbergSaxtonMultiPlane, doe.js:557-626): every
+// iteration visits the planes in order; at plane p the element
+// illum .* exp(i phi) is propagated by z_p, measured and projected on plane
+// p's target and ROI, propagated back by -z_p, and its phase becomes phi.
+// This is the image-plane GS step above with per-plane transfer functions,
+// targets and ROIs, so the same passes run with the plane's parameters: the
+// column pass that closes a step already applies H(z_{p+1}) of the next one.
+template <typename T>
+int run_gs_multiplane(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, int P,
+                      const doe_mp_plane* planes, const double* init_phase, double* phase_out,
+                      double* error_out, double* efficiency_out, double* per_plane_out) {
+  const int nx = plan->nx, ny = plan->ny, K = plan->iterations;
+  const long n = (long)nx * ny;
+  GsWork<T> W;
+  const size_t tb = size_t(n) * sizeof(T);
+  void* p = nullptr;
+  int rc;
+#define ALLOC(dst, bytes)                         \
+  do {                                            \
+    rc = W.alloc(&p, (bytes));                    \
+    if (rc) return rc;                            \
+    dst = (decltype(dst))p;                       \
+  } while (0)
+  T *illum_d, *tgt_d, *roi_d, *ph, *am;
+  ALLOC(W.fre, tb);
+  ALLOC(W.fim, tb);
+  ALLOC(illum_d, tb);
+  ALLOC(W.phi0, tb);
+  ALLOC(ph, tb);
+  ALLOC(am, tb);
+  ALLOC(tgt_d, tb * size_t(P));
+  ALLOC(roi_d, tb * size_t(P));
+  GsState* st_d;
+  ALLOC(st_d, sizeof(GsState));
+  const size_t hk = size_t(K > 0 ? K : 1);
+  ALLOC(W.hist, sizeof(double) * hk * size_t(P + 1));  // per-plane error, then efficiency scratch
+  AsmField* hp_d;
+  ALLOC(hp_d, sizeof(AsmField) * size_t(P));
+#undef ALLOC
+  rc = upload_t<T>(ctx, illum, illum_d, n);
+  std::vector<double> ones;
+  for (int q = 0; q < P && !rc; ++q) {
+    rc = upload_t<T>(ctx, planes[q].target_amp, tgt_d + size_t(q) * n, n);
+    if (!rc) {
+      if (!planes[q].roi && ones.empty()) ones.assign(n, 1.0);
+      rc = upload_t<T>(ctx, planes[q].roi ? planes[q].roi : ones.data(), roi_d + size_t(q) * n, n);
+    }
+  }
+  std::vector<double> phi;
+  if (!init_phase) phi.assign(n, 0.0);  // flatPhase (doe.js:572)
+  if (!rc) rc = upload_t<T>(ctx, init_phase ? init_phase : phi.data(), W.phi0, n);
+  if (rc) return rc;
+  std::vector<AsmField> hp(P);
+  for (int q = 0; q < P; ++q) {
+    hp[q].two_pi_z = 2 * kPi * planes[q].z;
+    hp[q].lim_x = hp[q].lim_y = hp[q].inner_x = hp[q].inner_y = 0;
+    hp[q].band = 0;  // bandLimit: false (doe.js:580, 605)
+    hp[q].zero = planes[q].z == 0 ? 1 : 0;
+  }
+  DOE_HIP(hipMemcpyAsync(hp_d, hp.data(), sizeof(AsmField) * P, hipMemcpyHostToDevice, ctx->stream));
+  DOE_HIP(hipMemsetAsync(W.hist, 0, sizeof(double) * hk * size_t(P + 1), ctx->stream));
+  hipLaunchKernelGGL(k_gs_init_state<T>, dim3(1), dim3(256), 0, ctx->stream, st_d, 1, (T*)nullptr,
+                     (const T*)nullptr, 0L, 0);
+  DOE_HIP(hipGetLastError());
+
+  GsBuf<T> b{};
+  b.illum = illum_d;
+  b.w = nullptr;
+  b.phase[0] = b.phase[1] = ph;
+  b.amp[0] = b.amp[1] = am;
+  b.state = st_d;
+  b.eff_hist = W.hist + hk * size_t(P);
+  b.nx = nx;
+  b.ny = ny;
+  b.dxdy_x = plan->dx;
+  b.dxdy_y = plan->dy;
+  b.norm = 1 / std::sqrt(double(nx) * double(ny));
+  b.gsign = 1.0;
+  b.beta = 0;
+  b.feedback = 0;
+  b.escape = std::numeric_limits<int>::max();
+  b.weighted = 0;
+  b.fstride = n;
+  b.hstride = 0;
+
+  Plan2 pl;
+  rc = make_plan2(ctx, nx, ny, sizeof(T), col_split<T>(nx, ny), &pl);
+  if (rc) return rc;
+  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF, 1);
+  if (rc) return rc;
+  T* re = W.fre;
+  T* im = W.fim;
+  const GsSizes sz{};
+  OpAsmH h{};
+  h.dfx = 1 / (nx * plan->dx);
+  h.dfy = 1 / (ny * plan->dy);
+  h.inv_l2 = 1 / (plan->lambda * plan->lambda);
+  h.nx = nx;
+  h.ny = ny;
+  h.evanescent = 1;
+  auto hfor = [&](int q, int conj) {
+    OpAsmH r = h;
+    r.fp = hp_d + q;
+    r.conj = conj;
+    return r;
+  };
+  // element -> rows forward -> column blocks F . H(z_0) . F^-1
+  OpIPInit<T> op0{b, W.phi0};
+  DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "mp_init_row", op0, re, im, re, im, 1, pl,
+                                                         sz)));
+  if (K > 0) DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_h", hfor(0, 0), re, im, re, im, 1, pl, sz)));
+  for (int it = 0; it < K; ++it) {
+    for (int q = 0; q < P; ++q) {
+      GsBuf<T> bq = b;
+      bq.tgt = tgt_d + size_t(q) * n;
+      bq.roi = roi_d + size_t(q) * n;
+      bq.err_hist = W.hist + hk * size_t(q);
+      OpIPMP<T> op{bq, ctx->partials, ctx->counters, it};
+      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_project", op, re, im, re, im, 1,
+                                                           pl, sz)));
+      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_hconj", hfor(q, 1), re, im, re, im, 1, pl,
+                                              sz)));
+      OpIPPhase<T> opp{bq};
+      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_phase", opp, re, im, re, im, 1,
+                                                           pl, sz)));
+      const bool last = it == K - 1 && q == P - 1;
+      if (!last) {
+        const int nq = q + 1 < P ? q + 1 : 0;
+        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_h", hfor(nq, 0), re, im, re, im, 1, pl,
+                                                sz)));
+      }
+    }
+  }
+  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  rc = download_t<T>(ctx, K > 0 ? ph : W.phi0, phase_out, n);
+  if (rc) return rc;
+  if (K > 0) {
+    std::vector<double> hist(hk * size_t(P));
+    DOE_HIP(hipMemcpy(hist.data(), W.hist, sizeof(double) * hist.size(), hipMemcpyDeviceToHost));
+    for (int it = 0; it < K; ++it) {
+      double e = 0;
+      for (int q = 0; q < P; ++q) e += hist[size_t(q) * hk + it] * planes[q].weight;
+      e /= P;
+      if (error_out) error_out[it] = e;
+      if (efficiency_out) efficiency_out[it] = 1 - e;
+    }
+    if (per_plane_out)
+      for (int q = 0; q < P; ++q)
+        std::copy(hist.begin() + size_t(q) * hk, hist.begin() + size_t(q) * hk + K, per_plane_out + size_t(q) * K);
+  }
+  return DOE_OK;
+}
+
 #define DOE_GS_INST(T)                                                                          \
   template int run_gs<T>(doe_ctx*, const doe_gs_plan*, const double*, const double*, const double*, \
                          const double*, unsigned, double*, double*, double*, double*, double*,    \
-                         double*, double*);
+                         double*, double*);                                                      \
+  template int run_gs_multiplane<T>(doe_ctx*, const doe_gs_plan*, const double*, int,            \
+                                    const doe_mp_plane*, const double*, double*, double*, double*, \
+                                    double*);
 #if DOE_PART == 0
 extern template int run_gs<float>(doe_ctx*, const doe_gs_plan*, const double*, const double*,
                                   const double*, const double*, unsigned, double*, double*, double*,
@@ -439,6 +595,12 @@ extern template int run_gs<float>(doe_ctx*, const doe_gs_plan*, const double*, c
 extern template int run_gs<double>(doe_ctx*, const doe_gs_plan*, const double*, const double*,
                                    const double*, const double*, unsigned, double*, double*,
                                    double*, double*, double*, double*, double*);
+extern template int run_gs_multiplane<float>(doe_ctx*, const doe_gs_plan*, const double*, int,
+                                             const doe_mp_plane*, const double*, double*, double*,
+                                             double*, double*);
+extern template int run_gs_multiplane<double>(doe_ctx*, const doe_gs_plan*, const double*, int,
+                                              const doe_mp_plane*, const double*, double*, double*,
+                                              double*, double*);
 #elif DOE_PART == 1
 DOE_GS_INST(float)
 #elif DOE_PART == 2
@@ -483,6 +645,43 @@ extern "C" int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* i
                        best_rmse_out);
   else {
     set_error("doe_gs_run: dtype must be DOE_F32 or DOE_F64");
+    rc = DOE_ERR_UNSUPPORTED;
+  }
+  ctx->flags = saved;
+  hipStreamSynchronize(ctx->stream);
+  return rc;
+}
+
+extern "C" int doe_gs_multiplane_run(doe_ctx* ctx, const doe_gs_plan* plan,
+                                     const double* illumination, int nplanes,
+                                     const doe_mp_plane* planes, const double* init_phase,
+                                     double* phase_out, double* error_out, double* efficiency_out,
+                                     double* per_plane_error_out) {
+  DOE_REQUIRE(ctx && plan && illumination && phase_out, DOE_ERR_INVALID,
+              "gerchbergSaxtonMultiPlane: null argument");
+  DOE_REQUIRE(nplanes > 0 && planes, DOE_ERR_INVALID, "gerchbergSaxtonMultiPlane: no planes");
+  for (int q = 0; q < nplanes; ++q)
+    DOE_REQUIRE(planes[q].target_amp, DOE_ERR_INVALID, "gerchbergSaxtonMultiPlane: plane without target");
+  doe_field probe;
+  probe.nx = plan->nx;
+  probe.ny = plan->ny;
+  probe.batch = 1;
+  int rc = check_field(ctx, &probe);
+  if (rc) return rc;
+  DOE_REQUIRE(plan->nx >= 64 && plan->ny >= 64 && plan->nx <= 4096 && plan->ny <= 4096,
+              DOE_ERR_UNSUPPORTED, "gerchbergSaxtonMultiPlane: grid sides must be in [64, 4096]");
+  DOE_REQUIRE(plan->iterations >= 0, DOE_ERR_INVALID, "gerchbergSaxtonMultiPlane: negative iteration count");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const int saved = ctx->flags;
+  ctx->flags |= DOE_CTX_ASYNC;
+  if (plan->dtype == DOE_F64)
+    rc = run_gs_multiplane<double>(ctx, plan, illumination, nplanes, planes, init_phase, phase_out,
+                                   error_out, efficiency_out, per_plane_error_out);
+  else if (plan->dtype == DOE_F32)
+    rc = run_gs_multiplane<float>(ctx, plan, illumination, nplanes, planes, init_phase, phase_out,
+                                  error_out, efficiency_out, per_plane_error_out);
+  else {
+    set_error("gerchbergSaxtonMultiPlane: dtype must be DOE_F32 or DOE_F64");
     rc = DOE_ERR_UNSUPPORTED;
   }
   ctx->flags = saved;

@@ -254,6 +254,31 @@ __global__ __launch_bounds__(kMB) void k_met_elem(const S* __restrict__ re, cons
   block_store<E_N>(v, 0, part + (long)blockIdx.x * E_N);
 }
 
+// Same-pitch resample (field.js:443-465): dst[b][j][i] = src[b or 0][j - oy][i - ox]
+// inside src, else 0, with ox = floor((nx2 - nx) / 2), oy likewise (centred
+// zero-padding or crop).  src_batch 1 broadcasts one field to every output.
+template <class S>
+__global__ __launch_bounds__(kMB) void k_resample_centred(const S* __restrict__ sre, const S* __restrict__ sim,
+                                                          S* __restrict__ dre, S* __restrict__ dim_, int nx,
+                                                          int ny, int nx2, int ny2, int src_batch, long total,
+                                                          int ox, int oy) {
+  const long q = (long)blockIdx.x * kMB + threadIdx.x;
+  if (q >= total) return;
+  const long per = (long)nx2 * ny2;
+  const int b = int(q / per);
+  const long r = q - (long)b * per;
+  const int j2 = int(r / nx2), i2 = int(r - (long)j2 * nx2);
+  const int i = i2 - ox, j = j2 - oy;
+  S vr = S(0.0f), vi = S(0.0f);
+  if (i >= 0 && i < nx && j >= 0 && j < ny) {
+    const long k = (long)(src_batch == 1 ? 0 : b) * nx * ny + (long)j * nx + i;
+    vr = sre[k];
+    vi = sim[k];
+  }
+  dre[q] = vr;
+  dim_[q] = vi;
+}
+
 // First pixel index of each of `cells` cells along a side of n pixels, with
 // the reference's mapping c(i) = min(cells - 1, floor(i / n * cells)), which
 // never decreases with i: cell c is [b[c], b[c+1]), b[cells] = n.
@@ -421,6 +446,73 @@ int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* t
   }
   set_error("unknown dtype");
   return DOE_ERR_INVALID;
+}
+
+int doe_screen_metrics_batch(doe_ctx* ctx, const doe_field* screens, const doe_field* target,
+                             const doe_field* element, const doe_screen_opts* opts, doe_metrics* out) {
+  DOE_REQUIRE(ctx && screens && out, DOE_ERR_INVALID, "null argument");
+  const size_t plane = size_t(screens->nx) * screens->ny * dtype_size(screens->dtype);
+  for (int b = 0; b < screens->batch; ++b) {
+    doe_field v = *screens;
+    v.batch = 1;
+    v.owned = false;
+    v.re = static_cast<char*>(screens->re) + b * plane;
+    v.im = static_cast<char*>(screens->im) + b * plane;
+    const int rc = doe_screen_metrics(ctx, &v, target, element, opts, out + b);
+    if (rc) return rc;
+  }
+  return DOE_OK;
+}
+
+int doe_field_resample_centred(doe_ctx* ctx, const doe_field* src, doe_field* dst) {
+  DOE_REQUIRE(ctx && src && dst, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(src->dtype == dst->dtype, DOE_ERR_INVALID, "resample: source and destination dtypes differ");
+  DOE_REQUIRE(src->batch == dst->batch || src->batch == 1, DOE_ERR_SHAPE,
+              "resample: source batch must be 1 or the destination's");
+  DOE_REQUIRE(src->re != dst->re && src->im != dst->im, DOE_ERR_INVALID, "resample: source and destination overlap");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const long total = long(dst->count());
+  const int ox = int(std::floor((dst->nx - src->nx) / 2.0)), oy = int(std::floor((dst->ny - src->ny) / 2.0));
+  const dim3 g(unsigned((total + kMB - 1) / kMB));
+  hipStream_t st = ctx->stream;
+#define DOE_RS(S)                                                                                      \
+  hipLaunchKernelGGL(k_resample_centred<S>, g, dim3(kMB), 0, st, static_cast<const S*>(src->re),      \
+                     static_cast<const S*>(src->im), static_cast<S*>(dst->re), static_cast<S*>(dst->im), \
+                     src->nx, src->ny, dst->nx, dst->ny, src->batch, total, ox, oy)
+  switch (dst->dtype) {
+    case DOE_F32: DOE_RS(float); break;
+    case DOE_F64: DOE_RS(double); break;
+    case DOE_F16: DOE_RS(__half); break;
+    case DOE_BF16: DOE_RS(__hip_bfloat16); break;
+    default: set_error("unknown dtype"); return DOE_ERR_INVALID;
+  }
+#undef DOE_RS
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_field_copy_slices(doe_ctx* ctx, const doe_field* src, int src_first, doe_field* dst,
+                          int dst_first, int count) {
+  DOE_REQUIRE(ctx && src && dst, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(src->nx == dst->nx && src->ny == dst->ny && src->dtype == dst->dtype, DOE_ERR_SHAPE,
+              "copy_slices: grids or dtypes differ");
+  DOE_REQUIRE(count >= 0 && src_first >= 0 && dst_first >= 0 && src_first + count <= src->batch &&
+                  dst_first + count <= dst->batch,
+              DOE_ERR_SHAPE, "copy_slices: slice range outside the batch");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const size_t plane = size_t(src->nx) * src->ny * dtype_size(src->dtype);
+  const size_t bytes = plane * count;
+  if (bytes) {
+    DOE_HIP(hipMemcpyAsync(static_cast<char*>(dst->re) + dst_first * plane,
+                           static_cast<const char*>(src->re) + src_first * plane, bytes,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    DOE_HIP(hipMemcpyAsync(static_cast<char*>(dst->im) + dst_first * plane,
+                           static_cast<const char*>(src->im) + src_first * plane, bytes,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
 }
 
 }  // extern "C"

@@ -203,6 +203,37 @@ export function fourierCgh(illumination, targetAmp, opts = {}) {
   return { phase: r.phase, error, efficiency, amplitude: r.amplitude, intensity: r.intensity, iterations: o.iterations, field: null };
 }
 
+// Multi-plane GS (ref: doe.js:557-626): one element, a different target at
+// each distance.  The whole loop runs on the device (doe_gs_multiplane_run):
+// per plane an angular-spectrum step to z, the GS projection on the plane's
+// target inside its ROI and the step back, bandLimit false, flat initial
+// phase unless initPhase is given.
+export function gerchbergSaxtonMultiPlane(illumination, planes, opts = {}) {
+  const o = Object.assign({}, DEFAULT_DESIGN, opts);
+  const g = geometry(o, illumination);
+  if (planes.length === 0) throw new Error('gerchbergSaxtonMultiPlane: no planes');
+  const spec = planes.map((p) => ({
+    z: p.z,
+    targetAmp: f64(p.targetAmp),
+    roi: p.roi ? f64(p.roi) : null,
+    weight: def(p.weight, 1),
+  }));
+  const r = native.gsMultiPlane(f64(illumination), spec, g.nx, g.ny, g.dx, g.dy, o.lambda, o.iterations,
+    loopDtype(), o.initPhase ? f64(o.initPhase) : null);
+  const error = Array.from(r.error);
+  const efficiency = Array.from(r.efficiency);
+  const perPlaneError = r.perPlaneError.map((a) => Array.from(a));
+  if (o.onProgress) {
+    const every = def(o.progressEvery, 10);
+    for (let it = 0; it < o.iterations; it++) {
+      if (it % every === 0 || it === o.iterations - 1) {
+        o.onProgress({ iteration: it, error: error[it], perPlane: perPlaneError.map((a) => a[it]) });
+      }
+    }
+  }
+  return { phase: r.phase, error, efficiency, perPlaneError, iterations: o.iterations };
+}
+
 // Weighted GS (ref: doe.js:455-546).  With the drop-in's own base loops the
 // whole weighted iteration runs on the device; any other base function runs
 // the reference's host-side wrapper around it.

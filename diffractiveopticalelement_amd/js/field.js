@@ -444,6 +444,12 @@ export class ComplexField {
   // Centred pad/crop at the same pitch, else bilinear (ref: field.js:443-495).
   resample(nx2, ny2, dx2 = this.dx, dy2 = this.dy) {
     const out = new ComplexField(nx2, ny2, dx2, dy2);
+    if (dx2 === this.dx && dy2 === this.dy && this._where === 'device') {
+      // same pitch, data in HBM: centred pad / crop on the device
+      native.fieldResample(this._dev, out._deviceOut(this._devDtype));
+      out._markDevice();
+      return out;
+    }
     const sr = this._hostRe();
     const si = this._hostIm();
     const orr = out._re;

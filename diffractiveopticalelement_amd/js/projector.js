@@ -4,9 +4,9 @@
 // the GPU; screenMetrics() judges the device-resident screen on the GPU in
 // FP64 (the screen is never downloaded).
 import { ComplexField } from './field.js';
-import { native } from './native.js';
+import { native, dtypeCode } from './native.js';
 import { propagateAngularSpectrumInPlace, propagateFresnelFFT, focalPlaneSpectrum, lensPhase } from './propagate.js';
-import { designDOE, illuminationProfile } from './doe.js';
+import { designDOE, illuminationProfile, gerchbergSaxtonMultiPlane, errorDiffusionQuantize } from './doe.js';
 import {
   correlation,
   maskedSum,
@@ -18,6 +18,10 @@ import {
   zeroOrderFraction,
   summarizeMetrics,
 } from './metrics.js';
+
+function pick(v, d) {
+  return v === undefined || v === null ? d : v;
+}
 
 // 5x7 bitmap font (data: the rows the reference's target raster is made of,
 // ref: projector.js:48-94).
@@ -420,24 +424,26 @@ export class ProjectorSystem {
   // (doe_screen_metrics); the incident power is the element's sum |E| dx dy,
   // i.e. that of the design's illumination.
   screenMetrics(field, { target = null, roi = null, phase = null } = {}) {
+    const a = this._metricsArgs(target, roi, phase);
+    return native.screenMetrics(field._device(), a.tgt, a.el, field.dx, field.dy, this.dx, this.dy,
+      a.cols, a.rows, a.minPeak, true);
+  }
+
+  // target device copy, FP64 element (its sums keep the host illumination's
+  // precision) and spot-grid options shared by the metric entry points
+  _metricsArgs(target, roi, phase) {
     const tgt = target || this.target;
     const mask = roi || tgt.roi;
     const element = new ComplexField(this.nx, this.ny, this.dx, this.dy);
     element.setAmplitudePhase(this.result.illumination, phase || this.result.phase);
     const grid = tgt.grid || null;
-    return native.screenMetrics(
-      field._device(),
-      this._targetDevice(tgt.irradiance, mask),
-      element._device(1), // FP64: the element sums keep the host illumination's precision
-      field.dx,
-      field.dy,
-      this.dx,
-      this.dy,
-      grid ? (grid.cols === undefined ? 8 : grid.cols) : 0,
-      grid ? (grid.rows === undefined ? 8 : grid.rows) : 0,
-      grid && grid.minPeak !== undefined ? grid.minPeak : 0.05,
-      true,
-    );
+    return {
+      tgt: this._targetDevice(tgt.irradiance, mask),
+      el: element._device(1),
+      cols: grid ? (grid.cols === undefined ? 8 : grid.cols) : 0,
+      rows: grid ? (grid.rows === undefined ? 8 : grid.rows) : 0,
+      minPeak: grid && grid.minPeak !== undefined ? grid.minPeak : 0.05,
+    };
   }
 
   // The same figures on the host from the downloaded irradiance (FP64 loops
@@ -484,27 +490,65 @@ export class ProjectorSystem {
     };
   }
 
-  // ref: projector.js:622-641
+  // Image-plane screens of simulate({ z, pad }) for every z, as one device
+  // batch of nx x ny fields: the element is padded and broadcast once and one
+  // angular-spectrum call propagates every slot by its own distance
+  // (ref: projector.js:492-497, once per z there).
+  _screensBatch(zs, pad) {
+    if (!this.result) throw new Error('ProjectorSystem.simulate: call design() first');
+    const r = this.result;
+    const dt = dtypeCode();
+    const el = new ComplexField(this.nx, this.ny, this.dx, this.dy);
+    el.setAmplitudePhase(r.illumination, r.phase);
+    const P = zs.length;
+    const big = pad > 1
+      ? native.fieldCreate(this.nx * pad, this.ny * pad, P, dt)
+      : native.fieldCreate(this.nx, this.ny, P, dt);
+    native.fieldResample(el._device(dt), big);
+    native.asm(big, big, this.dx, this.dy, this.lambda, Float64Array.from(zs), false, true, 0.02);
+    if (!(pad > 1)) return big;
+    const small = native.fieldCreate(this.nx, this.ny, P, dt);
+    native.fieldResample(big, small);
+    return small;
+  }
+
+  // ref: projector.js:622-641 (image plane: all distances in one batch,
+  // metrics reduced on the device; far field: one lens transform per z)
   defocusCurve({ points = 21, span = 0.6, pad = 2, mode = null } = {}) {
     const z0 = this.z * (1 - span);
     const z1 = this.z * (1 + span);
-    const out = [];
-    for (let i = 0; i < points; i++) {
-      const z = z0 + ((z1 - z0) * i) / (points - 1);
-      const m = this.screenMetrics(this.simulate({ z, pad, mode }));
-      out.push({ z, zMm: z * 1e3, fidelity: m.fidelity, correlation: m.correlation, ssim: m.ssim, rmse: m.rmse, efficiency: m.efficiency, uniformity: m.uniformity });
+    const zs = [];
+    for (let i = 0; i < points; i++) zs.push(z0 + ((z1 - z0) * i) / (points - 1));
+    const entry = (z, m) => ({ z, zMm: z * 1e3, fidelity: m.fidelity, correlation: m.correlation, ssim: m.ssim, rmse: m.rmse, efficiency: m.efficiency, uniformity: m.uniformity });
+    const useMode = mode === null ? this.cfg.mode : mode;
+    if (useMode === 'far-field' || zs.length === 0) {
+      return zs.map((z) => entry(z, this.screenMetrics(this.simulate({ z, pad, mode }))));
     }
-    return out;
+    const screens = this._screensBatch(zs, pad);
+    const a = this._metricsArgs(null, null, null);
+    const ms = native.screenMetricsBatch(screens, a.tgt, a.el, this.dx, this.dy, this.dx, this.dy,
+      a.cols, a.rows, a.minPeak, true);
+    return zs.map((z, i) => entry(z, ms[i]));
   }
 
-  // ref: projector.js:644-653
+  // ref: projector.js:644-653 (image plane: one batch)
   focusStack({ points = 9, span = 0.5, pad = 2, mode = null } = {}) {
-    const planes = [];
-    for (let i = 0; i < points; i++) {
-      const z = this.z * (1 - span) + (this.z * 2 * span * i) / (points - 1);
-      planes.push({ z, intensity: this.simulate({ z, pad, mode }).intensity() });
+    const zs = [];
+    for (let i = 0; i < points; i++) zs.push(this.z * (1 - span) + (this.z * 2 * span * i) / (points - 1));
+    const useMode = mode === null ? this.cfg.mode : mode;
+    if (useMode === 'far-field' || zs.length === 0) {
+      return zs.map((z) => ({ z, intensity: this.simulate({ z, pad, mode }).intensity() }));
     }
-    return planes;
+    const screens = this._screensBatch(zs, pad);
+    const n = this.nx * this.ny;
+    const re = new Float64Array(n * zs.length);
+    const im = new Float64Array(n * zs.length);
+    native.fieldDownload(screens, re, im);
+    return zs.map((z, i) => {
+      const I = new Float64Array(n);
+      for (let k = 0, q = i * n; k < n; k++, q++) I[k] = re[q] * re[q] + im[q] * im[q];
+      return { z, intensity: I };
+    });
   }
 
   // ref: projector.js:660-669
@@ -517,8 +561,44 @@ export class ProjectorSystem {
     return f;
   }
 
-  designMultiPlane() {
-    throw new Error('ProjectorSystem.designMultiPlane: multi-plane GS is not part of this GPU build');
+  // ref: projector.js:675-712 (the loop runs on the device)
+  designMultiPlane(planeSpecs, opts = {}) {
+    const illumination = this.illumination();
+    const planes = planeSpecs.map((p) => {
+      const t = p.irradiance ? p : makeTarget(p.kind, this.nx, this.ny, p);
+      const amp = new Float64Array(this.nx * this.ny);
+      for (let k = 0; k < amp.length; k++) amp[k] = Math.sqrt(Math.max(0, t.irradiance[k]));
+      const roi = t.roi === undefined || t.roi === null ? autoRoi(t.irradiance, this.nx, this.ny) : t.roi;
+      return { z: p.z, targetAmp: amp, roi, weight: p.weight === undefined || p.weight === null ? 1 : p.weight };
+    });
+    const res = gerchbergSaxtonMultiPlane(illumination, planes, {
+      nx: this.nx,
+      ny: this.ny,
+      dx: this.dx,
+      dy: this.dy,
+      lambda: this.lambda,
+      iterations: pick(opts.iterations, this.cfg.iterations),
+      seed: pick(opts.seed, this.cfg.seed),
+      initPhase: opts.initPhase,
+      onProgress: opts.onProgress,
+    });
+    const levels = pick(opts.levels, this.cfg.levels);
+    const out = Object.assign({}, res, {
+      illumination,
+      lambda: this.lambda,
+      nx: this.nx,
+      ny: this.ny,
+      dx: this.dx,
+      dy: this.dy,
+      z: this.z,
+      mode: 'multi-plane',
+      planes,
+      target: this.target,
+      roi: this.target.roi,
+    });
+    if (levels > 1) out.phase = errorDiffusionQuantize(out.phase, levels);
+    this.result = out;
+    return out;
   }
 
   report(metrics) {

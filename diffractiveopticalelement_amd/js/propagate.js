@@ -5,7 +5,7 @@
 // libdoehip.so.  Scalar helpers (local frequency limit, critical distance,
 // lens phase, wrapping) stay in JavaScript.
 import { ComplexField } from './field.js';
-import { native } from './native.js';
+import { native, dtypeCode } from './native.js';
 
 export const DEFAULT_WAVELENGTH = 532e-9;
 
@@ -129,13 +129,25 @@ export function wrapPhase(phi) {
   return phi;
 }
 
-// ref: propagate.js:427-434
+// ref: propagate.js:427-434.  All distances run as one batch: the field is
+// broadcast into steps+1 device slots, one angular-spectrum call propagates
+// each slot by its own z, and every plane is handed out as its own field.
 export function focusSweep(field, lambda, zStart, zEnd, steps, opts = {}) {
+  const count = steps >= 0 ? Math.floor(steps) + 1 : 0;
+  if (count === 0) return [];
+  const { bandLimit = true, evanescent = true, softEdge = 0.02 } = opts;
+  const zs = new Float64Array(count);
+  for (let s = 0; s < count; s++) zs[s] = zStart + ((zEnd - zStart) * s) / steps;
+  const dt = dtypeCode();
+  const batch = native.fieldCreate(field.nx, field.ny, count, dt);
+  native.fieldResample(field._device(dt), batch);
+  native.asm(batch, batch, field.dx, field.dy, lambda, zs, bandLimit, evanescent, softEdge);
   const planes = [];
-  for (let s = 0; s <= steps; s++) {
-    const z = zStart + ((zEnd - zStart) * s) / steps;
-    planes.push({ z, field: propagateAngularSpectrum(field, lambda, z, opts) });
+  for (let s = 0; s < count; s++) {
+    const out = new ComplexField(field.nx, field.ny, field.dx, field.dy);
+    native.fieldCopySlices(batch, s, out._deviceOut(dt), 0, 1);
+    out._markDevice();
+    planes.push({ z: zs[s], field: out });
   }
   return planes;
 }
-

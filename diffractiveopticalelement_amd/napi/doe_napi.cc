@@ -339,39 +339,9 @@ double prop(napi_env env, napi_value obj, const char* k, double dflt) {
 }
 
 // gsRun(plan, illumination, targetAmp, roi|null, initPhase|null, seed)
-// screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, minPeak, weightTarget)
-// -> the reference's screenMetrics object (projector.js:597-613); target is
-// an FP64 field handle with re = irradiance, im = ROI.
-napi_value ScreenMetrics(napi_env env, napi_callback_info info) {
-  Args a;
-  if (!get_args(env, info, a) || a.n < 7) {
-    napi_throw_type_error(env, nullptr,
-                          "screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, "
-                          "minPeak, weightTarget)");
-    return nullptr;
-  }
-  doe_field* s = field_of(env, a.v[0]);
-  doe_field* t = field_of(env, a.v[1]);
-  doe_field* e = field_of(env, a.v[2]);
-  if (!s || !t || !e) {
-    napi_throw_type_error(env, nullptr, "screenMetrics: screen, target and element must be field handles");
-    return nullptr;
-  }
-  doe_screen_opts o;
-  o.sdx = num(env, a.v[3]);
-  o.sdy = num(env, a.v[4]);
-  o.edx = num(env, a.v[5]);
-  o.edy = num(env, a.v[6]);
-  o.grid_cols = a.n > 7 ? (int)num(env, a.v[7], 0) : 0;
-  o.grid_rows = a.n > 8 ? (int)num(env, a.v[8], 0) : 0;
-  o.grid_min_peak = a.n > 9 ? num(env, a.v[9], 0.05) : 0.05;
-  o.ssim_weight_target = a.n > 10 ? (boolean(env, a.v[10], true) ? 1 : 0) : 1;
-  doe_ctx* c = ctx(env);
-  if (!c) return nullptr;
-  doe_metrics m;
-  if (throw_status(env, doe_screen_metrics(c, s, t, e, &o, &m))) return nullptr;
+napi_value metrics_object(napi_env env, const doe_metrics& m) {
   napi_value obj;
-  CHECK_NAPI(napi_create_object(env, &obj));
+  if (napi_create_object(env, &obj) != napi_ok) return nullptr;
   napi_value nul;
   napi_get_null(env, &nul);
   set(env, obj, "efficiency", make_num(env, m.efficiency));
@@ -392,6 +362,175 @@ napi_value ScreenMetrics(napi_env env, napi_callback_info info) {
   set(env, obj, "powerInSignal", make_num(env, m.power_in_signal));
   set(env, obj, "powerAtScreen", make_num(env, m.power_at_screen));
   set(env, obj, "powerIncident", make_num(env, m.power_incident));
+  return obj;
+}
+
+bool screen_opts(napi_env env, const Args& a, doe_screen_opts* o) {
+  o->sdx = num(env, a.v[3]);
+  o->sdy = num(env, a.v[4]);
+  o->edx = num(env, a.v[5]);
+  o->edy = num(env, a.v[6]);
+  o->grid_cols = a.n > 7 ? (int)num(env, a.v[7], 0) : 0;
+  o->grid_rows = a.n > 8 ? (int)num(env, a.v[8], 0) : 0;
+  o->grid_min_peak = a.n > 9 ? num(env, a.v[9], 0.05) : 0.05;
+  o->ssim_weight_target = a.n > 10 ? (boolean(env, a.v[10], true) ? 1 : 0) : 1;
+  return true;
+}
+
+// screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, minPeak, weightTarget)
+// -> the reference's screenMetrics object (projector.js:597-613); target is
+// an FP64 field handle with re = irradiance, im = ROI.
+napi_value ScreenMetrics(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 7) {
+    napi_throw_type_error(env, nullptr,
+                          "screenMetrics(screen, target, element, sdx, sdy, edx, edy, cols, rows, "
+                          "minPeak, weightTarget)");
+    return nullptr;
+  }
+  doe_field* s = field_of(env, a.v[0]);
+  doe_field* t = field_of(env, a.v[1]);
+  doe_field* e = field_of(env, a.v[2]);
+  if (!s || !t || !e) {
+    napi_throw_type_error(env, nullptr, "screenMetrics: screen, target and element must be field handles");
+    return nullptr;
+  }
+  doe_screen_opts o;
+  screen_opts(env, a, &o);
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  doe_metrics m;
+  if (throw_status(env, doe_screen_metrics(c, s, t, e, &o, &m))) return nullptr;
+  return metrics_object(env, m);
+}
+
+// screenMetricsBatch(screens, target, element, ...same options) -> array of metrics
+napi_value ScreenMetricsBatch(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 7) {
+    napi_throw_type_error(env, nullptr, "screenMetricsBatch(screens, target, element, sdx, sdy, edx, edy, ...)");
+    return nullptr;
+  }
+  doe_field* s = field_of(env, a.v[0]);
+  doe_field* t = field_of(env, a.v[1]);
+  doe_field* e = field_of(env, a.v[2]);
+  if (!s || !t || !e) {
+    napi_throw_type_error(env, nullptr, "screenMetricsBatch: screens, target and element must be field handles");
+    return nullptr;
+  }
+  int batch = 0;
+  doe_field_info(s, nullptr, nullptr, &batch, nullptr, nullptr, nullptr);
+  doe_screen_opts o;
+  screen_opts(env, a, &o);
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  std::vector<doe_metrics> m(batch > 0 ? batch : 1);
+  if (throw_status(env, doe_screen_metrics_batch(c, s, t, e, &o, m.data()))) return nullptr;
+  napi_value arr;
+  CHECK_NAPI(napi_create_array_with_length(env, batch, &arr));
+  for (int b = 0; b < batch; ++b) napi_set_element(env, arr, b, metrics_object(env, m[b]));
+  return arr;
+}
+
+// fieldResample(src, dst): centred same-pitch resample (src batch 1 broadcasts)
+napi_value FieldResample(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  doe_field* d = a.n > 1 ? field_of(env, a.v[1]) : nullptr;
+  if (!s || !d) {
+    napi_throw_type_error(env, nullptr, "fieldResample(src, dst)");
+    return nullptr;
+  }
+  throw_status(env, doe_field_resample_centred(ctx(env), s, d));
+  return nullptr;
+}
+
+// fieldCopySlices(src, srcFirst, dst, dstFirst, count)
+napi_value FieldCopySlices(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  doe_field* d = a.n > 2 ? field_of(env, a.v[2]) : nullptr;
+  if (!s || !d || a.n < 5) {
+    napi_throw_type_error(env, nullptr, "fieldCopySlices(src, srcFirst, dst, dstFirst, count)");
+    return nullptr;
+  }
+  throw_status(env, doe_field_copy_slices(ctx(env), s, (int)num(env, a.v[1]), d, (int)num(env, a.v[3]),
+                                          (int)num(env, a.v[4])));
+  return nullptr;
+}
+
+// gsMultiPlane(illum, planes[{z, targetAmp, roi, weight}], nx, ny, dx, dy, lambda,
+//              iterations, dtype, initPhase|null) -> {phase, error, efficiency, perPlaneError[]}
+napi_value GsMultiPlane(napi_env env, napi_callback_info info) {
+  Args a;
+  if (!get_args(env, info, a) || a.n < 9) {
+    napi_throw_type_error(env, nullptr,
+                          "gsMultiPlane(illum, planes, nx, ny, dx, dy, lambda, iterations, dtype, initPhase)");
+    return nullptr;
+  }
+  double* illum = f64(env, a.v[0], nullptr);
+  bool is_arr = false;
+  napi_is_array(env, a.v[1], &is_arr);
+  if (!illum || !is_arr) {
+    napi_throw_type_error(env, nullptr, "gsMultiPlane: illumination must be a Float64Array, planes an array");
+    return nullptr;
+  }
+  uint32_t np = 0;
+  napi_get_array_length(env, a.v[1], &np);
+  std::vector<doe_mp_plane> planes(np);
+  for (uint32_t q = 0; q < np; ++q) {
+    napi_value el, v;
+    napi_get_element(env, a.v[1], q, &el);
+    napi_get_named_property(env, el, "z", &v);
+    planes[q].z = num(env, v);
+    napi_get_named_property(env, el, "targetAmp", &v);
+    planes[q].target_amp = f64(env, v, nullptr);
+    napi_get_named_property(env, el, "roi", &v);
+    planes[q].roi = f64(env, v, nullptr);
+    napi_get_named_property(env, el, "weight", &v);
+    planes[q].weight = num(env, v, 1.0);
+    if (!planes[q].target_amp) {
+      napi_throw_type_error(env, nullptr, "gsMultiPlane: every plane needs a Float64Array targetAmp");
+      return nullptr;
+    }
+  }
+  doe_gs_plan plan{};
+  plan.nx = (int)num(env, a.v[2]);
+  plan.ny = (int)num(env, a.v[3]);
+  plan.dx = num(env, a.v[4]);
+  plan.dy = num(env, a.v[5]);
+  plan.lambda = num(env, a.v[6]);
+  plan.iterations = (int)num(env, a.v[7]);
+  plan.dtype = (int)num(env, a.v[8], DOE_F32);
+  const double* init = a.n > 9 ? f64(env, a.v[9], nullptr) : nullptr;
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  const size_t n = size_t(plan.nx) * plan.ny;
+  const size_t K = plan.iterations > 0 ? size_t(plan.iterations) : 0;
+  double *phase, *err, *eff;
+  napi_value jphase = make_f64(env, n, &phase);
+  napi_value jerr = make_f64(env, K, &err);
+  napi_value jeff = make_f64(env, K, &eff);
+  std::vector<double> per(np * K + 1);
+  if (throw_status(env, doe_gs_multiplane_run(c, &plan, illum, (int)np, planes.data(), init, phase, err,
+                                              eff, per.data())))
+    return nullptr;
+  napi_value jper;
+  CHECK_NAPI(napi_create_array_with_length(env, np, &jper));
+  for (uint32_t q = 0; q < np; ++q) {
+    double* d;
+    napi_value arr = make_f64(env, K, &d);
+    std::memcpy(d, per.data() + q * K, K * sizeof(double));
+    napi_set_element(env, jper, q, arr);
+  }
+  napi_value obj;
+  CHECK_NAPI(napi_create_object(env, &obj));
+  set(env, obj, "phase", jphase);
+  set(env, obj, "error", jerr);
+  set(env, obj, "efficiency", jeff);
+  set(env, obj, "perPlaneError", jper);
   return obj;
 }
 
@@ -465,6 +604,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"lens", nullptr, Lens, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gsRun", nullptr, GsRun, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"screenMetrics", nullptr, ScreenMetrics, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"screenMetricsBatch", nullptr, ScreenMetricsBatch, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldResample", nullptr, FieldResample, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldCopySlices", nullptr, FieldCopySlices, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"gsMultiPlane", nullptr, GsMultiPlane, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

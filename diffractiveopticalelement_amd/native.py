@@ -64,6 +64,11 @@ class Metrics(C.Structure):
     _fields_ = [(n, C.c_double) for n in METRIC_NAMES] + [("has_spots", C.c_int)]
 
 
+class MpPlane(C.Structure):
+    _fields_ = [("z", C.c_double), ("target_amp", C.c_void_p), ("roi", C.c_void_p),
+                ("weight", C.c_double)]
+
+
 # Every symbol the header declares: (name, restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = [
@@ -135,6 +140,13 @@ SIGNATURES = [
     ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
     ("doe_warmstart_apply_phasor", C.c_int, [_VP, C.c_int, _VP]),
     ("doe_screen_metrics", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts), C.POINTER(Metrics)]),
+    ("doe_screen_metrics_batch", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
+                                           C.POINTER(Metrics)]),
+    ("doe_field_resample_centred", C.c_int, [_VP, _VP, _VP]),
+    ("doe_field_copy_slices", C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, C.c_int]),
+    ("doe_gs_multiplane_run", C.c_int, [_VP, C.POINTER(GsPlan), c_double_p, C.c_int,
+                                        C.POINTER(MpPlane), c_double_p, c_double_p, c_double_p,
+                                        c_double_p, c_double_p]),
 ]
 
 GSB_GRAPH = 1
@@ -330,6 +342,54 @@ class Context:
         out = {n: getattr(m, n) for n in METRIC_NAMES}
         if not m.has_spots:
             out["spot_uniformity"] = None
+        return out
+
+    def screen_metrics_batch(self, screens: "Field", target: "Field", element: "Field", sdx, sdy,
+                             edx, edy, grid=None, min_peak: float = 0.05,
+                             weight_target: bool = True) -> list:
+        o = ScreenOpts(sdx, sdy, edx, edy, grid[0] if grid else 0, grid[1] if grid else 0, min_peak,
+                       1 if weight_target else 0)
+        arr = (Metrics * screens.batch)()
+        check(self.lib.doe_screen_metrics_batch(self.h, screens.h, target.h, element.h, C.byref(o),
+                                                arr))
+        out = []
+        for m in arr:
+            d = {n: getattr(m, n) for n in METRIC_NAMES}
+            if not m.has_spots:
+                d["spot_uniformity"] = None
+            out.append(d)
+        return out
+
+    def resample_centred(self, src: "Field", dst: "Field") -> None:
+        check(self.lib.doe_field_resample_centred(self.h, src.h, dst.h))
+
+    def copy_slices(self, src: "Field", src_first: int, dst: "Field", dst_first: int, count: int) -> None:
+        check(self.lib.doe_field_copy_slices(self.h, src.h, src_first, dst.h, dst_first, count))
+
+    def gs_multiplane(self, illumination, planes, nx, ny, dx, dy, lam, iterations, dtype="f32",
+                      init_phase=None) -> dict:
+        """planes: list of (z, target_amp, roi or None, weight)"""
+        keep = []
+        arr = (MpPlane * len(planes))()
+        for q, (z, t, roi, w) in enumerate(planes):
+            t = _darr(t).reshape(-1)
+            keep.append(t)
+            arr[q].z, arr[q].target_amp, arr[q].weight = z, t.ctypes.data, w
+            if roi is not None:
+                r = _darr(roi).reshape(-1)
+                keep.append(r)
+                arr[q].roi = r.ctypes.data
+        plan = GsPlan(iterations=iterations, nx=nx, ny=ny, dx=dx, dy=dy, lambda_=lam,
+                      dtype=DTYPES[dtype])
+        n = nx * ny
+        ill = _darr(illumination).reshape(-1)
+        ip = None if init_phase is None else _darr(init_phase).reshape(-1)
+        out = {"phase": np.zeros(n), "error": np.zeros(iterations), "efficiency": np.zeros(iterations),
+               "per_plane": np.zeros((len(planes), iterations))}
+        check(self.lib.doe_gs_multiplane_run(self.h, C.byref(plan), _dptr(ill), len(planes), arr,
+                                             None if ip is None else _dptr(ip), _dptr(out["phase"]),
+                                             _dptr(out["error"]), _dptr(out["efficiency"]),
+                                             _dptr(out["per_plane"])))
         return out
 
     def gs_run(self, plan: GsPlan, illumination, target_amp, roi=None, init_phase=None,

@@ -183,6 +183,28 @@ int doe_gs_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* illumination
                double* amplitude_out, double* intensity_out, double* weights_out,
                double* best_rmse_out);
 
+/* Multi-plane phase retrieval (gerchbergSaxtonMultiPlane, src/doe.js:557-626):
+ * one element that projects a different target at each distance.  Every
+ * iteration visits the planes in order (angular spectrum to z, GS
+ * projection on the plane's target inside its ROI, back to the element,
+ * phase kept), bandLimit false, flat initial phase when init_phase is NULL.
+ * plan: nx, ny, dx, dy, lambda, iterations and dtype (DOE_F32 / DOE_F64);
+ * mode, algorithm, z, beta, feedback and escape are not used.  Outputs:
+ * final phase; error[it] = sum_p weight_p * e_p / nplanes and
+ * efficiency[it] = 1 - error[it] (as the reference reports them); per-plane
+ * errors plane-major (nplanes x iterations).  Outputs other than phase are
+ * nullable. */
+typedef struct doe_mp_plane {
+  double z;
+  const double* target_amp; /* nx*ny */
+  const double* roi;        /* nx*ny or NULL (whole plane) */
+  double weight;            /* reference default 1 */
+} doe_mp_plane;
+int doe_gs_multiplane_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* illumination,
+                          int nplanes, const doe_mp_plane* planes, const double* init_phase,
+                          double* phase_out, double* error_out, double* efficiency_out,
+                          double* per_plane_error_out);
+
 /* Batched far-field phase retrieval.  No single counterpart in the
  * reference, which designs one element per call: each design of a batch runs
  * fourierCgh (src/doe.js:639-730), under DOE_ALG_WGS with the weight update
@@ -290,6 +312,23 @@ typedef struct doe_metrics {
 
 int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
                        const doe_field* element, const doe_screen_opts* opts, doe_metrics* out);
+/* The same for every field of a batch of screens (out: screens->batch
+ * entries), e.g. the planes of ProjectorSystem#defocusCurve
+ * (projector.js:622-641) propagated in one batched call. */
+int doe_screen_metrics_batch(doe_ctx* ctx, const doe_field* screens, const doe_field* target,
+                             const doe_field* element, const doe_screen_opts* opts,
+                             doe_metrics* out);
+
+/* Same-pitch resample of ComplexField#resample (ref: src/field.js:443-465):
+ * centred zero-padding or crop, dst[j][i] = src[j - oy][i - ox] with
+ * ox = floor((dst.nx - src.nx) / 2).  src batch 1 is broadcast to every
+ * field of dst (one element, many distances); otherwise batches match.
+ * Same dtype, no overlap. */
+int doe_field_resample_centred(doe_ctx* ctx, const doe_field* src, doe_field* dst);
+/* Copy `count` whole fields from src[src_first..] to dst[dst_first..]
+ * (same grid and dtype): splits a batched result into per-plane fields. */
+int doe_field_copy_slices(doe_ctx* ctx, const doe_field* src, int src_first, doe_field* dst,
+                          int dst_first, int count);
 
 #ifdef __cplusplus
 }

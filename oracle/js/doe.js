@@ -148,6 +148,65 @@ export function gerchbergSaxton(illumination, targetAmp, opts) {
   return { phase: phase, error: error, efficiency: efficiency, amplitude: amp, intensity: inten, iterations: iterations, field: doe };
 }
 
+// Multi-plane GS (ref: doe.js:557-626): per iteration the planes are visited
+// in order; each visit propagates the element illum * exp(i phase) to the
+// plane's z (no band limit), measures the amplitude error inside the plane's
+// ROI, imposes the target amplitude there (the field elsewhere is kept),
+// propagates back by -z and takes the phase.  error[it] is the
+// weight-averaged plane error and efficiency[it] = 1 - error[it], as the
+// reference reports them.  Flat initial phase unless initPhase is given.
+export function gerchbergSaxtonMultiPlane(illumination, planes, opts) {
+  const o = merged(opts);
+  const nx = pick(o.nx, Math.round(Math.sqrt(illumination.length)));
+  const ny = pick(o.ny, nx);
+  const dx = pick(o.dx, 8e-6);
+  const dy = pick(o.dy, dx);
+  if (planes.length === 0) throw new Error('gerchbergSaxtonMultiPlane: no planes');
+  const field = new ComplexField(nx, ny, dx, dy);
+  const phase = o.initPhase ? Float64Array.from(o.initPhase) : flatPhase(nx, ny);
+  const amp = new Float64Array(nx * ny);
+  const error = [];
+  const efficiency = [];
+  const perPlaneError = planes.map(function () { return []; });
+  for (let it = 0; it < o.iterations; it++) {
+    let acc = 0;
+    for (let p = 0; p < planes.length; p++) {
+      const pl = planes[p];
+      const roi = pick(pl.roi, null);
+      const t = pl.targetAmp;
+      field.setAmplitudePhase(illumination, phase);
+      propagateAngularSpectrumInPlace(field, o.lambda, pl.z, { bandLimit: false });
+      let num = 0;
+      let den = 0;
+      for (let k = 0; k < field.size; k++) {
+        amp[k] = Math.hypot(field.re[k], field.im[k]);
+        const m = roi ? roi[k] : 1;
+        if (m > 0) {
+          const d = (amp[k] - t[k]) * m;
+          num += d * d;
+          den += t[k] * t[k];
+        }
+      }
+      const e = den > 0 ? Math.sqrt(num / den) : 0;
+      perPlaneError[p].push(e);
+      acc += e * pick(pl.weight, 1);
+      for (let k = 0; k < field.size; k++) {
+        const m = roi ? roi[k] : 1;
+        if (m > 0 && amp[k] > 0) {
+          const s = t[k] / amp[k];
+          field.re[k] *= s;
+          field.im[k] *= s;
+        }
+      }
+      propagateAngularSpectrumInPlace(field, o.lambda, -pl.z, { bandLimit: false });
+      for (let k = 0; k < field.size; k++) phase[k] = Math.atan2(field.im[k], field.re[k]);
+    }
+    error.push(acc / planes.length);
+    efficiency.push(1 - error[error.length - 1]);
+  }
+  return { phase: phase, error: error, efficiency: efficiency, perPlaneError: perPlaneError, iterations: o.iterations };
+}
+
 // Weighted GS with bounded weights and best-iterate bookkeeping
 // (ref: doe.js:455-546).
 export function weightedGerchbergSaxton(illumination, targetAmp, opts, baseFn) {

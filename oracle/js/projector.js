@@ -347,4 +347,38 @@ export class ProjectorSystem {
       powerIncident: incident,
     };
   }
+
+  // ref: projector.js:622-641: screen metrics at evenly spaced distances
+  // around the design throw, one simulate() per distance
+  defocusCurve(opts) {
+    const o = opts || {};
+    const points = o.points === undefined ? 21 : o.points;
+    const span = o.span === undefined ? 0.6 : o.span;
+    const pad = o.pad === undefined ? 2 : o.pad;
+    const mode = o.mode === undefined ? null : o.mode;
+    const z0 = this.z * (1 - span);
+    const z1 = this.z * (1 + span);
+    const out = [];
+    for (let i = 0; i < points; i++) {
+      const z = z0 + ((z1 - z0) * i) / (points - 1);
+      const m = this.screenMetrics(this.simulate({ z: z, pad: pad, mode: mode }));
+      out.push({ z: z, zMm: z * 1e3, fidelity: m.fidelity, correlation: m.correlation, ssim: m.ssim, rmse: m.rmse, efficiency: m.efficiency, uniformity: m.uniformity });
+    }
+    return out;
+  }
+
+  // ref: projector.js:644-653: screen irradiance along the axis
+  focusStack(opts) {
+    const o = opts || {};
+    const points = o.points === undefined ? 9 : o.points;
+    const span = o.span === undefined ? 0.5 : o.span;
+    const pad = o.pad === undefined ? 2 : o.pad;
+    const mode = o.mode === undefined ? null : o.mode;
+    const out = [];
+    for (let i = 0; i < points; i++) {
+      const z = this.z * (1 - span) + (this.z * 2 * span * i) / (points - 1);
+      out.push({ z: z, intensity: this.simulate({ z: z, pad: pad, mode: mode }).intensity() });
+    }
+    return out;
+  }
 }

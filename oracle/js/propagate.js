@@ -274,3 +274,14 @@ export function lensPhase(field, lambda, f, opts) {
   }
   return wrap ? wrapPhase(phi) : phi;
 }
+
+// ref: propagate.js:427-434: the field propagated to steps + 1 evenly spaced
+// distances from zStart to zEnd, one propagation per distance.
+export function focusSweep(field, lambda, zStart, zEnd, steps, opts) {
+  const out = [];
+  for (let s = 0; s <= steps; s++) {
+    const z = zStart + ((zEnd - zStart) * s) / steps;
+    out.push({ z: z, field: propagateAngularSpectrum(field, lambda, z, opts) });
+  }
+  return out;
+}

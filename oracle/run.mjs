@@ -13,8 +13,9 @@ import {
   propagateFresnelFFT,
   propagateFresnelFFTInverse,
   focalPlaneSpectrum,
+  focusSweep,
 } from './js/propagate.js';
-import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase } from './js/doe.js';
+import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane } from './js/doe.js';
 import { ProjectorSystem } from './js/projector.js';
 import { ssim, rmsError, correlation, regionStats } from './js/metrics.js';
 
@@ -105,6 +106,27 @@ if (op === 'fft2d' || op === 'dft2d') {
   res.error = r.error;
   res.efficiency = r.efficiency;
   res.rmse = r.rmse === undefined ? null : r.rmse;
+} else if (op === 'multiplane') {
+  // gerchbergSaxtonMultiPlane with explicit inputs: in_target<q>, in_roi<q>
+  const planes = a.planes.map(function (p, q) {
+    return { z: p.z, targetAmp: inp('target' + q), roi: inp('roi' + q), weight: p.weight };
+  });
+  const r = gerchbergSaxtonMultiPlane(inp('illum'), planes, {
+    nx: a.nx, ny: a.ny, dx: a.dx, dy: a.dy, lambda: a.lambda, iterations: a.iterations, initPhase: inp('phase0'),
+  });
+  out('phase', r.phase);
+  res.error = r.error;
+  res.efficiency = r.efficiency;
+  res.perPlaneError = r.perPlaneError;
+} else if (op === 'focus_sweep') {
+  const sw = focusSweep(fieldOf(a.nx, a.ny, a.dx, a.dy), a.lambda, a.z0, a.z1, a.steps, { bandLimit: !!a.bandLimit });
+  const n = a.nx * a.ny;
+  const re = new Float64Array(n * sw.length);
+  const im = new Float64Array(n * sw.length);
+  sw.forEach(function (p, s) { re.set(p.field.re, s * n); im.set(p.field.im, s * n); });
+  out('re', re);
+  out('im', im);
+  res.z = sw.map(function (p) { return p.z; });
 } else if (op === 'random_phase') {
   out('phase', randomPhase(a.nx, a.ny, a.seed));
 } else if (op === 'projector') {
@@ -132,6 +154,15 @@ if (op === 'fft2d' || op === 'dft2d') {
   res.metrics = m;
   res.design_ms = t1 - t0;
   res.simulate_metrics_ms = t2 - t1;
+  if (a.defocus) res.defocus = sys.defocusCurve(a.defocus);
+  if (a.stack) {
+    const st = sys.focusStack(a.stack);
+    const n = sys.nx * sys.ny;
+    const all = new Float64Array(n * st.length);
+    st.forEach(function (p, i) { all.set(p.intensity, i * n); });
+    out('stack', all);
+    res.stack_z = st.map(function (p) { return p.z; });
+  }
 } else if (op === 'projector_eval') {
   // screenMetrics(simulate()) of a given element phase (e.g. one designed on
   // the GPU) under ProjectorSystem(cfg)

@@ -6,7 +6,7 @@ import * as offt from '../../oracle/js/fft.js';
 import * as oprop from '../../oracle/js/propagate.js';
 import { ComplexField as OField } from '../../oracle/js/field.js';
 import { ProjectorSystem as OProjector } from '../../oracle/js/projector.js';
-import { designDOE as oDesign } from '../../oracle/js/doe.js';
+import { designDOE as oDesign, gerchbergSaxtonMultiPlane as oMultiPlane } from '../../oracle/js/doe.js';
 
 const results = [];
 function check(name, ok, info) {
@@ -187,6 +187,91 @@ for (const prec of ['f32', 'f64']) {
   check('designDOE deterministic', md < 1e-12 && wrapped && a.error.length === 10, md);
   const ref = oDesign(cfg);
   check('designDOE error history vs oracle', Math.abs(a.error[9] - ref.error[9]) < 1e-3, [a.error[9], ref.error[9]]);
+}
+
+// Multi-distance callers (ref: propagate.js:427-434, projector.js:622-653,
+// doe.js:557-626): batched on the device, equal to the oracle's per-z loops
+for (const prec of ['f32', 'f64']) {
+  gpu.setPrecision(prec);
+  const tol = prec === 'f64' ? 1e-12 : 1e-5;
+  {
+    const [g, o] = pair(64, 64, 8e-6, 31);
+    const sg = gpu.focusSweep(g, LAM, 0.01, 0.03, 4, { bandLimit: true });
+    const so = oprop.focusSweep(o, LAM, 0.01, 0.03, 4, { bandLimit: true });
+    let worst = 0;
+    for (let s = 0; s < so.length; s++) worst = Math.max(worst, rel(sg[s].field.re, sg[s].field.im, so[s].field.re, so[s].field.im));
+    check(`focusSweep ${prec}`, sg.length === 5 && worst < tol && sg[4].z === so[4].z, worst);
+  }
+  {
+    const cfg = { n: 64, pitch: 8e-6, lambda: 532e-9, distance: 0.03, mode: 'image-plane', iterations: 8, seed: 3, target: { kind: 'disc' } };
+    const gs = new gpu.ProjectorSystem(cfg);
+    const os = new OProjector(cfg);
+    gs.design();
+    os.design();
+    // the same element on both sides, so the curves compare the callers alone
+    os.result.phase = Float64Array.from(gs.result.phase);
+    const dg = gs.defocusCurve({ points: 5, span: 0.5, pad: 2 });
+    const dor = os.defocusCurve({ points: 5, span: 0.5, pad: 2 });
+    const mtol = prec === 'f64' ? 1e-9 : 1e-3;
+    let bad = [];
+    for (let i = 0; i < 5; i++) {
+      for (const k of ['fidelity', 'correlation', 'ssim', 'rmse', 'efficiency', 'uniformity']) {
+        if (!(Math.abs(dg[i][k] - dor[i][k]) <= mtol)) bad.push([i, k, dg[i][k], dor[i][k]]);
+      }
+      if (dg[i].z !== dor[i].z) bad.push([i, 'z', dg[i].z, dor[i].z]);
+    }
+    check(`defocusCurve ${prec}`, bad.length === 0, bad);
+    const fg = gs.focusStack({ points: 3, span: 0.5, pad: 2 });
+    const fo = os.focusStack({ points: 3, span: 0.5, pad: 2 });
+    let worst = 0;
+    for (let i = 0; i < 3; i++) {
+      let num = 0;
+      let den = 0;
+      for (let k = 0; k < fo[i].intensity.length; k++) {
+        num += (fg[i].intensity[k] - fo[i].intensity[k]) ** 2;
+        den += fo[i].intensity[k] ** 2;
+      }
+      worst = Math.max(worst, Math.sqrt(num / den));
+    }
+    check(`focusStack ${prec}`, worst < (prec === 'f64' ? 1e-12 : 1e-5), worst);
+  }
+  {
+    const n = 64;
+    const ill = new Float64Array(n * n);
+    const ta = new Float64Array(n * n);
+    const tb = new Float64Array(n * n);
+    const roi = new Float64Array(n * n);
+    for (let j = 0; j < n; j++) {
+      for (let i = 0; i < n; i++) {
+        const r = Math.hypot(i - n / 2, j - n / 2);
+        ill[i + j * n] = r <= 0.45 * n ? 1 : 0;
+        ta[i + j * n] = r <= 0.2 * n ? 1.3 : 0;
+        tb[i + j * n] = Math.abs(i - n / 2) < 8 && Math.abs(j - n / 2) < 20 ? 1.1 : 0;
+        roi[i + j * n] = r <= 0.28 * n ? 1 : 0;
+      }
+    }
+    const planes = [{ z: 0.015, targetAmp: ta, roi }, { z: 0.03, targetAmp: tb, weight: 0.5 }];
+    const opts = { nx: n, ny: n, dx: 8e-6, lambda: LAM, iterations: 10 };
+    const mg = gpu.gerchbergSaxtonMultiPlane(ill, planes, opts);
+    const mo = oMultiPlane(ill, planes, opts);
+    let worst = 0;
+    for (let q = 0; q < 2; q++) for (let it = 0; it < 10; it++) worst = Math.max(worst, Math.abs(mg.perPlaneError[q][it] - mo.perPlaneError[q][it]));
+    check(`gerchbergSaxtonMultiPlane ${prec}`, worst < (prec === 'f64' ? 1e-9 : 1e-4) && mg.error.length === 10, worst);
+    let threw = false;
+    try {
+      gpu.gerchbergSaxtonMultiPlane(ill, [], opts);
+    } catch (err) {
+      threw = /no planes/.test(err.message);
+    }
+    check('gerchbergSaxtonMultiPlane without planes throws', threw);
+  }
+}
+{
+  gpu.setPrecision('f32');
+  const sys = new gpu.ProjectorSystem({ n: 64, pitch: 8e-6, lambda: 532e-9, distance: 0.03, iterations: 6, target: { kind: 'disc' } });
+  const r = sys.designMultiPlane([{ z: 0.02, kind: 'disc' }, { z: 0.04, kind: 'text', text: 'A' }], { iterations: 6 });
+  check('designMultiPlane', r.mode === 'multi-plane' && r.phase.length === 64 * 64 && r.perPlaneError.length === 2 &&
+    r.error.every((e) => Number.isFinite(e)), r.error);
 }
 
 const failed = results.filter((r) => !r.ok);

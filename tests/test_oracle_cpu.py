@@ -325,6 +325,108 @@ def test_published_far_field_design():
     assert abs(m["efficiency"] - 0.81) < 0.006, m["efficiency"]
 
 
+# ----------------------------------------------- multi-plane, multi-z -----
+def multiplane_case(n=64, dx=8e-6):
+    ill = tophat(n, dx, 0.45 * n * dx, 2 * dx)
+    ta = np.sqrt(disc(n, 0.25 * n))
+    ta *= np.sqrt(np.sum(ill ** 2) / np.sum(ta ** 2))
+    tb = np.zeros((n, n))
+    tb[n // 4:3 * n // 4, n // 3:2 * n // 3] = 1.0
+    tb *= np.sqrt(np.sum(ill ** 2) / np.sum(tb ** 2))
+    return ill, ta, tb
+
+
+def test_multiplane_single_plane_is_gerchberg_saxton():
+    """One plane, weight 1: the multi-plane loop is the image-plane GS loop
+    (flat start, no band limit), so errors and phases coincide."""
+    n, dx, z = 64, 8e-6, 0.02
+    ill, ta, _ = multiplane_case(n, dx)
+    roi = disc(n, 0.3 * n)
+    for r in (None, roi):
+        mp, rmp = oj.run("multiplane", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "iterations": 12,
+                                        "planes": [{"z": z, "weight": 1}]},
+                         {"illum": ill, "target0": ta, "roi0": r})
+        gs, rgs = oj.run("gs", {"mode": "image-plane", "algorithm": "gs", "nx": n, "ny": n, "dx": dx, "dy": dx,
+                                "lambda": LAM, "z": z, "iterations": 12},
+                         {"illum": ill, "target": ta, "roi": r})
+        assert np.max(np.abs(np.array(rmp["perPlaneError"][0]) - rgs["error"])) < 1e-12
+        assert np.max(np.abs(np.array(rmp["error"]) - rgs["error"])) < 1e-12
+        assert np.allclose(rmp["efficiency"], 1 - np.array(rmp["error"]), atol=0, rtol=0)
+        assert np.max(np.abs(np.exp(1j * mp["phase"]) - np.exp(1j * gs["phase"]))) < 1e-9
+
+
+def test_multiplane_plane_repeated_is_two_gs_steps():
+    """The same plane listed twice: each iteration is two GS steps."""
+    n, dx, z = 64, 8e-6, 0.015
+    ill, ta, _ = multiplane_case(n, dx)
+    mp, rmp = oj.run("multiplane", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "iterations": 5,
+                                    "planes": [{"z": z, "weight": 1}, {"z": z, "weight": 1}]},
+                     {"illum": ill, "target0": ta, "target1": ta})
+    gs, rgs = oj.run("gs", {"mode": "image-plane", "algorithm": "gs", "nx": n, "ny": n, "dx": dx, "dy": dx,
+                            "lambda": LAM, "z": z, "iterations": 10}, {"illum": ill, "target": ta})
+    e = np.array(rgs["error"])
+    assert np.max(np.abs(np.array(rmp["perPlaneError"][0]) - e[0::2])) < 1e-12
+    assert np.max(np.abs(np.array(rmp["perPlaneError"][1]) - e[1::2])) < 1e-12
+    assert np.max(np.abs(np.exp(1j * mp["phase"]) - np.exp(1j * gs["phase"]))) < 1e-9
+
+
+def test_multiplane_closed_forms_and_weights():
+    """z = 0 plane: the first error is |illum - t| / |t| over the ROI (flat
+    start, identity propagation); error = weighted mean of plane errors."""
+    n, dx = 64, 8e-6
+    ill, ta, tb = multiplane_case(n, dx)
+    roi = disc(n, 0.3 * n)
+    _, r = oj.run("multiplane", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "iterations": 4,
+                                 "planes": [{"z": 0.0, "weight": 2.0}, {"z": 0.02, "weight": 0.5}]},
+                  {"illum": ill, "target0": ta, "roi0": roi, "target1": tb})
+    m = roi > 0
+    e0 = np.sqrt(np.sum(((ill - ta) * roi)[m] ** 2) / np.sum(ta[m] ** 2))
+    assert abs(r["perPlaneError"][0][0] - e0) < 1e-12
+    pe = np.array(r["perPlaneError"])
+    assert np.allclose(r["error"], (2.0 * pe[0] + 0.5 * pe[1]) / 2, rtol=1e-15, atol=0)
+    with pytest.raises(RuntimeError):
+        oj.run("multiplane", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "iterations": 1, "planes": []},
+               {"illum": ill})
+
+
+def test_focus_sweep_semigroup():
+    """Plane s propagated by the step equals plane s+1 (H(a) H(b) = H(a + b)
+    without band limit); the z = 0 plane is the input; power is conserved."""
+    n, dx = 64, 8e-6
+    re, im = lcg(n * n, 5), lcg(n * n, 6)
+    out, r = oj.run("focus_sweep", {"nx": n, "ny": n, "dx": dx, "lambda": LAM, "z0": 0.0, "z1": 0.03,
+                                    "steps": 3, "bandLimit": False}, {"re": re, "im": im})
+    assert np.allclose(r["z"], [0.0, 0.01, 0.02, 0.03], rtol=1e-15, atol=1e-18)
+    f = (out["re"] + 1j * out["im"]).reshape(4, n, n)
+    assert np.max(np.abs(f[0] - (re + 1j * im).reshape(n, n))) < 1e-15
+    for s in range(3):
+        nxt, _ = oj.run("asm", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "z": 0.01,
+                                "bandLimit": False}, {"re": f[s].real.ravel(), "im": f[s].imag.ravel()})
+        g = (nxt["re"] + 1j * nxt["im"]).reshape(n, n)
+        assert rel(g, f[s + 1]) < 1e-9  # kz z reaches ~3.5e5 rad: FP64 phase rounding ~4e-11
+    p0 = np.sum(np.abs(f[0]) ** 2)
+    for s in range(4):
+        assert abs(np.sum(np.abs(f[s]) ** 2) / p0 - 1) < 0.02  # evanescent part removed only
+
+
+def test_defocus_curve_and_focus_stack_distances():
+    """defocusCurve spans z (1 +- span) evenly and its centre point is the
+    design-distance screenMetrics; focusStack spans z (1 +- span)."""
+    cfg = {"n": 64, "pitch": 8e-6, "lambda": 532e-9, "distance": 0.03, "mode": "image-plane",
+           "algorithm": "gs", "iterations": 5, "seed": 2, "target": {"kind": "disc"}}
+    out, r = oj.run("projector", {"cfg": cfg, "defocus": {"points": 5, "span": 0.4, "pad": 1},
+                                  "stack": {"points": 3, "span": 0.5, "pad": 1}})
+    zs = [d["z"] for d in r["defocus"]]
+    assert np.allclose(zs, 0.03 * np.array([0.6, 0.8, 1.0, 1.2, 1.4]), rtol=1e-12)
+    centre = r["defocus"][2]
+    for k in ("rmse", "efficiency", "correlation", "ssim", "uniformity"):
+        assert abs(centre[k] - r["metrics"][k]) < 1e-12, k
+    assert np.allclose(r["stack_z"], [0.015, 0.03, 0.045], rtol=1e-12)
+    st = out["stack"].reshape(3, -1)
+    scr = out["screen_re"] ** 2 + out["screen_im"] ** 2
+    assert np.max(np.abs(st[1] - scr)) < 1e-12 * scr.max()
+
+
 # --------------------------------------------------------- metrics --------
 def test_metrics_closed_forms():
     n = 32
