@@ -291,6 +291,8 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   DOE_HIP(hipMemsetAsync(W.hist, 0, sizeof(double) * 2 * size_t(K > 0 ? K : 1), ctx->stream));
   hipLaunchKernelGGL(k_gs_init_state<T>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, ctx->stream,
                      st_d, 1, w_d, (const T*)W.roi_d, n, weighted ? 1 : 0);
+  hipLaunchKernelGGL(k_gs_consts<T>, dim3(1), dim3(1024), 0, ctx->stream, st_d, (const T*)tgt_d,
+                     (const T*)W.roi_d, n, n);
   DOE_HIP(hipGetLastError());
 
   GsBuf<T>& b = W.b;

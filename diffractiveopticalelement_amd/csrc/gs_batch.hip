@@ -122,6 +122,8 @@ int gsb_start(doe_gs_batch* B) {
     const long n = long(c.n) * c.n * nd;
     hipLaunchKernelGGL(k_gs_init_state<float>, dim3(unsigned((std::max<long>(n, nd) + 255) / 256)),
                        dim3(256), 0, ctx->stream, c.state, nd, c.b.w, c.b.roi, n, c.b.weighted);
+    hipLaunchKernelGGL(k_gs_consts<float>, dim3(unsigned(nd)), dim3(1024), 0, ctx->stream, c.state,
+                       c.b.tgt, c.b.roi, long(c.n) * c.n, long(c.n) * c.n);
     DOE_HIP(hipGetLastError());
     OpFFInit<float> op0{c.b, B->phi0 + c.off};
     float* re = B->re + c.off;

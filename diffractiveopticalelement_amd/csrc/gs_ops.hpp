@@ -21,6 +21,9 @@ struct GsState {
   int best_amp_slot;
   int improved;
   int pad;
+  // sums of the design's target that stay constant over a run (far field):
+  // t0^2 and t0^4 inside the ROI, t0^2 over the whole plane
+  double t2_roi, t4_roi, t2_all;
 };
 
 template <typename T>
@@ -57,6 +60,9 @@ enum { R_ERRNUM, R_ERRDEN, R_INROI, R_TOTAL, R_RNUM, R_RDEN, R_WSUM, R_TSUM, R_N
 // T = t0^2 and s = sum(m*I) / sum(T), rmse^2 = (sum I^2 - 2 s sum I T +
 // s^2 sum T^2) / (s^2 sum T^2) over m > 0.
 enum { R_SI2 = R_N, R_SIT, R_ST2, R_TALL, R_NFF };
+// the far-field column pass accumulates only what changes between
+// iterations; the target's constant sums come from GsState
+enum { FF_TOTAL, FF_INROI, FF_SI2, FF_SIT, FF_ERRNUM, FF_ERRDEN, FF_RNUM, FF_WSUM, FF_N };
 
 template <typename T>
 __device__ __forceinline__ T clampv(T v, T lo, T hi) {
@@ -111,11 +117,16 @@ struct OpFFInit {
 // 490-521) on the centred-index spectrum.
 template <typename T>
 struct OpFFB {
-  static constexpr int NACC = R_NFF;
+  static constexpr int NACC = FF_N;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
-  // 512-thread tiles of 32 points per thread with both LDS planes: one
-  // workgroup per CU, but a 256-register budget instead of spilling at 128
-  static constexpr int COL_P = 32, COL_PLANES = 2;
+  // FP32 column tiles, both LDS planes, at most 16 columns: lines up to 512
+  // points run 16 points per thread (128 registers, 4 waves per SIMD, two
+  // workgroups per CU), longer lines 32 (a 256-register budget).  Measured
+  // on 32 x 1024^2 WGS designs (512-point column lines): 579 us per column
+  // pass against 613 us for 32 x 32 tiles; narrower tiles (8 columns), 32
+  // points per thread, or a 256-point column split were slower.
+  template <typename TT, int N>
+  using ColGeomT = ColGeom<TT, N, (N >= 1024 ? 32 : 16), 16, 2>;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -126,39 +137,37 @@ struct OpFFB {
   template <typename U>
   __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
   __device__ __forceinline__ void mid(int f, int x, int y, T& re, T& im, double* acc) const {
-    const long k = f * b.fstride + (long)y * b.nx + x;
+    // per-design base pointers (uniform across the workgroup), 32-bit offsets
+    const long base = f * b.fstride;
+    const int k = y * b.nx + x;
     const GsState* st = b.state + f;
     const T a = sqrt(re * re + im * im) * T(b.norm);
-    const T m = b.roi ? b.roi[k] : T(1);
-    const T t0 = b.tgt[k];
+    const T m = b.roi ? (b.roi + base)[k] : T(1);
+    const T t0 = (b.tgt + base)[k];
     T w = T(1);
-    if (b.weighted) w = b.w[k] * T(st->nrm);
+    if (b.weighted) w = (b.w + base)[k] * T(st->nrm);
     const T t = t0 * w;
     const double I = double(a) * double(a), Tt = double(t0) * double(t0);
-    acc[R_TOTAL] += I;
-    acc[R_TALL] += Tt;
+    acc[FF_TOTAL] += I;
     if (m > T(0)) {
-      acc[R_INROI] += I * double(m);
-      acc[R_SI2] += I * I;
-      acc[R_SIT] += I * Tt;
-      acc[R_ST2] += Tt * Tt;
+      acc[FF_INROI] += I * double(m);
+      acc[FF_SI2] += I * I;
+      acc[FF_SIT] += I * Tt;
       const double d = (double(a) - double(t)) * double(m);
-      acc[R_ERRNUM] += d * d;
-      acc[R_ERRDEN] += double(t) * double(t);
+      acc[FF_ERRNUM] += d * d;
+      acc[FF_ERRDEN] += double(t) * double(t);
       if (b.weighted) {
         const double d0 = double(a) - double(t0);
-        acc[R_RNUM] += d0 * d0;
-        acc[R_RDEN] += double(t0) * double(t0);
+        acc[FF_RNUM] += d0 * d0;
         const T wn = wgs_update<T>(w, a, t0, T(b.beta));
-        b.w[k] = wn;
-        acc[R_WSUM] += double(wn) * double(wn) * double(t0) * double(t0);
-        acc[R_TSUM] += double(t0) * double(t0);
+        (b.w + base)[k] = wn;
+        acc[FF_WSUM] += double(wn) * double(wn) * Tt;
       }
     }
     if (b.weighted) {
-      b.amp[1 - st->best_amp_slot][k] = a;
+      (b.amp[1 - st->best_amp_slot] + base)[k] = a;
     } else if (write_amp) {
-      b.amp[0][k] = a;
+      (b.amp[0] + base)[k] = a;
     }
     const T tp = m > T(0) ? t : a;
     if (a > T(0)) {
@@ -176,15 +185,15 @@ struct OpFFB {
     double* err_hist = b.err_hist + f * b.hstride;
     double* eff_hist = b.eff_hist + f * b.hstride;
     if (b.scr_hist) {
-      const double sc = s[R_TALL] > 0 ? s[R_INROI] / s[R_TALL] : 0.0;
-      const double den = sc * sc * s[R_ST2];
-      const double num = s[R_SI2] - 2 * sc * s[R_SIT] + sc * sc * s[R_ST2];
+      const double sc = st->t2_all > 0 ? s[FF_INROI] / st->t2_all : 0.0;
+      const double den = sc * sc * st->t4_roi;
+      const double num = s[FF_SI2] - 2 * sc * s[FF_SIT] + sc * sc * st->t4_roi;
       b.scr_hist[f * b.hstride + it] = den > 0 ? sqrt(fmax(num, 0.0) / den) : 0.0;
     }
-    const double err = s[R_ERRDEN] > 0 ? sqrt(s[R_ERRNUM] / s[R_ERRDEN]) : 0.0;
-    const double eff = s[R_TOTAL] > 0 ? s[R_INROI] / s[R_TOTAL] : 0.0;
+    const double err = s[FF_ERRDEN] > 0 ? sqrt(s[FF_ERRNUM] / s[FF_ERRDEN]) : 0.0;
+    const double eff = s[FF_TOTAL] > 0 ? s[FF_INROI] / s[FF_TOTAL] : 0.0;
     if (b.weighted) {
-      const double rmse = s[R_RDEN] > 0 ? sqrt(s[R_RNUM] / s[R_RDEN]) : 0.0;
+      const double rmse = st->t2_roi > 0 ? sqrt(s[FF_RNUM] / st->t2_roi) : 0.0;
       if (rmse < st->best_rmse) {
         st->best_rmse = rmse;
         st->improved = 1;
@@ -192,7 +201,7 @@ struct OpFFB {
       } else {
         st->improved = 0;
       }
-      st->nrm = s[R_WSUM] > 0 ? sqrt(s[R_TSUM] / s[R_WSUM]) : 1.0;
+      st->nrm = s[FF_WSUM] > 0 ? sqrt(st->t2_roi / s[FF_WSUM]) : 1.0;
       err_hist[it] = st->best_rmse;
     } else {
       err_hist[it] = err;
@@ -243,6 +252,40 @@ __global__ void k_gs_init_state(GsState* st, int nst, T* w, const T* roi, long n
     st[i].pad = 0;
   }
   if (weighted && i < n) w[i] = (roi ? (roi[i] > T(0) ? T(1) : T(0)) : T(1));
+}
+
+// Constant target sums of each design (one 1024-thread block per design,
+// fixed-order reduction): t0^2 and t0^4 where the ROI is positive, t0^2
+// over the plane.  Designs of `n` points, `fstride` apart.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_gs_consts(GsState* st, const T* tgt, const T* roi, long n,
+                                                     long fstride) {
+  const long base = (long)blockIdx.x * fstride;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (long k = threadIdx.x; k < n; k += 1024) {
+    const double t2 = double(tgt[base + k]) * double(tgt[base + k]);
+    v[2] += t2;
+    if (!roi || roi[base + k] > T(0)) {
+      v[0] += t2;
+      v[1] += t2 * t2;
+    }
+  }
+  __shared__ double red[16][3];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    for (int off = 32; off >= 1; off >>= 1) v[c] += __shfl_xor(v[c], off, 64);
+    if (lane == 0) red[wv][c] = v[c];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 16; ++j)
+      for (int c = 0; c < 3; ++c) r[c] += red[j][c];
+    st[blockIdx.x].t2_roi = r[0];
+    st[blockIdx.x].t4_roi = r[1];
+    st[blockIdx.x].t2_all = r[2];
+  }
 }
 
 template <typename T>

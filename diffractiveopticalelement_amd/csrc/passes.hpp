@@ -146,17 +146,46 @@ struct OpWide<Op, std::void_t<decltype(Op::COL_WIDE)>> {
   static constexpr bool value = Op::COL_WIDE;
 };
 
-// Column geometry an op asks for (COL_P points per thread, COL_PLANES LDS
-// planes) for FP32 lines of 512+ points; default: the generic choice.
-template <typename T, int N, class Op, class = void>
-struct OpColCfg {
+// Column geometry an op asks for.  Three forms, by priority: ColGeomT<T, N>
+// (the op names its FP32 geometry per line length, lines of 256+ points);
+// COL_P / COL_PLANES (and optionally COL_CMAX: widest tile, default 64
+// columns) for FP32 lines of 512+ points; otherwise the generic choice.
+template <class Op, class = void>
+struct HasColGeomT : std::false_type {};
+template <class Op>
+struct HasColGeomT<Op, std::void_t<typename Op::template ColGeomT<float, 256>>> : std::true_type {};
+template <class Op, class = void>
+struct HasColP : std::false_type {};
+template <class Op>
+struct HasColP<Op, std::void_t<decltype(Op::COL_P), decltype(Op::COL_PLANES)>> : std::true_type {};
+template <class Op, class = void>
+struct OpCmax {
+  static constexpr int value = 64;
+};
+template <class Op>
+struct OpCmax<Op, std::void_t<decltype(Op::COL_CMAX)>> {
+  static constexpr int value = Op::COL_CMAX;
+};
+template <typename T, int N, class Op,
+          int KIND = HasColGeomT<Op>::value ? 2 : (HasColP<Op>::value ? 1 : 0)>
+struct OpColCfgK {
   using type = ColCfg<T, N, OpWide<Op>::value>;
 };
 template <typename T, int N, class Op>
-struct OpColCfg<T, N, Op, std::void_t<decltype(Op::COL_P), decltype(Op::COL_PLANES)>> {
+struct OpColCfgK<T, N, Op, 1> {
   using type = typename std::conditional<(sizeof(T) == 4 && N >= 512),
-                                         ColGeom<T, N, Op::COL_P, 64, Op::COL_PLANES>,
+                                         ColGeom<T, N, Op::COL_P, OpCmax<Op>::value, Op::COL_PLANES>,
                                          ColCfg<T, N, OpWide<Op>::value>>::type;
+};
+template <typename T, int N, class Op>
+struct OpColCfgK<T, N, Op, 2> {
+  using type = typename std::conditional<(sizeof(T) == 4 && N >= 256),
+                                         typename Op::template ColGeomT<T, N>,
+                                         ColCfg<T, N, OpWide<Op>::value>>::type;
+};
+template <typename T, int N, class Op>
+struct OpColCfg {
+  using type = typename OpColCfgK<T, N, Op>::type;
 };
 
 // Which elementwise hooks an op uses (ops that do not say use both).  The
