@@ -119,14 +119,19 @@ template <typename T>
 struct OpFFB {
   static constexpr int NACC = FF_N;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
-  // FP32 column tiles, both LDS planes, at most 16 columns: lines up to 512
-  // points run 16 points per thread (128 registers, 4 waves per SIMD, two
-  // workgroups per CU), longer lines 32 (a 256-register budget).  Measured
-  // on 32 x 1024^2 WGS designs (512-point column lines): 579 us per column
-  // pass against 613 us for 32 x 32 tiles; narrower tiles (8 columns), 32
-  // points per thread, or a 256-point column split were slower.
+  // FP32 column tiles.  Lines up to 512 points: the plain transform's tile
+  // (32 points per thread, 32 columns, one LDS plane) with the per-point
+  // inputs (ROI, target, weight) loaded with the column before the first
+  // transform (PREFETCH), so their latency overlaps the field loads; longer
+  // lines: 32 points x 16 columns with both planes, inputs loaded between the
+  // transforms.  Measured on 32 x 1024^2 WGS designs (512-point column
+  // lines): 437 us per column pass against 579 us for 16 x 16 tiles loading
+  // the inputs between the transforms.
   template <typename TT, int N>
-  using ColGeomT = ColGeom<TT, N, (N >= 1024 ? 32 : 16), 16, 2>;
+  using ColGeomT = typename std::conditional<(N <= 512), ColGeom<TT, N, 32, 32, 1>,
+                                             ColGeom<TT, N, 32, 16, 2>>::type;
+  template <typename TT, int N>
+  static constexpr bool PREFETCH = sizeof(TT) == 4 && N <= 512;
   GsBuf<T> b;
   double* partials;
   unsigned* counters;
@@ -136,16 +141,33 @@ struct OpFFB {
   __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
   template <typename U>
   __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
-  __device__ __forceinline__ void mid(int f, int x, int y, T& re, T& im, double* acc) const {
+  // per-point inputs of the projection
+  struct Aux {
+    T m, t0, w;
+  };
+  __device__ __forceinline__ Aux load_aux(int f, int x, int y) const {
     // per-design base pointers (uniform across the workgroup), 32-bit offsets
+    const long base = f * b.fstride;
+    const int k = y * b.nx + x;
+    Aux r;
+    r.m = b.roi ? (b.roi + base)[k] : T(1);
+    r.t0 = (b.tgt + base)[k];
+    r.w = b.weighted ? (b.w + base)[k] : T(1);
+    return r;
+  }
+  __device__ __forceinline__ void mid(int f, int x, int y, T& re, T& im, double* acc) const {
+    mid_aux(f, x, y, re, im, acc, load_aux(f, x, y));
+  }
+  __device__ __forceinline__ void mid_aux(int f, int x, int y, T& re, T& im, double* acc,
+                                          const Aux& ax) const {
     const long base = f * b.fstride;
     const int k = y * b.nx + x;
     const GsState* st = b.state + f;
     const T a = sqrt(re * re + im * im) * T(b.norm);
-    const T m = b.roi ? (b.roi + base)[k] : T(1);
-    const T t0 = (b.tgt + base)[k];
+    const T m = ax.m;
+    const T t0 = ax.t0;
     T w = T(1);
-    if (b.weighted) w = (b.w + base)[k] * T(st->nrm);
+    if (b.weighted) w = ax.w * T(st->nrm);
     const T t = t0 * w;
     const double I = double(a) * double(a), Tt = double(t0) * double(t0);
     acc[FF_TOTAL] += I;
@@ -221,13 +243,26 @@ struct OpFFA {
   __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
   template <typename U>
   __device__ __forceinline__ void post(int, int, int, U&, U&, double*) const {}
-  __device__ __forceinline__ void mid(int f, int x, int y, T& re, T& im, double*) const {
+  // the illumination is loaded with the row, before the first transform
+  struct Aux {
+    T il;
+  };
+  template <typename TT, int N>
+  static constexpr bool PREFETCH = sizeof(TT) == 4;
+  __device__ __forceinline__ Aux load_aux(int f, int x, int y) const {
+    return Aux{(b.illum + f * b.fstride)[y * b.nx + x]};
+  }
+  __device__ __forceinline__ void mid(int f, int x, int y, T& re, T& im, double* acc) const {
+    mid_aux(f, x, y, re, im, acc, load_aux(f, x, y));
+  }
+  __device__ __forceinline__ void mid_aux(int f, int x, int y, T& re, T& im, double*,
+                                          const Aux& ax) const {
     const long k = f * b.fstride + (long)y * b.nx + x;
     const T cb = ((x + y) & 1) ? T(-1) : T(1);
     const bool wp = b.weighted ? (b.state[f].improved != 0) : (write_phase != 0);
     if (wp) b.phase[0][k] = atan2(cb * im, cb * re);
     const T r = sqrt(re * re + im * im);
-    const T il = b.illum[k];
+    const T il = ax.il;
     if (r > T(0)) {
       const T s = il / r;
       re *= s;

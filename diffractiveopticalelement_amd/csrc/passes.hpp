@@ -188,6 +188,23 @@ struct OpColCfg {
   using type = typename OpColCfgK<T, N, Op>::type;
 };
 
+// Ops whose mid hook reads per-point inputs can have them prefetched with the
+// column (Op::Aux, load_aux, mid_aux; Op::template PREFETCH<T, N> says for
+// which line lengths).
+template <class Op, typename T, int N, class = void>
+struct OpPrefetch : std::false_type {};
+template <class Op, typename T, int N>
+struct OpPrefetch<Op, T, N, std::void_t<typename Op::Aux, decltype(Op::template PREFETCH<T, N>)>>
+    : std::integral_constant<bool, Op::template PREFETCH<T, N>> {};
+template <class Op, bool PF>
+struct OpAuxOf {
+  using type = int;
+};
+template <class Op>
+struct OpAuxOf<Op, true> {
+  using type = typename Op::Aux;
+};
+
 // Which elementwise hooks an op uses (ops that do not say use both).  The
 // cross-row step may move past a hook that is a no-op.
 template <class Op, class = void>
@@ -471,6 +488,15 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
 #pragma unroll
     for (int i = 0; i < P; ++i) op.pre(f, PL::template in_pos<false>(t, i), y, vr[i], vi[i], acc);
   }
+  // per-point inputs of the mid hook loaded with the row (see col_body)
+  constexpr bool PF = MODE == 1 && OpPrefetch<Op, T, NX>::value;
+  typename OpAuxOf<Op, PF>::type aux[PF ? P : 1];
+  if constexpr (PF) {
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < P; ++i) aux[i] = op.load_aux(f, PL::template out_pos<false>(t, i), y);
+    }
+  }
   // the two line FFTs (MODE 1) or one (MODE 0)
   constexpr int CROSS1 = !LANES ? 0 : XIN ? 2 : (MODE == 0 && XOUT) ? 1 : 0;
   constexpr int CROSS2 = !LANES ? 0 : XOUT ? 1 : 0;
@@ -501,7 +527,10 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
           vr[i] *= invN;
           vi[i] *= invN;
         }
-        op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
+        if constexpr (PF)
+          op.mid_aux(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc, aux[i]);
+        else
+          op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
       }
     }
     // sums complete after the mid hook: sum them before the second FFT
@@ -613,6 +642,18 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
 #pragma unroll
     for (int i = 0; i < P; ++i) vr[i] = vi[i] = T(0);
   }
+  // ops with per-point inputs for the mid hook (Op::Aux) may ask for them to
+  // be loaded here, with the column, so their latency overlaps the field
+  // loads instead of stalling the workgroup between the two transforms
+  constexpr bool PF = MODE == 1 && OpPrefetch<Op, T, N2>::value;
+  typename OpAuxOf<Op, PF>::type aux[PF ? P : 1];
+  if constexpr (PF) {
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < P; ++i)
+        aux[i] = op.load_aux(f, x, k1 + n1cnt * PL::template out_pos<false>(t, i));
+    }
+  }
   const ColLIdx<C> li{col};
   fft_line<T, N2, P, INV1, false, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   constexpr T invN = T(1) / T(N2);
@@ -626,7 +667,10 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
           vr[i] *= invN;
           vi[i] *= invN;
         }
-        op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
+        if constexpr (PF)
+          op.mid_aux(f, x, k1 + n1cnt * p, vr[i], vi[i], acc, aux[i]);
+        else
+          op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
       }
     }
     // sums complete after the mid hook: sum them across the workgroup before
