@@ -150,6 +150,21 @@ struct OpWide<Op, std::void_t<decltype(Op::COL_WIDE)>> {
 // (the op names its FP32 geometry per line length, lines of 256+ points);
 // COL_P / COL_PLANES (and optionally COL_CMAX: widest tile, default 64
 // columns) for FP32 lines of 512+ points; otherwise the generic choice.
+// Field plane loads of the passes are non-temporal: each plane is read once
+// per pass, and a non-temporal load still hits what the previous pass left
+// in the memory-side cache (the alternating pass direction relies on that)
+// while not filling the caches with read-once data.  Stores stay ordinary,
+// so the next pass finds them there.  Measured on 16 x 2048^2 FP32 fft2d:
+// 412 us against 430 us with ordinary loads; non-temporal stores as well:
+// 433 us.
+template <typename T, typename S>
+__device__ __forceinline__ T ld_field(const S* p) {
+  if constexpr (std::is_same_v<S, float> || std::is_same_v<S, double>)
+    return T(__builtin_nontemporal_load(p));
+  else
+    return ld<T>(p);
+}
+
 template <class Op, class = void>
 struct HasColGeomT : std::false_type {};
 template <class Op>
@@ -473,8 +488,8 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const int pos = PL::template in_pos<false>(t, i);
-      vr[i] = ld<T>(ire + base + pos);
-      vi[i] = ld<T>(iim + base + pos);
+      vr[i] = ld_field<T>(ire + base + pos);
+      vi[i] = ld_field<T>(iim + base + pos);
     }
   } else {
 #pragma unroll
@@ -634,8 +649,8 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
       const int p = PL::template in_pos<false>(t, i);
       const int row = NAT_IN ? k1 + n1cnt * p : N2 * k1 + p;
       const long a = fbase + (long)row * nx + x;
-      vr[i] = ld<T>(ire + a);
-      vi[i] = ld<T>(iim + a);
+      vr[i] = ld_field<T>(ire + a);
+      vi[i] = ld_field<T>(iim + a);
       op.pre(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
     }
   } else {
