@@ -14,6 +14,9 @@ step, timed with HIP events on the library's stream; `cpu_baseline` is the
 FP64 JavaScript oracle (the reference's algorithm: radix-2 FFT) on one node
 thread, on a bounded sample.
 
+"storage_variants" times the same fft2d step with the batch stored in FP64,
+FP16 and BF16 (the half-width ones compute in FP32).
+
 The second half of the metric, WGS time to screen RMSE <= 1 % at 1024^2
 (learned warm start vs plain random start), is reported under "wgs": a
 network is trained on pairs from the batched WGS engine (bounded: 128
@@ -83,6 +86,38 @@ def traffic_of(tag: str) -> int | None:
     if k is None:
         return None
     return int(round((k["fetch_kib"] * d["scale"]["fetch"] + k["write_kib"] * d["scale"]["write"]) * 1024))
+
+
+def storage_variants(ctx, field, steps: int) -> dict:
+    """The same fft2d step with the batch stored in FP64, FP16 and BF16 (FP32
+    arithmetic for the half-width ones): ms per step and algorithmic GB/s at
+    the storage's own bytes (re + im read and written per pass)."""
+    out = {}
+    re, im = None, None
+    for dt, size in (("f64", 8), ("f16", 2), ("bf16", 2)):
+        g = ctx.field(N, N, B, dt)
+        for b in range(B):  # convert field by field through the host
+            src = ctx.wrap(field.ptrs()[0] + b * N * N * 4, field.ptrs()[1] + b * N * N * 4, N, N, 1, "f32")
+            re, im = src.download()
+            src.free()
+            dst_re = g.ptrs()[0] + b * N * N * size
+            dst_im = g.ptrs()[1] + b * N * N * size
+            dst = ctx.wrap(dst_re, dst_im, N, N, 1, dt)
+            dst.upload(re, im)
+            dst.free()
+        ctx.autotune_fft2d(N, N, B, dt, reps=2)  # the configurations that exist for this storage
+        ctx.fft2d(g)
+        ctx.sync()
+        reps = max(3, steps // 2)
+        ctx.timer_start()
+        for _ in range(reps):
+            ctx.fft2d(g)
+        ms = ctx.timer_stop() / reps
+        # two passes, each reading and writing both planes
+        out[dt] = {"ms_per_step": round(ms, 4), "gbs_alg": round(8 * size * B * N * N / (ms * 1e-3) / 1e9, 1),
+                   "bytes_per_pt": 8 * size, "tuning": ctx.tune_get(N, N, B, dt)}
+        g.free()
+    return out
 
 
 def wgs_leg(ctx, train: int, steps: int, count: int, iters: int) -> dict:
@@ -222,6 +257,7 @@ def main() -> int:
     ctx.set_flags(asynchronous=True, profile=False)
     ctx.sync()
     scratch.free()
+    variants = storage_variants(ctx, field, args.steps)
     wgs = None if args.no_wgs else wgs_leg(ctx, args.wgs_train, args.wgs_steps, args.wgs_eval,
                                            args.wgs_iters)
 
@@ -258,6 +294,7 @@ def main() -> int:
         "tuning": tuning,
         "kernels": kernels,
         "pipeline": pipeline,
+        "storage_variants": variants,
         "wgs": wgs,
         "cpu_baseline": cpu,
     }
