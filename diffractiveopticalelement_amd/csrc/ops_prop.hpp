@@ -77,9 +77,17 @@ struct AsmField {
 struct OpAsmH {
   static constexpr int NACC = 0;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
-  // column tiles of 32 points per thread, both LDS planes, 512 threads: the
-  // FP64 transfer-function argument fits the 256-register budget unspilled
-  static constexpr int COL_P = 32, COL_PLANES = 2;
+  // FP32 column tiles.  Lines of 256 and 512 points: 16 points per thread x 16 columns,
+  // both LDS planes (512 threads at 128 registers, two workgroups per CU);
+  // measured on 16 x 2048^2: 367-371 us per pass against 478 us for 32
+  // points x 32 columns at one workgroup per CU (472 us with one plane), 410
+  // us for 8-column tiles and 375 us for 16 x 16 with one plane.  Longer
+  // lines would spill at 16 points per thread (1024-thread workgroups) and
+  // keep 32 points per thread with both planes (the FP64 transfer-function
+  // argument fits the 256-register budget unspilled).
+  template <typename TT, int N>
+  using ColGeomT = typename std::conditional<(N <= 512), ColGeom<TT, N, 16, 16, 2>,
+                                             ColGeom<TT, N, 32, 64, 2>>::type;
   const AsmField* fp;
   double dfx, dfy, inv_l2;
   int nx, ny, evanescent;
@@ -162,6 +170,13 @@ template <int PRE_MODE, int POST_MODE>
 struct OpChirp {
   static constexpr int NACC = 0;
   static constexpr bool HAS_PRE = PRE_MODE != 0, HAS_POST = POST_MODE != 0;
+  // FP32 column tiles: 512-point lines use the plain transform's tile (32
+  // points x 32 columns, one LDS plane, two workgroups per CU): 228 us per
+  // chirp column pass on 16 x 2048^2 against 280 us for the generic 16 x 64
+  // two-plane tile; longer lines spill with it and keep the generic tile.
+  template <typename TT, int N>
+  using ColGeomT = typename std::conditional<N == 512, ColGeom<TT, N, 32, 32, 1>,
+                                             ColCfg<TT, N, false>>::type;
   const ChirpField* fp;
   int nx, ny;
   OpChirp at(int f0) const {
