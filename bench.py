@@ -14,6 +14,10 @@ step, timed with HIP events on the library's stream; `cpu_baseline` is the
 FP64 JavaScript oracle (the reference's algorithm: radix-2 FFT) on one node
 thread, on a bounded sample.
 
+"gs_batch_uneven" runs the batched uneven GS configuration: 256 far-field
+WGS designs of sides {256, 512, 1024, 2048} (spot arrays, filled polygons),
+200 graph-captured iterations.
+
 "storage_variants" times the same fft2d step with the batch stored in FP64,
 FP16 and BF16 (the half-width ones compute in FP32).
 
@@ -120,6 +124,79 @@ def storage_variants(ctx, field, steps: int) -> dict:
     return out
 
 
+def c3_targets(n: int, rng, polygon: bool) -> np.ndarray:
+    """binary target irradiance of the batched uneven GS configuration: 4-64
+    Gaussian spots (sigma 1.5 px, uniform weights), or one random filled
+    convex polygon (3-8 vertices), inside the central half of the plane"""
+    I = np.zeros((n, n))
+    lo, hi = n // 4, 3 * n // 4
+    if not polygon:
+        k = int(rng.integers(4, 65))
+        d = np.arange(-4, 5)
+        g = np.exp(-(d[None, :] ** 2 + d[:, None] ** 2) / (2 * 1.5 ** 2))
+        for y, x in zip(rng.integers(lo + 4, hi - 4, k), rng.integers(lo + 4, hi - 4, k)):
+            I[y - 4:y + 5, x - 4:x + 5] += g
+        return (I > 0.5).astype(float)
+    m = int(rng.integers(3, 9))
+    c = rng.uniform(0.4 * n, 0.6 * n, 2)
+    r = rng.uniform(0.05 * n, 0.2 * n)
+    ang = np.sort(rng.uniform(0, 2 * np.pi, m))
+    vy, vx = c[0] + r * np.sin(ang), c[1] + r * np.cos(ang)
+    yy, xx = np.mgrid[lo:hi, lo:hi].astype(float)
+    inside = np.ones_like(yy, dtype=bool)
+    for a in range(m):  # convex: inside every edge's half plane (counter-clockwise)
+        b = (a + 1) % m
+        inside &= (vx[b] - vx[a]) * (yy - vy[a]) - (vy[b] - vy[a]) * (xx - vx[a]) >= 0
+    I[lo:hi, lo:hi] = inside
+    return I
+
+
+def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7) -> dict:
+    """Batched uneven far-field WGS: `count` designs, sides drawn from
+    {256, 512, 1024, 2048}, half spot arrays and half filled polygons, binary,
+    power-matched to a top-hat illumination; `iters` WGS iterations in one
+    graph-captured loop.  value: designs per second; GB/s at 52 B per point
+    and iteration (two passes + illumination, target, weights)."""
+    from diffractiveopticalelement_amd import warmstart as ws
+
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([256, 512, 1024, 2048], count).tolist()
+    t0 = time.perf_counter()
+    b = nat_batch(ctx, sides)
+    ills = {n: ws.illumination(n) for n in set(sides)}
+    for i, n in enumerate(sides):
+        I = c3_targets(n, rng, polygon=bool(i % 2))
+        t = np.sqrt(I)
+        t *= np.sqrt(np.sum(ills[n] ** 2) / max(np.sum(t ** 2), 1e-300))
+        b.set(i, ills[n], t)
+    b.random_init(seed)
+    setup_s = time.perf_counter() - t0
+    b.run(5, graph=True)  # warm-up (captures a 5-iteration graph)
+    ctx.sync()
+    ctx.timer_start()
+    b.run(iters, graph=True)
+    ms = ctx.timer_stop()
+    pts = sum(n * n for n in sides)
+    best = [b.get(i)["error"][-1] for i in range(0, count, max(1, count // 8))]
+    b.close()
+    return {
+        "config": {"workload": f"{count} far-field WGS designs, sides from {{256, 512, 1024, 2048}}^2 "
+                               f"(seed {seed}), spot arrays and filled polygons, {iters} iterations, "
+                               "FP32, hipGraph-captured", "points": pts,
+                   "sides": {str(n): sides.count(n) for n in sorted(set(sides))}},
+        "ms": round(ms, 2), "designs_per_s": round(count / (ms * 1e-3), 1),
+        "us_per_design_iteration_mean": round(ms * 1e3 / iters / count, 2),
+        "gbs_alg": round(52 * pts * iters / (ms * 1e-3) / 1e9, 1), "setup_s": round(setup_s, 1),
+        "sample_best_rmse": [round(float(e), 4) for e in best],
+    }
+
+
+def nat_batch(ctx, sides):
+    from diffractiveopticalelement_amd import native as nat
+
+    return nat.GsBatch(ctx, sides, nat.ALG_WGS, 0.9)
+
+
 def wgs_leg(ctx, train: int, steps: int, count: int, iters: int) -> dict:
     """learned warm start vs plain WGS at 1024^2 (far field), bounded"""
     from diffractiveopticalelement_amd import warmstart as ws
@@ -165,6 +242,7 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the fft2d autotuner (built-in configuration)")
     ap.add_argument("--no-wgs", action="store_true", help="skip the WGS warm-start leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven GS leg")
     ap.add_argument("--wgs-train", type=int, default=128)
     ap.add_argument("--wgs-steps", type=int, default=600)
     ap.add_argument("--wgs-eval", type=int, default=32)
@@ -258,6 +336,8 @@ def main() -> int:
     ctx.sync()
     scratch.free()
     variants = storage_variants(ctx, field, args.steps)
+    field.free()
+    c3 = None if args.no_c3 else c3_leg(ctx)
     wgs = None if args.no_wgs else wgs_leg(ctx, args.wgs_train, args.wgs_steps, args.wgs_eval,
                                            args.wgs_iters)
 
@@ -295,6 +375,7 @@ def main() -> int:
         "kernels": kernels,
         "pipeline": pipeline,
         "storage_variants": variants,
+        "gs_batch_uneven": c3,
         "wgs": wgs,
         "cpu_baseline": cpu,
     }
