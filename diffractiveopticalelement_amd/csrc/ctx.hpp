@@ -251,6 +251,9 @@ inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const 
                        dim3(unsigned(grid)), dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim,
                        pl.ny, groups, (const T*)pl.tx.re, (const T*)pl.tx.im, (const T*)pl.ty.re,
                        (const T*)pl.ty.im, next_dir(ctx));
+    if constexpr (Op::NACC > 0)
+      hipLaunchKernelGGL((k_finalize<Op, CF::WG>), dim3(unsigned(batch)), dim3(CF::WG), 0,
+                         ctx->stream, op, int((pl.ny / N1) / CF::G));
     prof_end(ctx);
     return hipGetLastError();
   }
@@ -279,6 +282,9 @@ inline hipError_t launch_col(doe_ctx* ctx, const char* tag, const Op& op, const 
   hipLaunchKernelGGL((k_col_pass<T, S, N2, MODE, INV1, INV2, Op, CF>), dim3(unsigned(grid)),
                      dim3(CF::WG), 0, ctx->stream, op, ire, iim, ore, oim, pl.nx, tiles, pl.n1,
                      (const T*)pl.tc.re, (const T*)pl.tc.im, next_dir(ctx));
+  if constexpr (Op::NACC > 0)
+    hipLaunchKernelGGL((k_finalize<Op, CF::WG>), dim3(unsigned(batch)), dim3(CF::WG), 0,
+                       ctx->stream, op, tiles * pl.n1);
   prof_end(ctx);
   return hipGetLastError();
 }

@@ -55,60 +55,44 @@ __device__ __forceinline__ void block_sum(double (&acc)[NACC], double* sh, int t
   }
 }
 
-// Per-workgroup partials + last-arriving-workgroup finalisation for one field,
-// in two phases so a pass can sum its accumulators as soon as its hooks are
-// done (freeing their registers) and arrive later.
-// Phase 1: block-wide sum of the per-thread accumulators into `out` (valid in
-// thread 0).
-template <int NA, int NTHR>
-__device__ __forceinline__ void wg_sum(double (&acc)[NA], double* out, int tid) {
+// Per-workgroup partial sums, folded per field by a kernel of their own.
+// A pass's workgroup sums its per-thread accumulators (fixed order) and
+// thread 0 publishes them to partials[slot] with plain stores; k_finalize,
+// launched right after the pass, folds each field's partials in slot order
+// and calls op.finalize.  No cross-workgroup synchronisation inside the
+// pass: an arrival counter needs a release fence per workgroup, which on
+// this GPU is an agent-scope L2 writeback (measured: 18-24 % of a far-field
+// WGS column pass).
+template <class Op, int NTHR>
+__device__ __forceinline__ void publish_sums(const Op& op, double (&acc)[Op::NACC], int tid,
+                                             long slot) {
+  constexpr int NA = Op::NACC;
   __shared__ double sh[16 * NA];
   block_sum<NA>(acc, sh, tid, NTHR);
   if (tid == 0) {
 #pragma unroll
-    for (int k = 0; k < NA; ++k) out[k] = acc[k];
+    for (int k = 0; k < NA; ++k) op.partials[slot * NA + k] = acc[k];
   }
 }
-// Phase 2: thread 0 publishes the workgroup's sums to `partials[slot]` and
-// arrives on the field's counter; the last of `wg_in_field` workgroups folds
-// the field's partials in slot order and calls op.finalize.
+
+// One workgroup per field: fold its `per_field` partials (from slot
+// f * per_field on) and finalise.
 template <class Op, int NTHR>
-__device__ __forceinline__ void arrive_finalize(const Op& op, const double* wg_sums, int tid, int f,
-                                                long slot, int wg_in_field, long first_slot) {
+__global__ void __launch_bounds__(NTHR) k_finalize(Op op, int per_field) {
   constexpr int NA = Op::NACC;
   __shared__ double sh[16 * NA];
-  __shared__ int last;
-  if (tid == 0) {
+  const int f = blockIdx.x;
+  const int tid = threadIdx.x;
+  const long first = (long)f * per_field;
+  double s[NA];
 #pragma unroll
-    for (int k = 0; k < NA; ++k) op.partials[slot * NA + k] = wg_sums[k];
-    __threadfence();
-    const unsigned prev = atomicAdd(op.counters + f, 1u);
-    last = (prev == unsigned(wg_in_field - 1));
+  for (int k = 0; k < NA; ++k) s[k] = 0.0;
+  for (int w = tid; w < per_field; w += NTHR) {
+#pragma unroll
+    for (int k = 0; k < NA; ++k) s[k] += op.partials[(first + w) * NA + k];
   }
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    double s[NA];
-#pragma unroll
-    for (int k = 0; k < NA; ++k) s[k] = 0.0;
-    for (int w = tid; w < wg_in_field; w += NTHR) {
-#pragma unroll
-      for (int k = 0; k < NA; ++k) s[k] += op.partials[(first_slot + w) * NA + k];
-    }
-    block_sum<NA>(s, sh, tid, NTHR);
-    if (tid == 0) {
-      op.finalize(f, s);
-      op.counters[f] = 0u;
-    }
-  }
-}
-template <class Op, int NTHR>
-__device__ __forceinline__ void reduce_finalize(const Op& op, double (&acc)[Op::NACC], int tid,
-                                                int f, long slot, int wg_in_field,
-                                                long first_slot) {
-  __shared__ double sums[Op::NACC];
-  wg_sum<Op::NACC, NTHR>(acc, sums, tid);
-  arrive_finalize<Op, NTHR>(op, sums, tid, f, slot, wg_in_field, first_slot);
+  block_sum<NA>(s, sh, tid, NTHR);
+  if (tid == 0) op.finalize(f, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -478,8 +462,6 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
   const int y = n2 + N2 * n1;
   const long base = ((long)f * ny + y) * (long)NX;
   constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
-  constexpr bool EARLY_SUM = false;  // summing before the second FFT measured worse (spills)
-  __shared__ double wg_sums[NA];
   T vr[P], vi[P];
   double acc[NA];
 #pragma unroll
@@ -548,8 +530,6 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
           op.mid(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc);
       }
     }
-    // sums complete after the mid hook: sum them before the second FFT
-    if constexpr (EARLY_SUM) wg_sum<NA, CF::WG>(acc, wg_sums, tid);
     line_fft(std::bool_constant<INV2>{}, std::true_type{}, std::integral_constant<int, CROSS2>{});
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
@@ -575,14 +555,9 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
       st(oim + base + pos, vi[i]);
     }
   }
-  if constexpr (Op::NACC > 0) {
-    // a workgroup's groups never span two fields (the launcher checks G | N2)
-    const int wg_in_field = N2 / CF::G;
-    if constexpr (EARLY_SUM)
-      arrive_finalize<Op, CF::WG>(op, wg_sums, tid, f, wb, wg_in_field, (long)f * wg_in_field);
-    else
-      reduce_finalize<Op, CF::WG>(op, acc, tid, f, wb, wg_in_field, (long)f * wg_in_field);
-  }
+  // a workgroup's groups never span two fields (the launcher checks G | N2),
+  // so slot wb belongs to field f: slots f * (N2 / G) ... (f + 1) * (N2 / G) - 1
+  if constexpr (Op::NACC > 0) publish_sums<Op, CF::WG>(op, acc, tid, wb);
 }
 
 template <typename T, typename S, int NX, int N1, int MODE, bool INV1, bool INV2, bool XIN,
@@ -637,8 +612,6 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
   constexpr bool NAT_IN = MODE == 0 && INV1;
   constexpr bool NAT_OUT = MODE == 0 && !INV1;
   constexpr int NA = Op::NACC > 0 ? Op::NACC : 1;
-  constexpr bool EARLY_SUM = false;  // summing before the second FFT measured worse (spills)
-  __shared__ double wg_sums[NA];
   T vr[P], vi[P];
   double acc[NA];
 #pragma unroll
@@ -688,9 +661,6 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
           op.mid(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
       }
     }
-    // sums complete after the mid hook: sum them across the workgroup before
-    // the second FFT, so the FP64 accumulators are not live across it
-    if constexpr (EARLY_SUM) wg_sum<NA, CF::WG>(acc, wg_sums, tid);
     fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
@@ -709,11 +679,8 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
       st(oim + a, vi[i]);
     }
   }
-  if constexpr (EARLY_SUM) {
-    arrive_finalize<Op, CF::WG>(op, wg_sums, tid, f, lb, per_field, (long)f * per_field);
-  } else if constexpr (Op::NACC > 0) {
-    reduce_finalize<Op, CF::WG>(op, acc, tid, f, lb, per_field, (long)f * per_field);
-  }
+  // slot lb = f * per_field + (k1, tile): field f's slots are contiguous
+  if constexpr (Op::NACC > 0) publish_sums<Op, CF::WG>(op, acc, tid, lb);
 }
 
 template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op,
