@@ -46,6 +46,7 @@ SOURCES = [
     ("nn.hip", [None]),
     ("warmstart.hip", [None]),
     ("metrics.hip", [None]),
+    ("targets.hip", [None]),
 ]
 
 

@@ -245,6 +245,16 @@ static int convert_out(doe_ctx* ctx, const void* src, double* host, long n, int 
   return DOE_OK;
 }
 
+// Storage: any grid of at most 65536 x 65536 points per field.  Transforms
+// and propagators additionally need power-of-two sides (check_field).
+int check_storage(const doe_ctx* ctx, const doe_field* f) {
+  DOE_REQUIRE(ctx != nullptr, DOE_ERR_INVALID, "null context");
+  DOE_REQUIRE(f != nullptr, DOE_ERR_INVALID, "null field");
+  DOE_REQUIRE(f->nx >= 1 && f->ny >= 1 && f->nx <= 65536 && f->ny <= 65536, DOE_ERR_SHAPE,
+              "grid sides must be in [1, 65536]");
+  return DOE_OK;
+}
+
 int check_field(const doe_ctx* ctx, const doe_field* f) {
   DOE_REQUIRE(ctx != nullptr, DOE_ERR_INVALID, "null context");
   DOE_REQUIRE(f != nullptr, DOE_ERR_INVALID, "null field");
@@ -457,7 +467,7 @@ int doe_field_create(doe_ctx* ctx, int nx, int ny, int batch, int dtype, doe_fie
   f->ny = ny;
   f->batch = batch;
   f->dtype = dtype;
-  int rc = check_field(ctx, f);
+  int rc = check_storage(ctx, f);
   if (rc) {
     delete f;
     return rc;
@@ -489,7 +499,7 @@ int doe_field_wrap(doe_ctx* ctx, void* re, void* im, int nx, int ny, int batch, 
   f->re = re;
   f->im = im;
   f->owned = false;
-  int rc = check_field(ctx, f);
+  int rc = check_storage(ctx, f);
   if (rc) {
     delete f;
     return rc;
@@ -521,7 +531,7 @@ int doe_field_info(const doe_field* f, int* nx, int* ny, int* batch, int* dtype,
 }
 
 int doe_field_upload(doe_ctx* ctx, doe_field* f, const double* re, const double* im) {
-  int rc = check_field(ctx, f);
+  int rc = check_storage(ctx, f);
   if (rc) return rc;
   DOE_REQUIRE(re && im, DOE_ERR_INVALID, "null host plane");
   DOE_HIP(hipSetDevice(ctx->device));
@@ -531,7 +541,7 @@ int doe_field_upload(doe_ctx* ctx, doe_field* f, const double* re, const double*
 }
 
 int doe_field_download(doe_ctx* ctx, doe_field* f, double* re, double* im) {
-  int rc = check_field(ctx, f);
+  int rc = check_storage(ctx, f);
   if (rc) return rc;
   DOE_REQUIRE(re && im, DOE_ERR_INVALID, "null host plane");
   DOE_HIP(hipSetDevice(ctx->device));
@@ -541,9 +551,9 @@ int doe_field_download(doe_ctx* ctx, doe_field* f, double* re, double* im) {
 }
 
 int doe_field_copy(doe_ctx* ctx, const doe_field* src, doe_field* dst) {
-  int rc = check_field(ctx, src);
+  int rc = check_storage(ctx, src);
   if (rc) return rc;
-  rc = check_field(ctx, dst);
+  rc = check_storage(ctx, dst);
   if (rc) return rc;
   DOE_REQUIRE(src->nx == dst->nx && src->ny == dst->ny && src->batch == dst->batch &&
                   src->dtype == dst->dtype,
@@ -599,6 +609,11 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   doe_field* f = nullptr;
   int rc = doe_field_create(ctx, nx, ny, batch, dtype, &f);
   if (rc) return rc;
+  rc = check_field(ctx, f);
+  if (rc) {
+    doe_field_free(f);
+    return rc;
+  }
   const size_t bytes = f->count() * dtype_size(dtype);
   if (hipMemsetAsync(f->re, 0, bytes, ctx->stream) != hipSuccess ||
       hipMemsetAsync(f->im, 0, bytes, ctx->stream) != hipSuccess) {

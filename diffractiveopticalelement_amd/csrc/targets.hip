@@ -1,0 +1,782 @@
+// Target rasters, regions of interest and figure images on the device.
+//
+//   makeTarget (projector.js:169-255): every analytic kind evaluated per
+//     pixel; the 'text' and 'image' kinds as a host raster resampled
+//     bilinearly on the device (resampleRaster, projector.js:126-154);
+//   autoRoi (projector.js:262-315): threshold at a fraction of the plane's
+//     maximum, square dilation, box-blurred soft edge, as four separable
+//     passes over byte / integer planes;
+//   blurRaster (doe.js:761-794): separable Gaussian with clamped edges;
+//   planeToRGB / planeToGray (color.js:7-160): a field's scalar quantity
+//     mapped to 8-bit pixels, optionally laid out as PNG scanlines (each row
+//     led by filter byte 0, png.js:96-101), with the data range, or a
+//     percentile of the data as the upper end (an 8-pass radix select over
+//     order-preserving 64-bit keys), found on the device.
+//
+// Arithmetic is FP64 in the reference's operation order with floating-point
+// contraction off, so masks, dilation, box blur, bilinear resampling and the
+// linear colour mapping equal the reference's results bit for bit; exp,
+// sin, pow, log10 and atan2 come from the device math library.  Math.hypot
+// is evaluated as the ECMAScript engine evaluates it (normalised,
+// compensated sum of squares), so hypot-based masks match exactly too.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#include "ctx.hpp"
+
+#pragma clang fp contract(off)
+
+namespace doe {
+namespace {
+
+constexpr int kTB = 256;
+constexpr int kMaxTaps = 65536;  // blur half-width limit
+
+// ---- ECMAScript number semantics used by the reference ---------------------
+
+// Math.hypot(x, y) as V8 evaluates it: the larger magnitude m (Infinity wins
+// over NaN), then sqrt of the Kahan-compensated sum of (|x|/m)^2, (|y|/m)^2,
+// times m.
+__device__ inline double js_hypot(double x, double y) {
+  const double ax = fabs(x), ay = fabs(y);
+  double m = 0;
+  if (ax > m) m = ax;
+  if (ay > m) m = ay;
+  if (m == INFINITY) return INFINITY;
+  if (x != x || y != y) return NAN;
+  if (m == 0) return 0;
+  double sum = 0, comp = 0;
+  const double v[2] = {ax / m, ay / m};
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const double s = v[q] * v[q] - comp;
+    const double p = sum + s;
+    comp = (p - sum) - s;
+    sum = p;
+  }
+  return sqrt(sum) * m;
+}
+
+__host__ __device__ inline double js_max(double a, double b) {
+  if (a != a || b != b) return NAN;
+  if (a == 0 && b == 0) return std::signbit(a) ? b : a;
+  return a > b ? a : b;
+}
+__host__ __device__ inline double js_min(double a, double b) {
+  if (a != a || b != b) return NAN;
+  if (a == 0 && b == 0) return std::signbit(a) ? a : b;
+  return a < b ? a : b;
+}
+// Math.round: nearest integer, ties toward +Infinity.
+__host__ __device__ inline double js_round(double x) {
+  if (x != x || x == INFINITY || x == -INFINITY) return x;
+  const double f = floor(x);
+  return (x - f >= 0.5) ? f + 1 : f;
+}
+// Storing a number into a Uint8Array (ToUint8: truncate, modulo 256).
+__device__ inline unsigned char js_u8(double v) {
+  if (v != v || v == INFINITY || v == -INFINITY) return 0;
+  double m = fmod(trunc(v), 256.0);
+  if (m < 0) m += 256.0;
+  return (unsigned char)m;
+}
+
+// Order-preserving 64-bit key of a double (TypedArray sort order: -0 before
+// +0, every NaN after +Infinity).
+__device__ inline unsigned long long dkey(double v) {
+  unsigned long long u = v != v ? 0x7FF8000000000000ull : (unsigned long long)__double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double dval(unsigned long long k) {
+  const unsigned long long u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+
+template <class S>
+__device__ __forceinline__ double ld(const S* p, long k) {
+  return static_cast<double>(static_cast<float>(p[k]));
+}
+template <>
+__device__ __forceinline__ double ld<float>(const float* p, long k) {
+  return p[k];
+}
+template <>
+__device__ __forceinline__ double ld<double>(const double* p, long k) {
+  return p[k];
+}
+
+// ---- makeTarget ------------------------------------------------------------
+
+struct TgtRec {
+  int kind;
+  int rw, rh;        // raster size
+  int mcols, mrows;  // dark-cell table size (0: none)
+  int gamma_on;
+  long roff;         // raster offset (doubles) in the raster block
+  long moff;         // dark-cell table offset (bytes) in the table block
+  double p0, p1, p2; // kind parameters (radii, pitches and sigma, cells, period and duty)
+  double s, ox, oy;  // raster placement: scale and offsets
+  double gamma, background, gain;
+};
+
+__global__ __launch_bounds__(kTB) void k_target(const TgtRec* __restrict__ recs,
+                                                const double* __restrict__ rasters,
+                                                const unsigned char* __restrict__ dark,
+                                                double* __restrict__ out, int nx, int ny) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const TgtRec& R = recs[b];
+  const int i = int(k % nx), j = int(k / nx);
+  const double cx = (nx - 1) / 2.0, cy = (ny - 1) / 2.0;
+  double v = 0;
+  switch (R.kind) {
+    case DOE_TGT_DISC:
+      v = js_hypot(i - cx, j - cy) <= R.p0 ? 1 : 0;
+      break;
+    case DOE_TGT_RING: {
+      const double r = js_hypot(i - cx, j - cy);
+      v = (r >= R.p0 && r <= R.p1) ? 1 : 0;
+      break;
+    }
+    case DOE_TGT_DOTS: {
+      const double px = R.p0, py = R.p1, sg = R.p2;
+      const double fy = fmod((j + 0.5) / py, 1.0) - 0.5;
+      const double fx = fmod((i + 0.5) / px, 1.0) - 0.5;
+      const double ddx = fx * px, ddy = fy * py;
+      v = exp(-(ddx * ddx + ddy * ddy) / (2 * sg * sg));
+      if (R.mcols > 0) {
+        const long ci = (long)(i / px), cj = (long)(j / py);
+        if (ci < R.mcols && cj < R.mrows && dark[R.moff + cj * R.mcols + ci]) v = 0;
+      }
+      break;
+    }
+    case DOE_TGT_CHECKER: {
+      const double u = floor(((double)i / nx) * R.p0);
+      const double w = floor(((double)j / ny) * R.p0);
+      v = fmod(u + w, 2.0) == 0 ? 1 : 0;
+      break;
+    }
+    case DOE_TGT_LINES:
+      v = (fmod((double)i, R.p0) / R.p0) < R.p1 ? 1 : 0;
+      break;
+    case DOE_TGT_ZONE: {
+      const double r = js_hypot(i - cx, j - cy);
+      v = sin((M_PI * r * r) / R.p0) > 0 ? 1 : 0;
+      break;
+    }
+    case DOE_TGT_RASTER: {
+      v = R.background;
+      const double fy = (j - R.oy) / R.s;
+      const double y0 = floor(fy);
+      if (y0 >= 0 && y0 < R.rh) {
+        const double fx = (i - R.ox) / R.s;
+        const double x0 = floor(fx);
+        if (x0 >= 0 && x0 < R.rw) {
+          const double ty = fy - y0, tx = fx - x0;
+          const long y1 = (long)fmin(y0 + 1, (double)(R.rh - 1));
+          const long x1 = (long)fmin(x0 + 1, (double)(R.rw - 1));
+          const double* src = rasters + R.roff;
+          const long r0 = (long)y0 * R.rw, r1 = y1 * R.rw;
+          const double a = src[(long)x0 + r0] * (1 - tx) * (1 - ty);
+          const double c = src[x1 + r0] * tx * (1 - ty);
+          const double d = src[(long)x0 + r1] * (1 - tx) * ty;
+          const double e = src[x1 + r1] * tx * ty;
+          v = (((a + c) + d) + e) * R.gain;
+        }
+      }
+      if (R.gamma_on) v = pow(js_max(0, v), R.gamma);
+      break;
+    }
+  }
+  out[(long)b * n + k] = v;
+}
+
+// ---- autoRoi ---------------------------------------------------------------
+
+// Per-field maximum over irr > m (NaN never selected, start at +0): the
+// non-negative doubles order like their bit patterns.
+__global__ __launch_bounds__(kTB) void k_roi_max(const double* __restrict__ irr, long n,
+                                                 unsigned long long* __restrict__ mx) {
+  const int b = blockIdx.y;
+  const double* p = irr + (long)b * n;
+  double m = 0;
+  for (long k = (long)blockIdx.x * kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
+    const double v = p[k];
+    if (v > m) m = v;
+  }
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double red[kTB / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kTB / 64; ++w) m = fmax(m, red[w]);
+    atomicMax(mx + b, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
+}
+
+// tmp = row-wise dilation (half-width r) of irr > threshold * max.
+__global__ __launch_bounds__(kTB) void k_roi_row(const double* __restrict__ irr,
+                                                 const unsigned long long* __restrict__ mx,
+                                                 unsigned char* __restrict__ tmp, int nx, int ny,
+                                                 int r, double threshold) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const double thr = threshold * __longlong_as_double((long long)mx[b]);
+  const int i = int(k % nx);
+  const double* row = irr + (long)b * n + (k - i);
+  const int i0 = max(0, i - r), i1 = min(nx - 1, i + r);
+  unsigned char v = 0;
+  for (int ii = i0; ii <= i1 && !v; ++ii) v = row[ii] > thr;
+  tmp[(long)b * n + k] = v;
+}
+
+// Column-wise dilation of tmp; the dilated mask goes to dil (bytes) or,
+// without a soft edge, straight to the ROI plane.
+__global__ __launch_bounds__(kTB) void k_roi_col(const unsigned char* __restrict__ tmp,
+                                                 unsigned char* __restrict__ dil,
+                                                 double* __restrict__ roi, int nx, int ny, int r) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const int i = int(k % nx), j = int(k / nx);
+  const unsigned char* col = tmp + (long)b * n + i;
+  const int j0 = max(0, j - r), j1 = min(ny - 1, j + r);
+  unsigned char v = 0;
+  for (int jj = j0; jj <= j1 && !v; ++jj) v = col[(long)jj * nx];
+  if (roi)
+    roi[(long)b * n + k] = v;
+  else
+    dil[(long)b * n + k] = v;
+}
+
+// Row sums of dil over the clipped window [i-s, i+s].
+__global__ __launch_bounds__(kTB) void k_box_row(const unsigned char* __restrict__ dil,
+                                                 int* __restrict__ rs, int nx, int ny, int s) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const int i = int(k % nx);
+  const unsigned char* row = dil + (long)b * n + (k - i);
+  const int i0 = max(0, i - s), i1 = min(nx - 1, i + s);
+  int acc = 0;
+  for (int ii = i0; ii <= i1; ++ii) acc += row[ii];
+  rs[(long)b * n + k] = acc;
+}
+
+// Column sums of the row sums over [j-s, j+s], divided by the clipped
+// window's sample count: exact integers, one rounding in the division.
+__global__ __launch_bounds__(kTB) void k_box_col(const int* __restrict__ rs, double* __restrict__ roi,
+                                                 int nx, int ny, int s) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const int i = int(k % nx), j = int(k / nx);
+  const int* col = rs + (long)b * n + i;
+  const int j0 = max(0, j - s), j1 = min(ny - 1, j + s);
+  const int i0 = max(0, i - s), i1 = min(nx - 1, i + s);
+  long acc = 0;
+  for (int jj = j0; jj <= j1; ++jj) acc += col[(long)jj * nx];
+  roi[(long)b * n + k] = (double)acc / (double)((long)(j1 - j0 + 1) * (i1 - i0 + 1));
+}
+
+// ---- blurRaster --------------------------------------------------------------
+
+// Row pass: tmp = sum_d src[clamp(i + d)] * ker[d + r], d = -r..r in order.
+__global__ __launch_bounds__(kTB) void k_blur_row(const double* __restrict__ src, double* __restrict__ tmp,
+                                                  const double* __restrict__ ker, int nx, int ny, int r) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const int i = int(k % nx);
+  const double* row = src + (long)b * n + (k - i);
+  double acc = 0;
+  for (int d = -r; d <= r; ++d) acc += row[min(nx - 1, max(0, i + d))] * ker[d + r];
+  tmp[(long)b * n + k] = acc;
+}
+
+__global__ __launch_bounds__(kTB) void k_blur_col(const double* __restrict__ tmp, double* __restrict__ dst,
+                                                  const double* __restrict__ ker, int nx, int ny, int r) {
+  const int b = blockIdx.y;
+  const long n = (long)nx * ny;
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const int i = int(k % nx), j = int(k / nx);
+  const double* col = tmp + (long)b * n + i;
+  double acc = 0;
+  for (int d = -r; d <= r; ++d) acc += col[(long)min(ny - 1, max(0, j + d)) * nx] * ker[d + r];
+  dst[(long)b * n + k] = acc;
+}
+
+// ---- planeToRGB / planeToGray ------------------------------------------------
+
+// The scalar quantity of sample k (ComplexField re / im / intensity /
+// amplitude / phase, field.js:98-135).
+template <class S>
+__device__ __forceinline__ double quantity(const S* re, const S* im, long k, int q) {
+  const double a = ld(re, k);
+  if (q == DOE_Q_RE) return a;
+  const double b = ld(im, k);
+  switch (q) {
+    case DOE_Q_IM: return b;
+    case DOE_Q_INTENSITY: return a * a + b * b;
+    case DOE_Q_AMPLITUDE: return js_hypot(a, b);
+    default: return atan2(b, a);
+  }
+}
+
+// mm[0] = min key, mm[1] = max key over the plane (NaN keys skipped unless
+// with_nan: the sort order's first element).
+template <class S>
+__global__ __launch_bounds__(kTB) void k_img_minmax(const S* __restrict__ re, const S* __restrict__ im,
+                                                    long n, int q, int with_nan,
+                                                    unsigned long long* __restrict__ mm) {
+  unsigned long long lo = ~0ull, hi = 0;
+  for (long k = (long)blockIdx.x * kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
+    const double v = quantity(re, im, k, q);
+    if (v != v && !with_nan) continue;
+    const unsigned long long key = dkey(v);
+    lo = key < lo ? key : lo;
+    hi = key > hi ? key : hi;
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, off, 64), c = __shfl_xor(hi, off, 64);
+    lo = a < lo ? a : lo;
+    hi = c > hi ? c : hi;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(mm, lo);
+    atomicMax(mm + 1, hi);
+  }
+}
+
+// One radix-select pass: histogram of the 8-bit digit at `shift` over the
+// keys that match the selected prefix so far.
+template <class S>
+__global__ __launch_bounds__(kTB) void k_sel_hist(const S* __restrict__ re, const S* __restrict__ im,
+                                                  long n, int q, const unsigned long long* __restrict__ st,
+                                                  unsigned* __restrict__ hist, int shift) {
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned long long prefix = st[0], mask = st[1];
+  for (long k = (long)blockIdx.x * kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
+    const unsigned long long key = dkey(quantity(re, im, k, q));
+    if ((key & mask) == prefix) atomicAdd(&h[(key >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// st = {prefix, mask, rank}: pick the digit that holds the rank, clear the
+// histogram for the next pass.
+__global__ void k_sel_pick(unsigned long long* __restrict__ st, unsigned* __restrict__ hist, int shift) {
+  if (threadIdx.x == 0) {
+    unsigned long long rank = st[2], cum = 0;
+    int d = 255;
+    for (int q = 0; q < 256; ++q) {
+      if (cum + hist[q] > rank) {
+        d = q;
+        break;
+      }
+      cum += hist[q];
+    }
+    st[2] = rank - cum;
+    st[0] |= (unsigned long long)d << shift;
+    st[1] |= 255ull << shift;
+  }
+  __syncthreads();
+  hist[threadIdx.x] = 0;
+}
+
+struct ImgParams {
+  int q, gray, nstops, log, invert, png, has_min, has_max, pct, gamma_on;
+  double min, max, gamma, log_floor, lf;
+  double pos[8];
+  double rgb[8][3];
+};
+
+// range[0] = lo, range[1] = hi as planeToRGB / planeToGray settle them.
+__global__ void k_img_range(const ImgParams P, const unsigned long long* __restrict__ mm,
+                            const unsigned long long* __restrict__ st, double* __restrict__ range) {
+  if (threadIdx.x != 0) return;
+  range[0] = P.has_min ? P.min : dval(mm[0]);
+  range[1] = P.has_max ? P.max : (P.pct ? dval(st[0]) : dval(mm[1]));
+}
+
+template <class S>
+__global__ __launch_bounds__(kTB) void k_image(const S* __restrict__ re, const S* __restrict__ im, int nx,
+                                               long n, const ImgParams P, const double* __restrict__ range,
+                                               unsigned char* __restrict__ out) {
+  const long k = (long)blockIdx.x * kTB + threadIdx.x;
+  if (k >= n) return;
+  const double lo = range[0], hi = range[1];
+  double span = hi - lo;
+  if (span == 0 || span != span) span = 1;
+  const double p = quantity(re, im, k, P.q);
+  const int ch = P.gray ? 1 : 3;
+  const long x = k % nx, y = k / nx;
+  const long stride = (long)P.png + (long)nx * ch;
+  unsigned char* o = out + y * stride + P.png + x * ch;
+  if (P.png && x == 0) out[y * stride] = 0;
+  if (P.gray) {
+    double t = js_max(0, js_min(1, (p - lo) / span));
+    if (P.invert) t = 1 - t;
+    o[0] = js_u8(js_round(t * 255));
+    return;
+  }
+  double t;
+  if (P.log) {
+    const double v = js_max(p, hi * P.log_floor);
+    t = (log10(v / hi) - P.lf) / -P.lf;
+  } else {
+    t = (p - lo) / span;
+  }
+  if (P.gamma_on) t = pow(js_max(0, js_min(1, t)), P.gamma);
+  const double xx = js_min(1, js_max(0, t));
+  for (int s = 1; s < P.nstops; ++s) {
+    const double p0 = P.pos[s - 1], p1 = P.pos[s];
+    if (xx <= p1) {
+      const double f = p1 == p0 ? 0 : (xx - p0) / (p1 - p0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double c0 = P.rgb[s - 1][c], c1 = P.rgb[s][c];
+        o[c] = js_u8(js_round(c0 + (c1 - c0) * f));
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) o[c] = js_u8(P.rgb[P.nstops - 1][c]);
+}
+
+inline unsigned blocks_for(long n) { return unsigned((n + kTB - 1) / kTB); }
+inline unsigned red_blocks(long n) {
+  const long b = (n + kTB * 8 - 1) / (kTB * 8);
+  return unsigned(std::max(1L, std::min(b, 2048L)));
+}
+
+template <class F>
+int with_dt(int dt, F&& f) {
+  switch (dt) {
+    case DOE_F32: return f(float{});
+    case DOE_F64: return f(double{});
+    case DOE_F16: return f(__half{});
+    case DOE_BF16: return f(__hip_bfloat16{});
+  }
+  set_error("unknown dtype");
+  return DOE_ERR_INVALID;
+}
+
+}  // namespace
+}  // namespace doe
+
+using namespace doe;
+
+extern "C" {
+
+int doe_target_spec_init(doe_target_spec* s, int kind) {
+  DOE_REQUIRE(s, DOE_ERR_INVALID, "null argument");
+  std::memset(s, 0, sizeof(*s));
+  s->kind = kind;
+  s->radius = 0.35;
+  s->inner = 0.25;
+  s->outer = 0.4;
+  s->cols = 8;
+  s->rows = 8;
+  s->sigma = 0.22;
+  s->cells = 8;
+  s->period = kind == DOE_TGT_LINES ? 12 : 16;
+  s->duty = 0.5;
+  s->gamma = 1;
+  s->gain = 1;
+  return DOE_OK;
+}
+
+int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) {
+  DOE_REQUIRE(ctx && specs && out, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(out->dtype == DOE_F64, DOE_ERR_INVALID, "makeTarget: the target field is FP64");
+  DOE_REQUIRE(out->nx >= 1 && out->ny >= 1 && out->batch >= 1, DOE_ERR_SHAPE, "makeTarget: empty grid");
+  const int nx = out->nx, ny = out->ny, B = out->batch;
+  std::vector<TgtRec> recs(B);
+  std::vector<double> rasters;
+  std::vector<unsigned char> dark;
+  const double mn = std::min((double)nx, (double)ny);
+  for (int b = 0; b < B; ++b) {
+    const doe_target_spec& S = specs[b];
+    TgtRec& R = recs[b];
+    std::memset(&R, 0, sizeof(R));
+    R.kind = S.kind;
+    switch (S.kind) {
+      case DOE_TGT_DISC: R.p0 = S.radius * mn * 0.5; break;
+      case DOE_TGT_RING:
+        R.p0 = S.inner * mn * 0.5;
+        R.p1 = S.outer * mn * 0.5;
+        break;
+      case DOE_TGT_DOTS: {
+        DOE_REQUIRE(S.cols > 0 && S.rows > 0 && std::isfinite(S.cols) && std::isfinite(S.rows),
+                    DOE_ERR_INVALID, "makeTarget: dots/grid need positive cols and rows");
+        DOE_REQUIRE(S.n_missing == 0 || S.missing, DOE_ERR_INVALID, "makeTarget: missing cells pointer is null");
+        R.p0 = nx / S.cols;
+        R.p1 = ny / S.rows;
+        R.p2 = S.sigma * std::min(R.p0, R.p1);
+        if (S.n_missing > 0) {
+          R.mcols = int((long)((nx - 1) / R.p0) + 1);
+          R.mrows = int((long)((ny - 1) / R.p1) + 1);
+          DOE_REQUIRE((long)R.mcols * R.mrows <= (1L << 24), DOE_ERR_INVALID, "makeTarget: too many cells");
+          R.moff = (long)dark.size();
+          dark.resize(dark.size() + size_t(R.mcols) * R.mrows, 0);
+          for (int m = 0; m < S.n_missing; ++m) {
+            const int mi = S.missing[2 * m], mj = S.missing[2 * m + 1];
+            if (mi >= 0 && mi < R.mcols && mj >= 0 && mj < R.mrows) dark[R.moff + (long)mj * R.mcols + mi] = 1;
+          }
+        }
+        break;
+      }
+      case DOE_TGT_CHECKER: R.p0 = S.cells; break;
+      case DOE_TGT_LINES:
+        R.p0 = S.period;
+        R.p1 = S.duty;
+        break;
+      case DOE_TGT_ZONE: R.p0 = S.period; break;
+      case DOE_TGT_RASTER: {
+        DOE_REQUIRE(S.raster && S.raster_w >= 1 && S.raster_h >= 1, DOE_ERR_INVALID,
+                    "makeTarget: a raster target needs {raster, width, height}");
+        R.rw = S.raster_w;
+        R.rh = S.raster_h;
+        const double sx = (double)nx / S.raster_w, sy = (double)ny / S.raster_h;
+        R.s = S.cover ? std::max(sx, sy) : std::min(sx, sy);
+        R.ox = (nx - S.raster_w * R.s) / 2;
+        R.oy = (ny - S.raster_h * R.s) / 2;
+        R.gamma_on = S.gamma != 1;
+        R.gamma = S.gamma;
+        R.background = S.background;
+        R.gain = S.gain;
+        R.roff = (long)rasters.size();
+        rasters.insert(rasters.end(), S.raster, S.raster + (size_t)S.raster_w * S.raster_h);
+        break;
+      }
+      default: set_error("makeTarget: unknown kind"); return DOE_ERR_INVALID;
+    }
+  }
+  DOE_HIP(hipSetDevice(ctx->device));
+  // one parameter block: records | rasters | dark-cell tables
+  const size_t rb = recs.size() * sizeof(TgtRec), ab = rasters.size() * sizeof(double);
+  std::vector<char> blk(rb + ab + dark.size() + 8);
+  std::memcpy(blk.data(), recs.data(), rb);
+  if (ab) std::memcpy(blk.data() + rb, rasters.data(), ab);
+  if (!dark.empty()) std::memcpy(blk.data() + rb + ab, dark.data(), dark.size());
+  void* dev = nullptr;
+  int rc = upload_params(ctx, blk.data(), blk.size(), &dev);
+  if (rc) return rc;
+  char* base = static_cast<char*>(dev);
+  const long n = (long)nx * ny;
+  hipLaunchKernelGGL(k_target, dim3(blocks_for(n), B), dim3(kTB), 0, ctx->stream, (const TgtRec*)base,
+                     (const double*)(base + rb), (const unsigned char*)(base + rb + ab), (double*)out->re, nx,
+                     ny);
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold) {
+  DOE_REQUIRE(ctx && t, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(t->dtype == DOE_F64, DOE_ERR_INVALID, "autoRoi: the target field is FP64");
+  DOE_REQUIRE(t->nx >= 1 && t->ny >= 1 && t->batch >= 1, DOE_ERR_SHAPE, "autoRoi: empty grid");
+  DOE_REQUIRE(dilate == dilate && soft == soft && threshold == threshold, DOE_ERR_INVALID,
+              "autoRoi: dilate, soft and threshold must be numbers");
+  const int nx = t->nx, ny = t->ny, B = t->batch;
+  const long n = (long)nx * ny;
+  const int big = std::max(nx, ny);
+  const int r = (int)std::min((double)big, js_max(1, js_round(dilate)));
+  const bool has_soft = !(soft <= 0);
+  const int s = has_soft ? (int)std::min((double)big, js_max(1, js_round(soft))) : 0;
+  DOE_HIP(hipSetDevice(ctx->device));
+  int rc = ensure_work(ctx, size_t(n) * B * sizeof(int));
+  if (rc) return rc;
+  rc = ensure_reduce(ctx, size_t(B), 0);
+  if (rc) return rc;
+  unsigned char* tmp = static_cast<unsigned char*>(ctx->work_re);
+  unsigned char* dil = tmp + size_t(n) * B;
+  int* rs = static_cast<int*>(ctx->work_im);
+  unsigned long long* mx = reinterpret_cast<unsigned long long*>(ctx->partials);
+  double* irr = static_cast<double*>(t->re);
+  double* roi = static_cast<double*>(t->im);
+  hipStream_t st = ctx->stream;
+  const dim3 g(blocks_for(n), B);
+  hipLaunchKernelGGL(k_fill_u64, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, st, mx, B, 0ull);
+  hipLaunchKernelGGL(k_roi_max, dim3(red_blocks(n), B), dim3(kTB), 0, st, irr, n, mx);
+  hipLaunchKernelGGL(k_roi_row, g, dim3(kTB), 0, st, irr, mx, tmp, nx, ny, r, threshold);
+  hipLaunchKernelGGL(k_roi_col, g, dim3(kTB), 0, st, tmp, dil, has_soft ? nullptr : roi, nx, ny, r);
+  if (has_soft) {
+    hipLaunchKernelGGL(k_box_row, g, dim3(kTB), 0, st, dil, rs, nx, ny, s);
+    hipLaunchKernelGGL(k_box_col, g, dim3(kTB), 0, st, rs, roi, nx, ny, s);
+  }
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_raster_blur(doe_ctx* ctx, const doe_field* src, doe_field* dst, double sigma, const double* taps) {
+  DOE_REQUIRE(ctx && src && dst, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(src->dtype == DOE_F64 && dst->dtype == DOE_F64, DOE_ERR_INVALID, "blurRaster: FP64 fields");
+  DOE_REQUIRE(src->nx == dst->nx && src->ny == dst->ny && src->batch == dst->batch, DOE_ERR_SHAPE,
+              "blurRaster: source and destination grids differ");
+  DOE_REQUIRE(sigma == sigma, DOE_ERR_INVALID, "blurRaster: sigma must be a number");
+  const int nx = src->nx, ny = src->ny, B = src->batch;
+  const long n = (long)nx * ny;
+  DOE_HIP(hipSetDevice(ctx->device));
+  if (sigma <= 0) {
+    if (src->re != dst->re)
+      DOE_HIP(hipMemcpyAsync(dst->re, src->re, size_t(n) * B * sizeof(double), hipMemcpyDeviceToDevice,
+                             ctx->stream));
+    DOE_HIP(finish(ctx));
+    return DOE_OK;
+  }
+  const double rr = std::max(1.0, std::ceil(3 * sigma));
+  DOE_REQUIRE(rr <= kMaxTaps, DOE_ERR_INVALID, "blurRaster: sigma too large (half-width above 65536)");
+  const int r = (int)rr;
+  std::vector<double> ker(2 * r + 1);
+  if (taps) {
+    std::copy(taps, taps + ker.size(), ker.begin());
+  } else {
+    double ks = 0;
+    for (int i = -r; i <= r; ++i) {
+      ker[i + r] = std::exp(-(double)(i * (long)i) / (2 * sigma * sigma));
+      ks += ker[i + r];
+    }
+    for (double& k : ker) k /= ks;
+  }
+  void* dker = nullptr;
+  int rc = upload_params(ctx, ker.data(), ker.size() * sizeof(double), &dker);
+  if (rc) return rc;
+  rc = ensure_work(ctx, size_t(n) * B * sizeof(double));
+  if (rc) return rc;
+  double* tmp = static_cast<double*>(ctx->work_re);
+  const dim3 g(blocks_for(n), B);
+  hipLaunchKernelGGL(k_blur_row, g, dim3(kTB), 0, ctx->stream, (const double*)src->re, tmp,
+                     (const double*)dker, nx, ny, r);
+  hipLaunchKernelGGL(k_blur_col, g, dim3(kTB), 0, ctx->stream, (const double*)tmp, (double*)dst->re,
+                     (const double*)dker, nx, ny, r);
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_image_opts_init(doe_image_opts* o) {
+  DOE_REQUIRE(o, DOE_ERR_INVALID, "null argument");
+  std::memset(o, 0, sizeof(*o));
+  o->quantity = DOE_Q_RE;
+  o->nstops = 2;
+  o->stop_pos[1] = 1;
+  for (int c = 0; c < 3; ++c) o->stop_rgb[1][c] = 255;
+  o->gamma = 1;
+  o->log_floor = 1e-4;
+  return DOE_OK;
+}
+
+int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image_opts* o, unsigned char* out,
+                    long out_len, double* range_out) {
+  DOE_REQUIRE(ctx && f && o && out, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(index >= 0 && index < f->batch, DOE_ERR_SHAPE, "image: field index outside the batch");
+  DOE_REQUIRE(o->quantity >= DOE_Q_RE && o->quantity <= DOE_Q_PHASE, DOE_ERR_INVALID, "image: unknown quantity");
+  DOE_REQUIRE(o->gray || (o->nstops >= 1 && o->nstops <= 8), DOE_ERR_INVALID,
+              "image: a colour map has 1 to 8 stops");
+  DOE_REQUIRE(!o->has_percentile || (o->percentile >= 0 && o->percentile == o->percentile), DOE_ERR_INVALID,
+              "image: percentile must be a non-negative number");
+  const int nx = f->nx, ny = f->ny;
+  const long n = (long)nx * ny;
+  DOE_REQUIRE(n >= 1, DOE_ERR_SHAPE, "image: empty grid");
+  const int ch = o->gray ? 1 : 3;
+  const long need = (long)ny * ((o->png_rows ? 1 : 0) + (long)nx * ch);
+  DOE_REQUIRE(out_len >= need, DOE_ERR_SHAPE, "image: output buffer too small");
+  ImgParams P;
+  std::memset(&P, 0, sizeof(P));
+  P.q = o->quantity;
+  P.gray = o->gray ? 1 : 0;
+  P.nstops = o->nstops;
+  P.log = o->log ? 1 : 0;
+  P.invert = o->invert ? 1 : 0;
+  P.png = o->png_rows ? 1 : 0;
+  P.has_min = o->has_min ? 1 : 0;
+  P.has_max = o->has_max ? 1 : 0;
+  // planeToGray has no percentile option (color.js:140-160)
+  P.pct = (!P.gray && o->has_percentile) ? 1 : 0;
+  P.gamma_on = o->gamma != 1;
+  P.min = o->min;
+  P.max = o->max;
+  P.gamma = o->gamma;
+  P.log_floor = o->log_floor;
+  P.lf = std::log10(js_max(1e-12, o->log_floor));
+  for (int s = 0; s < 8; ++s) {
+    P.pos[s] = o->stop_pos[s];
+    for (int c = 0; c < 3; ++c) P.rgb[s][c] = o->stop_rgb[s][c];
+  }
+  DOE_HIP(hipSetDevice(ctx->device));
+  int rc = ensure_work(ctx, size_t(need));
+  if (rc) return rc;
+  // reduction scratch: mm[2], st[3], range[2] (doubles), hist[256] (u32)
+  rc = ensure_reduce(ctx, 8 + 128, 0);
+  if (rc) return rc;
+  unsigned long long* mm = reinterpret_cast<unsigned long long*>(ctx->partials);
+  unsigned long long* st = mm + 2;
+  double* range = ctx->partials + 5;
+  unsigned* hist = reinterpret_cast<unsigned*>(ctx->partials + 8);
+  unsigned char* img = static_cast<unsigned char*>(ctx->work_re);
+  hipStream_t sm = ctx->stream;
+  const size_t plane = size_t(n) * dtype_size(f->dtype);
+  const void* re = static_cast<const char*>(f->re) + plane * index;
+  const void* im = static_cast<const char*>(f->im) + plane * index;
+  const bool need_scan = !(P.has_min && P.has_max);
+  rc = with_dt(f->dtype, [&](auto tag) -> int {
+    using S = decltype(tag);
+    const S* r_ = static_cast<const S*>(re);
+    const S* i_ = static_cast<const S*>(im);
+    if (need_scan) {
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, mm, 1, ~0ull);
+      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, mm + 1, 1, 0ull);
+      hipLaunchKernelGGL(k_img_minmax<S>, dim3(red_blocks(n)), dim3(kTB), 0, sm, r_, i_, n, P.q, P.pct, mm);
+      if (P.pct && !P.has_max) {
+        const double kk = std::floor((double)n * o->percentile);
+        const unsigned long long rank = (unsigned long long)std::min((double)(n - 1), kk);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, st, 2, 0ull);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, st + 2, 1, rank);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, sm,
+                           reinterpret_cast<unsigned long long*>(hist), 128, 0ull);
+        for (int shift = 56; shift >= 0; shift -= 8) {
+          hipLaunchKernelGGL(k_sel_hist<S>, dim3(red_blocks(n)), dim3(kTB), 0, sm, r_, i_, n, P.q,
+                             (const unsigned long long*)st, hist, shift);
+          hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(256), 0, sm, st, hist, shift);
+        }
+      }
+    }
+    hipLaunchKernelGGL(k_img_range, dim3(1), dim3(64), 0, sm, P, (const unsigned long long*)mm,
+                       (const unsigned long long*)st, range);
+    hipLaunchKernelGGL(k_image<S>, dim3(blocks_for(n)), dim3(kTB), 0, sm, r_, i_, nx, n, P,
+                       (const double*)range, img);
+    return DOE_OK;
+  });
+  if (rc) return rc;
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(hipMemcpyAsync(out, img, size_t(need), hipMemcpyDeviceToHost, sm));
+  if (range_out) DOE_HIP(hipMemcpyAsync(range_out, range, 2 * sizeof(double), hipMemcpyDeviceToHost, sm));
+  DOE_HIP(hipStreamSynchronize(sm));
+  return DOE_OK;
+}
+
+}  // extern "C"

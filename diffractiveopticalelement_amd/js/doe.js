@@ -324,7 +324,9 @@ function weightedHost(illumination, targetAmp, o, baseFn) {
   return result;
 }
 
-// Separable clamped Gaussian blur (ref: doe.js:761-794).
+// Separable clamped Gaussian blur (ref: doe.js:761-794) on the GPU
+// (doe_raster_blur).  The taps are built here exactly as the reference
+// builds them, so the device sums the same products in the same order.
 export function blurRaster(src, nx, ny, sigmaPx = 1) {
   if (sigmaPx <= 0) return Float64Array.from(src);
   const r = Math.max(1, Math.ceil(3 * sigmaPx));
@@ -332,23 +334,14 @@ export function blurRaster(src, nx, ny, sigmaPx = 1) {
   let ks = 0;
   for (let i = -r; i <= r; i++) ks += ker[i + r] = Math.exp(-(i * i) / (2 * sigmaPx * sigmaPx));
   for (let i = 0; i < ker.length; i++) ker[i] /= ks;
-  const pass = (inp, horizontal) => {
-    const out = new Float64Array(inp.length);
-    for (let j = 0; j < ny; j++) {
-      for (let i = 0; i < nx; i++) {
-        let acc = 0;
-        for (let d = -r; d <= r; d++) {
-          const k = horizontal
-            ? Math.min(nx - 1, Math.max(0, i + d)) + j * nx
-            : i + Math.min(ny - 1, Math.max(0, j + d)) * nx;
-          acc += inp[k] * ker[d + r];
-        }
-        out[i + j * nx] = acc;
-      }
-    }
-    return out;
-  };
-  return pass(pass(src, true), false);
+  const plane = src instanceof Float64Array ? src : Float64Array.from(src);
+  if (plane.length !== nx * ny) throw new Error(`blurRaster: raster length ${plane.length} does not match ${nx}x${ny}`);
+  const dev = native.fieldCreate(nx, ny, 1, 1);
+  native.fieldUpload(dev, plane, plane);
+  native.rasterBlur(dev, dev, sigmaPx, ker);
+  const out = new Float64Array(nx * ny);
+  native.fieldDownload(dev, out, new Float64Array(nx * ny));
+  return out;
 }
 
 function discTarget(nx, ny, frac) {

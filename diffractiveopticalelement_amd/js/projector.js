@@ -1,5 +1,5 @@
 // Drop-in for src/projector.js: the ProjectorSystem pipeline.  Targets and
-// ROIs are built on the host (they are uploaded once per design); design()
+// ROIs are built on the GPU in FP64 (returned as the reference's arrays); design()
 // runs the whole phase-retrieval loop on the GPU; simulate() propagates on
 // the GPU; screenMetrics() judges the device-resident screen on the GPU in
 // FP64 (the screen is never downloaded).
@@ -95,136 +95,101 @@ export function renderText(text, { scale = 8, spacing = 1 } = {}) {
   return { data, width: w2, height: h2 };
 }
 
-// ref: projector.js:126-154
-export function resampleRaster(src, sw, sh, dw, dh, { fit = 'contain', background = 0, gain = 1 } = {}) {
-  const out = new Float64Array(dw * dh).fill(background);
-  const s = fit === 'contain' ? Math.min(dw / sw, dh / sh) : Math.max(dw / sw, dh / sh);
-  const ox = (dw - sw * s) / 2;
-  const oy = (dh - sh * s) / 2;
-  for (let y = 0; y < dh; y++) {
-    const fy = (y - oy) / s;
-    const y0 = Math.floor(fy);
-    if (y0 < 0 || y0 >= sh) continue;
-    const ty = fy - y0;
-    const y1 = Math.min(y0 + 1, sh - 1);
-    for (let x = 0; x < dw; x++) {
-      const fx = (x - ox) / s;
-      const x0 = Math.floor(fx);
-      if (x0 < 0 || x0 >= sw) continue;
-      const tx = fx - x0;
-      const x1 = Math.min(x0 + 1, sw - 1);
-      out[x + y * dw] =
-        (src[x0 + y0 * sw] * (1 - tx) * (1 - ty) +
-          src[x1 + y0 * sw] * tx * (1 - ty) +
-          src[x0 + y1 * sw] * (1 - tx) * ty +
-          src[x1 + y1 * sw] * tx * ty) *
-        gain;
-    }
-  }
+// Reference's arrays are any array-like; the device takes FP64 rasters of at
+// least width * height samples (a short raster reads NaN, as the reference's
+// out-of-range reads do).
+function f64Raster(src, w, h) {
+  const n = Math.max(0, Math.floor(w)) * Math.max(0, Math.floor(h));
+  if (src instanceof Float64Array && src.length >= n) return src;
+  const out = new Float64Array(Math.max(n, src.length)).fill(NaN);
+  for (let k = 0; k < src.length; k++) out[k] = src[k];
   return out;
 }
 
-// ref: projector.js:169-255
+function f64Plane(a) {
+  return a instanceof Float64Array ? a : Float64Array.from(a);
+}
+
+// ref: projector.js:126-154.  Bilinear resample on the GPU (doe_target_make,
+// raster kind): 'contain' or 'cover' placement, background outside the
+// raster, gain.
+export function resampleRaster(src, sw, sh, dw, dh, { fit = 'contain', background = 0, gain = 1 } = {}) {
+  const dev = native.targetMake([{ kind: 'raster', raster: f64Raster(src, sw, sh), width: sw, height: sh,
+    cover: fit !== 'contain', background, gain }], dw, dh);
+  const out = new Float64Array(dw * dh);
+  native.fieldDownload(dev, out, new Float64Array(dw * dh));
+  return out;
+}
+
+// Element-wise equality (NaN equal to NaN) of a stored FP64 copy and an array.
+function sameValues(copy, a) {
+  if (copy.length !== a.length) return false;
+  for (let k = 0; k < a.length; k++) {
+    const x = copy[k];
+    const y = a[k];
+    if (x !== y && !(x !== x && y !== y)) return false;
+  }
+  return true;
+}
+
+// The makeTarget kinds evaluated per pixel on the device.
+const ANALYTIC = new Set(['disc', 'ring', 'dots', 'grid', 'checker', 'lines', 'zone']);
+
+// Dark cells of a spot array as (col, row) int pairs; entries that cannot
+// equal a cell index (non-integers) never match, as in the reference.
+function darkCells(missing) {
+  const pairs = [];
+  for (const m of missing || []) {
+    if (m && Number.isInteger(m[0]) && Number.isInteger(m[1]) && Math.abs(m[0]) < 2 ** 31 && Math.abs(m[1]) < 2 ** 31) {
+      pairs.push(m[0], m[1]);
+    }
+  }
+  return Int32Array.from(pairs);
+}
+
+// ref: projector.js:169-255.  The irradiance and, unless the caller passes
+// one, the ROI are computed on the GPU in FP64 (doe_target_make,
+// doe_target_auto_roi) and returned as host arrays.
 export function makeTarget(kind, nx, ny, opts = {}) {
-  const irr = new Float64Array(nx * ny);
-  const cx = (nx - 1) / 2;
-  const cy = (ny - 1) / 2;
-  const each = (fn) => {
-    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) irr[i + j * nx] = fn(i, j);
-  };
+  let spec;
   let name = kind;
   if (kind === 'image') {
     const { raster, width, height, gamma = 1 } = opts;
     if (!raster) throw new Error('makeTarget("image"): pass {raster, width, height}');
-    const res = resampleRaster(raster, width, height, nx, ny, { fit: 'contain' });
-    for (let k = 0; k < irr.length; k++) irr[k] = gamma === 1 ? res[k] : Math.pow(Math.max(0, res[k]), gamma);
+    spec = { kind: 'raster', raster: f64Raster(raster, width, height), width, height, gamma };
     name = opts.name === undefined ? 'image' : opts.name;
   } else if (kind === 'text') {
     const { text = 'DOE', scale = 6, spacing = 1 } = opts;
     const g = renderText(text, { scale, spacing });
-    irr.set(resampleRaster(g.data, g.width, g.height, nx, ny, { fit: 'contain' }));
+    spec = { kind: 'raster', raster: g.data, width: g.width, height: g.height };
     name = `text:${text}`;
-  } else if (kind === 'disc') {
-    const r = (opts.radius === undefined ? 0.35 : opts.radius) * Math.min(nx, ny) * 0.5;
-    each((i, j) => (Math.hypot(i - cx, j - cy) <= r ? 1 : 0));
-  } else if (kind === 'ring') {
-    const r0 = (opts.inner === undefined ? 0.25 : opts.inner) * Math.min(nx, ny) * 0.5;
-    const r1 = (opts.outer === undefined ? 0.4 : opts.outer) * Math.min(nx, ny) * 0.5;
-    each((i, j) => {
-      const r = Math.hypot(i - cx, j - cy);
-      return r >= r0 && r <= r1 ? 1 : 0;
-    });
-  } else if (kind === 'dots' || kind === 'grid') {
-    const cols = opts.cols === undefined ? 8 : opts.cols;
-    const rows = opts.rows === undefined ? 8 : opts.rows;
-    const px = nx / cols;
-    const py = ny / rows;
-    const sigma = (opts.sigma === undefined ? 0.22 : opts.sigma) * Math.min(px, py);
-    each((i, j) => {
-      const ddx = (((i + 0.5) / px) % 1 - 0.5) * px;
-      const ddy = (((j + 0.5) / py) % 1 - 0.5) * py;
-      let v = Math.exp(-(ddx * ddx + ddy * ddy) / (2 * sigma * sigma));
-      if (opts.missing && opts.missing.some(([mi, mj]) => mi === ((i / px) | 0) && mj === ((j / py) | 0))) v = 0;
-      return v;
-    });
-  } else if (kind === 'checker') {
-    const cells = opts.cells === undefined ? 8 : opts.cells;
-    each((i, j) => ((Math.floor((i / nx) * cells) + Math.floor((j / ny) * cells)) % 2 === 0 ? 1 : 0));
-  } else if (kind === 'lines') {
-    const period = opts.period === undefined ? 12 : opts.period;
-    const duty = opts.duty === undefined ? 0.5 : opts.duty;
-    each((i) => ((i % period) / period < duty ? 1 : 0));
-  } else if (kind === 'zone') {
-    const period = opts.period === undefined ? 16 : opts.period;
-    each((i, j) => {
-      const r = Math.hypot(i - cx, j - cy);
-      return Math.sin((Math.PI * r * r) / period) > 0 ? 1 : 0;
-    });
+  } else if (ANALYTIC.has(kind)) {
+    spec = Object.assign({}, opts, { kind, missing: darkCells(opts.missing), raster: undefined });
   } else {
     throw new Error(`makeTarget: unknown kind "${kind}"`);
   }
-  const roi = opts.roi ? opts.roi : autoRoi(irr, nx, ny, { dilate: opts.dilate === undefined ? 2 : opts.dilate, soft: opts.softEdge === undefined ? 2 : opts.softEdge });
+  const dev = native.targetMake([spec], nx, ny);
+  if (!opts.roi) {
+    native.targetAutoRoi(dev, Number(opts.dilate === undefined ? 2 : opts.dilate),
+      Number(opts.softEdge === undefined ? 2 : opts.softEdge), 0.05);
+  }
+  const irr = new Float64Array(nx * ny);
+  const devRoi = new Float64Array(nx * ny);
+  native.fieldDownload(dev, irr, devRoi);
+  const roi = opts.roi ? opts.roi : devRoi;
   const grid = kind === 'dots' || kind === 'grid' ? { cols: opts.cols === undefined ? 8 : opts.cols, rows: opts.rows === undefined ? 8 : opts.rows } : null;
   return { irradiance: irr, roi, name, grid };
 }
 
-// Threshold, separable dilation, box-blurred edge (ref: projector.js:262-315).
+// Threshold, separable dilation, box-blurred edge (ref: projector.js:262-315),
+// on the GPU: the ROI equals the reference's bit for bit.
 export function autoRoi(irr, nx, ny, { dilate = 2, soft = 2, threshold = 0.05 } = {}) {
-  let mx = 0;
-  for (let k = 0; k < irr.length; k++) mx = Math.max(mx, irr[k]);
-  const on = new Uint8Array(irr.length);
-  for (let k = 0; k < irr.length; k++) on[k] = irr[k] > threshold * mx ? 1 : 0;
-  const r = Math.max(1, Math.round(dilate));
-  const horiz = new Uint8Array(irr.length);
-  for (let j = 0; j < ny; j++) {
-    for (let i = 0; i < nx; i++) {
-      let v = 0;
-      for (let ii = Math.max(0, i - r); ii <= Math.min(nx - 1, i + r) && !v; ii++) v = on[ii + j * nx];
-      horiz[i + j * nx] = v;
-    }
-  }
-  const dil = new Float64Array(irr.length);
-  for (let j = 0; j < ny; j++) {
-    for (let i = 0; i < nx; i++) {
-      let v = 0;
-      for (let jj = Math.max(0, j - r); jj <= Math.min(ny - 1, j + r) && !v; jj++) v = horiz[i + jj * nx];
-      dil[i + j * nx] = v;
-    }
-  }
-  if (soft <= 0) return dil;
-  const s = Math.max(1, Math.round(soft));
-  const out = new Float64Array(irr.length);
-  for (let j = 0; j < ny; j++) {
-    const j0 = Math.max(0, j - s);
-    const j1 = Math.min(ny - 1, j + s);
-    for (let i = 0; i < nx; i++) {
-      const i0 = Math.max(0, i - s);
-      const i1 = Math.min(nx - 1, i + s);
-      let acc = 0;
-      for (let jj = j0; jj <= j1; jj++) for (let ii = i0; ii <= i1; ii++) acc += dil[ii + jj * nx];
-      out[i + j * nx] = acc / ((j1 - j0 + 1) * (i1 - i0 + 1));
-    }
-  }
+  const dev = native.fieldCreate(nx, ny, 1, 1);
+  const plane = f64Plane(irr);
+  native.fieldUpload(dev, plane, plane);
+  native.targetAutoRoi(dev, Number(dilate), Number(soft), Number(threshold));
+  const out = new Float64Array(nx * ny);
+  native.fieldDownload(dev, new Float64Array(nx * ny), out);
   return out;
 }
 
@@ -411,12 +376,21 @@ export class ProjectorSystem {
   // FP64 judge over the downloaded screen irradiance (ref: projector.js:554-614).
   // Device copy of a target's irradiance (re) and ROI (im), FP64, cached per
   // (irradiance, roi) pair.
+  // Device copy of (irradiance, ROI), re-uploaded only when the arrays or
+  // their contents changed since the last call (callers may edit a target
+  // in place between metric calls).
   _targetDevice(irradiance, mask) {
     const c = this._tgtDev;
-    if (c && c.irr === irradiance && c.mask === mask) return c.dev;
+    const m = mask || new Float64Array(irradiance.length).fill(1);
+    if (c && c.irr === irradiance && c.mask === mask && sameValues(c.irrCopy, irradiance) &&
+        sameValues(c.maskCopy, m)) {
+      return c.dev;
+    }
     const dev = native.fieldCreate(this.nx, this.ny, 1, 1);
-    native.fieldUpload(dev, irradiance, mask || new Float64Array(irradiance.length).fill(1));
-    this._tgtDev = { irr: irradiance, mask, dev };
+    const irr64 = Float64Array.from(irradiance);
+    const m64 = Float64Array.from(m);
+    native.fieldUpload(dev, irr64, m64);
+    this._tgtDev = { irr: irradiance, mask, dev, irrCopy: irr64, maskCopy: m64 };
     return dev;
   }
 

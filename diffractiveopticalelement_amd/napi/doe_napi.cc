@@ -8,6 +8,7 @@
 // One process-wide library context on device 0 is created on first use.
 #include <node_api.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -589,6 +590,229 @@ napi_value GsRun(napi_env env, napi_callback_info info) {
   return obj;
 }
 
+// ---- targets, ROIs, figure pixels -------------------------------------------
+napi_value prop(napi_env env, napi_value obj, const char* k) {
+  napi_value v = nullptr;
+  if (napi_get_named_property(env, obj, k, &v) != napi_ok) napi_get_undefined(env, &v);
+  return v;
+}
+bool present(napi_env env, napi_value v) {
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  return t != napi_undefined && t != napi_null;
+}
+double opt_num(napi_env env, napi_value obj, const char* k, double dflt) {
+  napi_value v = prop(env, obj, k);
+  return present(env, v) ? num(env, v, dflt) : dflt;
+}
+int32_t* i32(napi_env env, napi_value v, size_t* len) {
+  bool is = false;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return nullptr;
+  napi_typedarray_type t;
+  size_t n = 0;
+  void* data = nullptr;
+  napi_value ab;
+  size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &t, &n, &data, &ab, &off) != napi_ok || t != napi_int32_array)
+    return nullptr;
+  if (len) *len = n;
+  return (int32_t*)data;
+}
+std::string str(napi_env env, napi_value v) {
+  char buf[64] = {0};
+  size_t n = 0;
+  if (napi_get_value_string_utf8(env, v, buf, sizeof(buf), &n) != napi_ok) return std::string();
+  return std::string(buf, n);
+}
+
+// targetMake(specs[], nx, ny) -> FP64 field handle (batch = specs.length,
+// re = irradiance).  spec: {kind: 'disc'|'ring'|'dots'|'grid'|'checker'|
+// 'lines'|'zone'|'raster', radius, inner, outer, cols, rows, sigma,
+// missing: Int32Array of (col, row) pairs, cells, period, duty,
+// raster: Float64Array, width, height, cover, gamma, background, gain};
+// absent options take the reference's defaults.
+napi_value TargetMake(napi_env env, napi_callback_info info) {
+  Args a;
+  bool is_arr = false;
+  if (!get_args(env, info, a) || a.n < 3 || napi_is_array(env, a.v[0], &is_arr) != napi_ok || !is_arr) {
+    napi_throw_type_error(env, nullptr, "targetMake(specs, nx, ny)");
+    return nullptr;
+  }
+  uint32_t count = 0;
+  napi_get_array_length(env, a.v[0], &count);
+  if (count == 0) {
+    napi_throw_type_error(env, nullptr, "targetMake: no specs");
+    return nullptr;
+  }
+  static const char* kinds[] = {"disc", "ring", "dots", "checker", "lines", "zone", "raster"};
+  std::vector<doe_target_spec> specs(count);
+  for (uint32_t q = 0; q < count; ++q) {
+    napi_value o;
+    napi_get_element(env, a.v[0], q, &o);
+    std::string k = str(env, prop(env, o, "kind"));
+    if (k == "grid") k = "dots";
+    int kind = -1;
+    for (int t = 0; t < 7; ++t)
+      if (k == kinds[t]) kind = t;
+    if (kind < 0) {
+      napi_throw_error(env, nullptr, ("makeTarget: unknown kind \"" + k + "\"").c_str());
+      return nullptr;
+    }
+    doe_target_spec& S = specs[q];
+    doe_target_spec_init(&S, kind);
+    S.radius = opt_num(env, o, "radius", S.radius);
+    S.inner = opt_num(env, o, "inner", S.inner);
+    S.outer = opt_num(env, o, "outer", S.outer);
+    S.cols = opt_num(env, o, "cols", S.cols);
+    S.rows = opt_num(env, o, "rows", S.rows);
+    S.sigma = opt_num(env, o, "sigma", S.sigma);
+    S.cells = opt_num(env, o, "cells", S.cells);
+    S.period = opt_num(env, o, "period", S.period);
+    S.duty = opt_num(env, o, "duty", S.duty);
+    S.gamma = opt_num(env, o, "gamma", S.gamma);
+    S.background = opt_num(env, o, "background", S.background);
+    S.gain = opt_num(env, o, "gain", S.gain);
+    S.cover = boolean(env, prop(env, o, "cover"), false) ? 1 : 0;
+    size_t nm = 0;
+    const int32_t* m = i32(env, prop(env, o, "missing"), &nm);
+    if (m && nm >= 2) {
+      S.missing = m;
+      S.n_missing = int(nm / 2);
+    }
+    if (kind == DOE_TGT_RASTER) {
+      size_t len = 0;
+      S.raster = f64(env, prop(env, o, "raster"), &len);
+      S.raster_w = (int)opt_num(env, o, "width", 0);
+      S.raster_h = (int)opt_num(env, o, "height", 0);
+      if (S.raster && (long)len < (long)S.raster_w * S.raster_h) {
+        napi_throw_range_error(env, nullptr, "makeTarget: raster shorter than width * height");
+        return nullptr;
+      }
+    }
+  }
+  doe_ctx* c = ctx(env);
+  if (!c) return nullptr;
+  doe_field* f = nullptr;
+  if (throw_status(env, doe_field_create(c, (int)num(env, a.v[1]), (int)num(env, a.v[2]), (int)count, DOE_F64, &f)))
+    return nullptr;
+  if (throw_status(env, doe_target_make(c, specs.data(), f))) {
+    doe_field_free(f);
+    return nullptr;
+  }
+  napi_value ext;
+  CHECK_NAPI(napi_create_external(env, f, finalize_field, nullptr, &ext));
+  return ext;
+}
+
+// targetAutoRoi(field, dilate, soft, threshold): ROI of every field into im
+napi_value TargetAutoRoi(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* f = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  if (!f || a.n < 4) {
+    napi_throw_type_error(env, nullptr, "targetAutoRoi(field, dilate, soft, threshold)");
+    return nullptr;
+  }
+  throw_status(env, doe_target_auto_roi(ctx(env), f, num(env, a.v[1], NAN), num(env, a.v[2], NAN),
+                                        num(env, a.v[3], NAN)));
+  return nullptr;
+}
+
+// rasterBlur(src, dst, sigma, taps|null)
+napi_value RasterBlur(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  doe_field* d = a.n > 1 ? field_of(env, a.v[1]) : nullptr;
+  if (!s || !d || a.n < 3) {
+    napi_throw_type_error(env, nullptr, "rasterBlur(src, dst, sigma, taps)");
+    return nullptr;
+  }
+  const double* taps = a.n > 3 ? f64(env, a.v[3], nullptr) : nullptr;
+  throw_status(env, doe_raster_blur(ctx(env), s, d, num(env, a.v[2], NAN), taps));
+  return nullptr;
+}
+
+// fieldImage(field, index, {quantity: 're'|'im'|'intensity'|'amplitude'|'phase',
+//   gray, stops: Float64Array [pos, r, g, b]*, min, max, gamma, log, logFloor,
+//   percentile, invert, pngRows}) -> {bytes: Uint8Array, lo, hi}
+napi_value FieldImage(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* f = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  if (!f || a.n < 3) {
+    napi_throw_type_error(env, nullptr, "fieldImage(field, index, opts)");
+    return nullptr;
+  }
+  napi_value o = a.v[2];
+  doe_image_opts io;
+  doe_image_opts_init(&io);
+  static const char* qs[] = {"re", "im", "intensity", "amplitude", "phase"};
+  napi_value qv = prop(env, o, "quantity");
+  if (present(env, qv)) {
+    const std::string q = str(env, qv);
+    int found = -1;
+    for (int t = 0; t < 5; ++t)
+      if (q == qs[t]) found = t;
+    if (found < 0) {
+      napi_throw_error(env, nullptr, ("fieldImage: unknown quantity \"" + q + "\"").c_str());
+      return nullptr;
+    }
+    io.quantity = found;
+  }
+  io.gray = boolean(env, prop(env, o, "gray"), false) ? 1 : 0;
+  size_t ns = 0;
+  const double* st = f64(env, prop(env, o, "stops"), &ns);
+  if (st) {
+    if (ns % 4 != 0 || ns / 4 < 1 || ns / 4 > 8) {
+      napi_throw_range_error(env, nullptr, "fieldImage: stops hold 1 to 8 (pos, r, g, b) entries");
+      return nullptr;
+    }
+    io.nstops = int(ns / 4);
+    for (int q = 0; q < io.nstops; ++q) {
+      io.stop_pos[q] = st[4 * q];
+      for (int c = 0; c < 3; ++c) io.stop_rgb[q][c] = st[4 * q + 1 + c];
+    }
+  }
+  napi_value v = prop(env, o, "min");
+  if (present(env, v)) {
+    io.has_min = 1;
+    io.min = num(env, v, NAN);
+  }
+  v = prop(env, o, "max");
+  if (present(env, v)) {
+    io.has_max = 1;
+    io.max = num(env, v, NAN);
+  }
+  io.gamma = opt_num(env, o, "gamma", 1);
+  io.log = boolean(env, prop(env, o, "log"), false) ? 1 : 0;
+  io.log_floor = opt_num(env, o, "logFloor", 1e-4);
+  v = prop(env, o, "percentile");
+  if (present(env, v)) {
+    io.has_percentile = 1;
+    io.percentile = num(env, v, NAN);
+  }
+  io.invert = boolean(env, prop(env, o, "invert"), false) ? 1 : 0;
+  io.png_rows = boolean(env, prop(env, o, "pngRows"), false) ? 1 : 0;
+  int nx = 0, ny = 0;
+  doe_field_info(f, &nx, &ny, nullptr, nullptr, nullptr, nullptr);
+  const size_t need = size_t(ny) * ((io.png_rows ? 1 : 0) + size_t(nx) * (io.gray ? 1 : 3));
+  napi_value ab, arr;
+  void* p = nullptr;
+  CHECK_NAPI(napi_create_arraybuffer(env, need, &p, &ab));
+  CHECK_NAPI(napi_create_typedarray(env, napi_uint8_array, need, ab, 0, &arr));
+  double range[2] = {0, 0};
+  if (throw_status(env, doe_field_image(ctx(env), f, (int)num(env, a.v[1], 0), &io, (unsigned char*)p,
+                                        (long)need, range)))
+    return nullptr;
+  napi_value obj;
+  CHECK_NAPI(napi_create_object(env, &obj));
+  set(env, obj, "bytes", arr);
+  set(env, obj, "lo", make_num(env, range[0]));
+  set(env, obj, "hi", make_num(env, range[1]));
+  return obj;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -608,6 +832,10 @@ napi_value Init(napi_env env, napi_value exports) {
       {"fieldResample", nullptr, FieldResample, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fieldCopySlices", nullptr, FieldCopySlices, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gsMultiPlane", nullptr, GsMultiPlane, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"targetMake", nullptr, TargetMake, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"targetAutoRoi", nullptr, TargetAutoRoi, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"rasterBlur", nullptr, RasterBlur, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldImage", nullptr, FieldImage, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

@@ -69,6 +69,30 @@ class MpPlane(C.Structure):
                 ("weight", C.c_double)]
 
 
+TGT_KINDS = {"disc": 0, "ring": 1, "dots": 2, "grid": 2, "checker": 3, "lines": 4, "zone": 5,
+             "raster": 6}
+QUANTITIES = {"re": 0, "im": 1, "intensity": 2, "amplitude": 3, "phase": 4}
+
+
+class TargetSpec(C.Structure):
+    _fields_ = [("kind", C.c_int), ("radius", C.c_double), ("inner", C.c_double),
+                ("outer", C.c_double), ("cols", C.c_double), ("rows", C.c_double),
+                ("sigma", C.c_double), ("missing", C.c_void_p), ("n_missing", C.c_int),
+                ("cells", C.c_double), ("period", C.c_double), ("duty", C.c_double),
+                ("raster", C.c_void_p), ("raster_w", C.c_int), ("raster_h", C.c_int),
+                ("cover", C.c_int), ("gamma", C.c_double), ("background", C.c_double),
+                ("gain", C.c_double)]
+
+
+class ImageOpts(C.Structure):
+    _fields_ = [("quantity", C.c_int), ("gray", C.c_int), ("nstops", C.c_int),
+                ("stop_pos", C.c_double * 8), ("stop_rgb", (C.c_double * 3) * 8),
+                ("has_min", C.c_int), ("has_max", C.c_int), ("min", C.c_double),
+                ("max", C.c_double), ("gamma", C.c_double), ("log", C.c_int),
+                ("log_floor", C.c_double), ("has_percentile", C.c_int), ("percentile", C.c_double),
+                ("invert", C.c_int), ("png_rows", C.c_int)]
+
+
 # Every symbol the header declares: (name, restype, argtypes)
 _VP = C.c_void_p
 SIGNATURES = [
@@ -144,6 +168,13 @@ SIGNATURES = [
                                            C.POINTER(Metrics)]),
     ("doe_field_resample_centred", C.c_int, [_VP, _VP, _VP]),
     ("doe_field_copy_slices", C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, C.c_int]),
+    ("doe_target_spec_init", C.c_int, [C.POINTER(TargetSpec), C.c_int]),
+    ("doe_target_make", C.c_int, [_VP, C.POINTER(TargetSpec), _VP]),
+    ("doe_target_auto_roi", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double]),
+    ("doe_raster_blur", C.c_int, [_VP, _VP, _VP, C.c_double, c_double_p]),
+    ("doe_image_opts_init", C.c_int, [C.POINTER(ImageOpts)]),
+    ("doe_field_image", C.c_int, [_VP, _VP, C.c_int, C.POINTER(ImageOpts), _VP, C.c_long,
+                                  c_double_p]),
     ("doe_gs_multiplane_run", C.c_int, [_VP, C.POINTER(GsPlan), c_double_p, C.c_int,
                                         C.POINTER(MpPlane), c_double_p, c_double_p, c_double_p,
                                         c_double_p, c_double_p]),
@@ -391,6 +422,71 @@ class Context:
                                              _dptr(out["error"]), _dptr(out["efficiency"]),
                                              _dptr(out["per_plane"])))
         return out
+
+    # ---- targets, ROIs, figure images ----------------------------------
+    def target_spec(self, kind: str, **kw) -> TargetSpec:
+        """A spec with the reference's defaults for `kind`, overridden by kw
+        (raster: 2-D array; missing: list of (col, row))."""
+        spec = TargetSpec()
+        check(self.lib.doe_target_spec_init(C.byref(spec), TGT_KINDS[kind]))
+        keep = []
+        for k, v in kw.items():
+            if k == "raster":
+                a = _darr(v)
+                keep.append(a)
+                spec.raster, spec.raster_h, spec.raster_w = a.ctypes.data, a.shape[0], a.shape[1]
+            elif k == "missing":
+                m = np.ascontiguousarray(np.asarray(v, dtype=np.int32).reshape(-1, 2))
+                keep.append(m)
+                spec.missing, spec.n_missing = (m.ctypes.data if len(m) else None), len(m)
+            else:
+                setattr(spec, k, v)
+        spec._keep = keep
+        return spec
+
+    def target_make(self, specs, nx: int, ny: int, out: "Field" = None) -> "Field":
+        """makeTarget on the device: one FP64 field per spec (re = irradiance)."""
+        specs = list(specs)
+        arr = (TargetSpec * len(specs))(*specs)
+        if out is None:
+            out = self.field(nx, ny, len(specs), "f64")
+        check(self.lib.doe_target_make(self.h, arr, out.h))
+        return out
+
+    def target_auto_roi(self, t: "Field", dilate=2.0, soft=2.0, threshold=0.05) -> None:
+        check(self.lib.doe_target_auto_roi(self.h, t.h, dilate, soft, threshold))
+
+    def raster_blur(self, src: "Field", dst: "Field", sigma: float, taps=None) -> None:
+        tp = None if taps is None else _darr(taps)
+        check(self.lib.doe_raster_blur(self.h, src.h, dst.h, sigma, None if tp is None else _dptr(tp)))
+
+    def field_image(self, f: "Field", index: int = 0, quantity: str = "re", gray: bool = False,
+                    stops=None, vmin=None, vmax=None, gamma=1.0, log=False, log_floor=1e-4,
+                    percentile=None, invert=False, png_rows=False):
+        """planeToRGB / planeToGray of one field: (bytes as uint8 array, (lo, hi));
+        stops: list of (position, (r, g, b))"""
+        o = ImageOpts()
+        check(self.lib.doe_image_opts_init(C.byref(o)))
+        o.quantity, o.gray = QUANTITIES[quantity], int(gray)
+        if stops is not None:
+            o.nstops = len(stops)
+            for q, (pos, rgb) in enumerate(stops):
+                o.stop_pos[q] = pos
+                for c in range(3):
+                    o.stop_rgb[q][c] = rgb[c]
+        if vmin is not None:
+            o.has_min, o.min = 1, vmin
+        if vmax is not None:
+            o.has_max, o.max = 1, vmax
+        o.gamma, o.log, o.log_floor, o.invert, o.png_rows = gamma, int(log), log_floor, int(invert), int(png_rows)
+        if percentile is not None:
+            o.has_percentile, o.percentile = 1, percentile
+        ch = 1 if gray else 3
+        out = np.zeros(f.ny * (int(png_rows) + f.nx * ch), dtype=np.uint8)
+        rng = np.zeros(2)
+        check(self.lib.doe_field_image(self.h, f.h, index, C.byref(o), out.ctypes.data, out.size,
+                                       _dptr(rng)))
+        return out, (rng[0], rng[1])
 
     def gs_run(self, plan: GsPlan, illumination, target_amp, roi=None, init_phase=None,
                seed: int = 12345):

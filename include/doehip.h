@@ -14,7 +14,8 @@
  * Data layout: a field is a batch of `batch` complex nx-by-ny grids stored as
  * two planar arrays (real, imaginary), row-major with idx = x + y*nx, fields
  * back to back — the reference's ComplexField layout (src/field.js:22-40).
- * Grid sides are powers of two in [2, 8192].
+ * Transform grids have power-of-two sides in [2, 8192]; storage, targets,
+ * metrics and images take any side in [1, 65536].
  *
  * Citations "ref:" point into the reference toolkit's sources.
  */
@@ -78,7 +79,10 @@ int doe_ctx_get_chunk_bytes(doe_ctx* ctx, long* bytes);
 int doe_timer_start(doe_ctx* ctx);
 int doe_timer_stop(doe_ctx* ctx, double* ms);
 
-/* ---- device-resident planar fields (ref: src/field.js:22-196) --------- */
+/* ---- device-resident planar fields (ref: src/field.js:22-196) --------- *
+ * Any grid of sides 1..65536 can be stored, uploaded, downloaded, copied,
+ * resampled, measured and imaged; transforms, propagators and phase
+ * retrieval need power-of-two sides in [2, 8192] (DOE_ERR_UNSUPPORTED). */
 int doe_field_create(doe_ctx* ctx, int nx, int ny, int batch, int dtype, doe_field** out);
 /* Non-owning view of caller device memory (re, im: batch*nx*ny elements). */
 int doe_field_wrap(doe_ctx* ctx, void* re, void* im, int nx, int ny, int batch, int dtype,
@@ -329,6 +333,89 @@ int doe_field_resample_centred(doe_ctx* ctx, const doe_field* src, doe_field* ds
  * (same grid and dtype): splits a batched result into per-plane fields. */
 int doe_field_copy_slices(doe_ctx* ctx, const doe_field* src, int src_first, doe_field* dst,
                           int dst_first, int count);
+
+/* ---- targets, regions of interest, figure images ------------------------
+ * makeTarget (src/projector.js:169-255) on the device.  One spec per field
+ * of `out` (FP64; the irradiance is written to re, im is left alone).
+ * doe_target_spec_init fills the reference's defaults for a kind.  'text'
+ * and 'image' targets are DOE_TGT_RASTER: a host raster (renderText output,
+ * or an image) resampled bilinearly, 'contain' unless cover is set
+ * (resampleRaster, projector.js:126-154), times gain, background outside
+ * the raster; gamma != 1 gives max(0, v)^gamma ('image').  The same spec
+ * with any out grid is resampleRaster.  dots/grid: missing holds n_missing
+ * (col, row) cell pairs left dark.  Positive cols and rows are required. */
+#define DOE_TGT_DISC 0
+#define DOE_TGT_RING 1
+#define DOE_TGT_DOTS 2 /* 'dots' and 'grid' */
+#define DOE_TGT_CHECKER 3
+#define DOE_TGT_LINES 4
+#define DOE_TGT_ZONE 5
+#define DOE_TGT_RASTER 6 /* 'text', 'image', resampleRaster */
+
+typedef struct doe_target_spec {
+  int kind;
+  double radius;            /* disc: fraction of min(nx, ny) / 2 (0.35) */
+  double inner, outer;      /* ring (0.25, 0.4) */
+  double cols, rows, sigma; /* dots/grid (8, 8, 0.22 of the smaller pitch) */
+  const int* missing;       /* dots/grid: dark (col, row) pairs */
+  int n_missing;
+  double cells;             /* checker (8) */
+  double period, duty;      /* lines (12, 0.5); zone (period 16) */
+  const double* raster;     /* raster_w * raster_h samples, row-major */
+  int raster_w, raster_h;
+  int cover;                /* 0: fit 'contain', 1: 'cover' */
+  double gamma, background, gain; /* 1, 0, 1 */
+} doe_target_spec;
+
+int doe_target_spec_init(doe_target_spec* spec, int kind);
+int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out);
+/* autoRoi (projector.js:262-315) of every field of t (FP64): reads the
+ * irradiance in re, writes the ROI to im.  Threshold at threshold * max
+ * (reference 0.05), square dilation of half-width max(1, round(dilate)),
+ * then a (2s+1)^2 box blur with s = max(1, round(soft)) unless soft <= 0.
+ * Exact: equals the reference's values bit for bit for the same irradiance. */
+int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold);
+/* blurRaster (src/doe.js:761-794) of the re plane of every field, src to dst
+ * (FP64, may be the same field): separable Gaussian, half-width
+ * r = max(1, ceil(3 sigma)), clamped edges; sigma <= 0 copies.  taps: the
+ * 2r+1 normalised weights, or NULL to compute them (exp in FP64). */
+int doe_raster_blur(doe_ctx* ctx, const doe_field* src, doe_field* dst, double sigma, const double* taps);
+
+/* planeToRGB / planeToGray (src/color.js:7-160) of one field's scalar
+ * quantity: 8-bit RGB (a piecewise-linear colour map of 1..8 stops) or gray
+ * pixels, row-major, or PNG scanlines (png_rows: each row led by filter
+ * byte 0, src/png.js:96-101) ready for deflate.  The range is min / max
+ * when given, else the data's (NaN skipped); with a percentile (colour
+ * maps) the upper end is the sorted plane's element at
+ * min(n - 1, floor(n * percentile)) and the lower its first element.
+ * range_out (nullable) receives the lo, hi used. */
+#define DOE_Q_RE 0
+#define DOE_Q_IM 1
+#define DOE_Q_INTENSITY 2 /* re^2 + im^2 */
+#define DOE_Q_AMPLITUDE 3 /* hypot(re, im) */
+#define DOE_Q_PHASE 4     /* atan2(im, re) */
+
+typedef struct doe_image_opts {
+  int quantity; /* DOE_Q_* */
+  int gray;     /* 1: planeToGray; 0: colour map */
+  int nstops;
+  double stop_pos[8];
+  double stop_rgb[8][3];
+  int has_min, has_max;
+  double min, max;
+  double gamma;     /* colour map display gamma (1: none) */
+  int log;          /* colour map log scaling */
+  double log_floor; /* 1e-4 */
+  int has_percentile;
+  double percentile;
+  int invert;   /* gray */
+  int png_rows; /* 1: PNG scanlines */
+} doe_image_opts;
+
+int doe_image_opts_init(doe_image_opts* opts);
+/* out_len >= ny * (png_rows + nx * (gray ? 1 : 3)). */
+int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image_opts* opts,
+                    unsigned char* out, long out_len, double* range_out);
 
 #ifdef __cplusplus
 }

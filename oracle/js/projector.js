@@ -170,42 +170,73 @@ export function autoRoi(irr, nx, ny, opts) {
   return out;
 }
 
-// Targets used by the configurations on the hot path: text, disc, dots/grid
-// (ref: projector.js:169-255).
+// Every target kind (ref: projector.js:169-255).
 export function makeTarget(kind, nx, ny, opts) {
   const o = opts || {};
   const irr = new Float64Array(nx * ny);
   const cx = (nx - 1) / 2;
   const cy = (ny - 1) / 2;
+  const dflt = (v, d) => (v === undefined || v === null ? d : v);
+  const fill = (fn) => {
+    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) irr[i + j * nx] = fn(i, j);
+  };
   let name = kind;
-  if (kind === 'text') {
-    const text = o.text === undefined ? 'DOE' : o.text;
-    const g = renderText(text, { scale: o.scale === undefined ? 6 : o.scale, spacing: o.spacing === undefined ? 1 : o.spacing });
+  if (kind === 'image') {
+    if (!o.raster) throw new Error('makeTarget("image"): pass {raster, width, height}');
+    const gamma = dflt(o.gamma, 1);
+    const res = resampleRaster(o.raster, o.width, o.height, nx, ny, { fit: 'contain' });
+    for (let k = 0; k < irr.length; k++) irr[k] = gamma === 1 ? res[k] : Math.pow(Math.max(0, res[k]), gamma);
+    name = dflt(o.name, 'image');
+  } else if (kind === 'text') {
+    const text = dflt(o.text, 'DOE');
+    const g = renderText(text, { scale: dflt(o.scale, 6), spacing: dflt(o.spacing, 1) });
     irr.set(resampleRaster(g.data, g.width, g.height, nx, ny, { fit: 'contain' }));
     name = 'text:' + text;
   } else if (kind === 'disc') {
-    const r = (o.radius === undefined ? 0.35 : o.radius) * Math.min(nx, ny) * 0.5;
-    for (let j = 0; j < ny; j++) for (let i = 0; i < nx; i++) irr[i + j * nx] = Math.hypot(i - cx, j - cy) <= r ? 1 : 0;
+    const r = dflt(o.radius, 0.35) * Math.min(nx, ny) * 0.5;
+    fill((i, j) => (Math.hypot(i - cx, j - cy) <= r ? 1 : 0));
+  } else if (kind === 'ring') {
+    const r0 = dflt(o.inner, 0.25) * Math.min(nx, ny) * 0.5;
+    const r1 = dflt(o.outer, 0.4) * Math.min(nx, ny) * 0.5;
+    fill((i, j) => {
+      const r = Math.hypot(i - cx, j - cy);
+      return r >= r0 && r <= r1 ? 1 : 0;
+    });
   } else if (kind === 'dots' || kind === 'grid') {
-    const cols = o.cols === undefined ? 8 : o.cols;
-    const rows = o.rows === undefined ? 8 : o.rows;
+    const cols = dflt(o.cols, 8);
+    const rows = dflt(o.rows, 8);
     const px = nx / cols;
     const py = ny / rows;
-    const sigma = (o.sigma === undefined ? 0.22 : o.sigma) * Math.min(px, py);
-    for (let j = 0; j < ny; j++) {
-      for (let i = 0; i < nx; i++) {
-        const fy = ((j + 0.5) / py) % 1 - 0.5;
-        const fx = ((i + 0.5) / px) % 1 - 0.5;
-        const ddx = fx * px;
-        const ddy = fy * py;
-        irr[i + j * nx] = Math.exp(-(ddx * ddx + ddy * ddy) / (2 * sigma * sigma));
-      }
-    }
+    const sigma = dflt(o.sigma, 0.22) * Math.min(px, py);
+    const dark = o.missing || [];
+    fill((i, j) => {
+      const fy = ((j + 0.5) / py) % 1 - 0.5;
+      const fx = ((i + 0.5) / px) % 1 - 0.5;
+      const ddx = fx * px;
+      const ddy = fy * py;
+      const ci = (i / px) | 0;
+      const cj = (j / py) | 0;
+      for (const m of dark) if (m[0] === ci && m[1] === cj) return 0;
+      return Math.exp(-(ddx * ddx + ddy * ddy) / (2 * sigma * sigma));
+    });
+  } else if (kind === 'checker') {
+    const cells = dflt(o.cells, 8);
+    fill((i, j) => ((Math.floor((i / nx) * cells) + Math.floor((j / ny) * cells)) % 2 === 0 ? 1 : 0));
+  } else if (kind === 'lines') {
+    const period = dflt(o.period, 12);
+    const duty = dflt(o.duty, 0.5);
+    fill((i) => ((i % period) / period < duty ? 1 : 0));
+  } else if (kind === 'zone') {
+    const period = dflt(o.period, 16);
+    fill((i, j) => {
+      const r = Math.hypot(i - cx, j - cy);
+      return Math.sin((Math.PI * r * r) / period) > 0 ? 1 : 0;
+    });
   } else {
     throw new Error('makeTarget: unknown kind "' + kind + '"');
   }
-  const roi = o.roi ? o.roi : autoRoi(irr, nx, ny, { dilate: o.dilate === undefined ? 2 : o.dilate, soft: o.softEdge === undefined ? 2 : o.softEdge });
-  const grid = kind === 'dots' || kind === 'grid' ? { cols: o.cols === undefined ? 8 : o.cols, rows: o.rows === undefined ? 8 : o.rows } : null;
+  const roi = o.roi ? o.roi : autoRoi(irr, nx, ny, { dilate: dflt(o.dilate, 2), soft: dflt(o.softEdge, 2) });
+  const grid = kind === 'dots' || kind === 'grid' ? { cols: dflt(o.cols, 8), rows: dflt(o.rows, 8) } : null;
   return { irradiance: irr, roi: roi, name: name, grid: grid };
 }
 

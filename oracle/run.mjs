@@ -15,8 +15,9 @@ import {
   focalPlaneSpectrum,
   focusSweep,
 } from './js/propagate.js';
-import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane } from './js/doe.js';
-import { ProjectorSystem } from './js/projector.js';
+import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane, blurRaster } from './js/doe.js';
+import { ProjectorSystem, makeTarget, autoRoi, resampleRaster, renderText } from './js/projector.js';
+import { planeToRGB, planeToGray } from './js/color.js';
 import { ssim, rmsError, correlation, regionStats } from './js/metrics.js';
 
 const dir = process.argv[2];
@@ -172,6 +173,32 @@ if (op === 'fft2d' || op === 'dft2d') {
   res.metrics = sys.screenMetrics(field);
   out('screen_re', field.re);
   out('screen_im', field.im);
+} else if (op === 'target') {
+  // makeTarget(kind, nx, ny, opts); an image raster comes in as in_raster
+  const o = Object.assign({}, a.opts || {});
+  if (a.kind === 'image') o.raster = inp('raster');
+  const t = makeTarget(a.kind, a.nx, a.ny, o);
+  out('irradiance', t.irradiance);
+  out('roi', t.roi);
+  res.name = t.name;
+  res.grid = t.grid;
+} else if (op === 'auto_roi') {
+  out('roi', autoRoi(inp('irr'), a.nx, a.ny, a.opts || {}));
+} else if (op === 'blur') {
+  out('out', blurRaster(inp('src'), a.nx, a.ny, a.sigma));
+} else if (op === 'resample') {
+  out('out', resampleRaster(inp('src'), a.sw, a.sh, a.dw, a.dh, a.opts || {}));
+} else if (op === 'render_text') {
+  const g = renderText(a.text, a.opts || {});
+  out('data', g.data);
+  res.width = g.width;
+  res.height = g.height;
+} else if (op === 'image') {
+  // planeToRGB / planeToGray of in_plane; bytes come back as Float64
+  const r = a.gray ? planeToGray(inp('plane'), a.opts || {}) : planeToRGB(inp('plane'), a.opts || {});
+  out('bytes', a.gray ? r.gray : r.rgb);
+  res.lo = r.lo;
+  res.hi = r.hi;
 } else if (op === 'metrics') {
   const p = inp('plane');
   const t = inp('target');

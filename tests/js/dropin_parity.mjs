@@ -6,7 +6,10 @@ import * as offt from '../../oracle/js/fft.js';
 import * as oprop from '../../oracle/js/propagate.js';
 import { ComplexField as OField } from '../../oracle/js/field.js';
 import { ProjectorSystem as OProjector } from '../../oracle/js/projector.js';
-import { designDOE as oDesign, gerchbergSaxtonMultiPlane as oMultiPlane } from '../../oracle/js/doe.js';
+import { designDOE as oDesign, gerchbergSaxtonMultiPlane as oMultiPlane, blurRaster as oBlur } from '../../oracle/js/doe.js';
+import { makeTarget as oMakeTarget, autoRoi as oAutoRoi, resampleRaster as oResample } from '../../oracle/js/projector.js';
+import { planeToRGB as oRGB, planeToGray as oGray } from '../../oracle/js/color.js';
+import zlib from 'zlib';
 
 const results = [];
 function check(name, ok, info) {
@@ -274,6 +277,108 @@ for (const prec of ['f32', 'f64']) {
     r.error.every((e) => Number.isFinite(e)), r.error);
 }
 
-const failed = results.filter((r) => !r.ok);
-console.log(JSON.stringify({ summary: true, total: results.length, failed: failed.length }));
-process.exit(failed.length ? 1 : 0);
+// targets, ROIs, blur, resample on the device (ref: projector.js:126-315, doe.js:761-794)
+function same(a, b) {
+  if (a.length !== b.length) return false;
+  for (let k = 0; k < a.length; k++) if (a[k] !== b[k] && !(a[k] !== a[k] && b[k] !== b[k])) return false;
+  return true;
+}
+function maxRel(a, b) {
+  let w = 0;
+  for (let k = 0; k < a.length; k++) w = Math.max(w, Math.abs(a[k] - b[k]) / Math.max(1e-300, Math.abs(b[k])));
+  return w;
+}
+{
+  const kinds = [['text', { text: 'DOE' }], ['disc', {}], ['ring', { inner: 0.2 }], ['checker', { cells: 7 }],
+    ['lines', { period: 10 }], ['zone', {}], ['grid', { cols: 5, rows: 4, missing: [[1, 1], [2.5, 0], [4, 3]] }],
+    ['dots', { cols: 3, rows: 3, dilate: 4, softEdge: 0 }]];
+  for (const [nx, ny] of [[128, 128], [90, 70]]) {
+    for (const [kind, opts] of kinds) {
+      const g = gpu.makeTarget(kind, nx, ny, opts);
+      const o = oMakeTarget(kind, nx, ny, opts);
+      const exactIrr = kind !== 'grid' && kind !== 'dots';
+      const irrOk = exactIrr ? same(g.irradiance, o.irradiance) : maxRel(g.irradiance, o.irradiance) < 1e-14;
+      check(`makeTarget ${kind} ${nx}x${ny}`, irrOk && same(g.roi, o.roi) && g.name === o.name &&
+        JSON.stringify(g.grid) === JSON.stringify(o.grid), { name: g.name });
+    }
+  }
+  const img = lcg(33 * 21, 9).map((v) => v + 0.5);
+  const gi = gpu.makeTarget('image', 100, 80, { raster: img, width: 33, height: 21, gamma: 1.8, name: 'pic' });
+  const oi = oMakeTarget('image', 100, 80, { raster: img, width: 33, height: 21, gamma: 1.8, name: 'pic' });
+  check('makeTarget image gamma', maxRel(gi.irradiance, oi.irradiance) < 1e-14 && gi.name === 'pic', maxRel(gi.irradiance, oi.irradiance));
+  const custom = lcg(77 * 61, 21).map((v) => (v + 0.5) ** 5);
+  check('autoRoi', same(gpu.autoRoi(custom, 77, 61, { dilate: 3, soft: 1.6, threshold: 0.1 }),
+    oAutoRoi(custom, 77, 61, { dilate: 3, soft: 1.6, threshold: 0.1 })));
+  const bg = gpu.blurRaster(custom, 77, 61, 1.7);
+  const bo = oBlur(custom, 77, 61, 1.7);
+  check('blurRaster', same(bg, bo), maxRel(bg, bo));
+  check('blurRaster sigma 0 copies', same(gpu.blurRaster(custom, 77, 61, 0), custom));
+  for (const fit of ['contain', 'cover']) {
+    const rg = gpu.resampleRaster(img, 33, 21, 64, 50, { fit, background: 0.25, gain: 2 });
+    const ro = oResample(img, 33, 21, 64, 50, { fit, background: 0.25, gain: 2 });
+    check(`resampleRaster ${fit}`, same(rg, ro));
+  }
+  let threw = false;
+  try {
+    gpu.makeTarget('star', 16, 16);
+  } catch (err) {
+    threw = /unknown kind "star"/.test(err.message);
+  }
+  check('makeTarget unknown kind throws', threw);
+  // setTarget through the projector keeps the reference's result
+  const sys = new gpu.ProjectorSystem({ n: 64, target: { kind: 'dots', cols: 4, rows: 4 } });
+  const osys = new OProjector({ n: 64, target: { kind: 'dots', cols: 4, rows: 4 } });
+  check('ProjectorSystem target', maxRel(sys.target.irradiance, osys.target.irradiance) < 1e-14 &&
+    same(sys.target.roi, osys.target.roi));
+}
+
+// figure pixels and PNG output (ref: color.js:87-160, png.js:89-180)
+async function figureChecks() {
+  const plane = lcg(97 * 53, 31).map((v, k) => (k % 7 === 0 ? v * 10 : v));
+  for (const opts of [{}, { map: 'turbo' }, { map: 'inferno', min: -0.2 }, { map: 'phase', percentile: 0.95 }]) {
+    const g = gpu.planeToRGB(plane, opts);
+    const o = oRGB(plane, opts).rgb;
+    check(`planeToRGB ${JSON.stringify(opts)}`, same(g, o));
+  }
+  for (const opts of [{}, { invert: true }, { min: 0, max: 2 }]) {
+    check(`planeToGray ${JSON.stringify(opts)}`, same(gpu.planeToGray(plane, opts), oGray(plane, opts).gray));
+  }
+  const f = new gpu.ComplexField(64, 32, 8e-6, 8e-6);
+  f.re.set(lcg(2048, 41));
+  f.im.set(lcg(2048, 43));
+  const inten = f.intensity();
+  check('fieldToRGB intensity', same(gpu.fieldToRGB(f, { map: 'viridis' }), oRGB(inten, { map: 'viridis' }).rgb));
+  const bytes = await gpu.writeFieldPNG(null, f, { map: 'viridis' });
+  // decode: chunks, CRCs, inflate, scanlines
+  let o = 8;
+  const u = Buffer.from(bytes);
+  let idat = [];
+  let crcOk = true;
+  while (o < u.length) {
+    const len = u.readUInt32BE(o);
+    const type = u.toString('latin1', o + 4, o + 8);
+    if (type === 'IDAT') idat.push(u.subarray(o + 8, o + 8 + len));
+    const c = zlib.crc32 ? zlib.crc32(u.subarray(o + 4, o + 8 + len)) : null;
+    if (c !== null && c !== u.readUInt32BE(o + 8 + len)) crcOk = false;
+    o += 12 + len;
+  }
+  const raw = zlib.inflateSync(Buffer.concat(idat));
+  const px = oRGB(inten, { map: 'viridis' }).rgb;
+  let rowsOk = raw.length === 32 * (1 + 192);
+  for (let y = 0; y < 32 && rowsOk; y++) {
+    rowsOk = raw[y * 193] === 0 && same(raw.subarray(y * 193 + 1, (y + 1) * 193), px.subarray(y * 192, (y + 1) * 192));
+  }
+  check('writeFieldPNG decodes to the mapped pixels', rowsOk && crcOk && u.readUInt32BE(16) === 64 && u.readUInt32BE(20) === 32);
+  const gpng = await gpu.writePlanePNG(null, Float64Array.from({ length: 16 }, (_, k) => k / 15), 4, 4);
+  const graw = zlib.inflateSync(Buffer.from(gpng.subarray(41, 41 + Buffer.from(gpng).readUInt32BE(33))));
+  check('writePlanePNG brightest pixel', graw.length === 20 && graw[3 * 5 + 1 + 3] > 240, graw[3 * 5 + 1 + 3]);
+}
+
+figureChecks().then(() => {
+  const failed = results.filter((r) => !r.ok);
+  console.log(JSON.stringify({ summary: true, total: results.length, failed: failed.length }));
+  process.exit(failed.length ? 1 : 0);
+}, (err) => {
+  console.log(JSON.stringify({ name: 'figure checks', ok: false, info: String(err && err.stack) }));
+  process.exit(1);
+});

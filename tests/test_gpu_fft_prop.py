@@ -78,8 +78,13 @@ def test_fft2d_host_dropin_and_errors(ctx):
     with pytest.raises(nat.DoeError) as ei:
         ctx.fft2d_host(np.zeros(10), np.zeros(10), 4, 4, False, "f64")
     assert "does not match 4x4" in str(ei.value)
-    with pytest.raises(nat.DoeError):
-        ctx.field(12, 16)  # not a power of two (ref: field.js:30-32)
+    # any grid can be stored; a transform needs power-of-two sides (ref: field.js:30-32)
+    f = ctx.field(12, 16)
+    with pytest.raises(nat.DoeError, match="powers of two"):
+        ctx.fft2d(f)
+    f.free()
+    with pytest.raises(nat.DoeError, match="in \\[1, 65536\\]"):
+        ctx.field(0, 16)
 
 
 def test_fft2d_batch_linearity_and_parseval(ctx):
