@@ -21,6 +21,10 @@ WGS designs of sides {256, 512, 1024, 2048} (spot arrays, filled polygons),
 "storage_variants" times the same fft2d step with the batch stored in FP64,
 FP16 and BF16 (the half-width ones compute in FP32).
 
+"targets" times makeTarget + autoRoi on the device for 64 mixed 1024^2
+targets (text and image rasters, disc, ring, spot grid, zone plate, checker,
+lines) and planeToRGB (viridis, percentile 0.99) of a 2048^2 field.
+
 The second half of the metric, WGS time to screen RMSE <= 1 % at 1024^2
 (learned warm start vs plain random start), is reported under "wgs": a
 network is trained on pairs from the batched WGS engine (bounded: 128
@@ -69,6 +73,10 @@ def cpu_baseline(n: int = 2048, reps: int = 8) -> dict | None:
     _, res = oj.run("time", {"what": "fft2d", "n": n, "reps": reps}, timeout=900)
     ms = res["ms"]
     gbs = 32 * n * n / (ms * 1e-3) / 1e9
+    # the target / figure leg's host counterparts (one node thread)
+    _, rt = oj.run("time", {"what": "target", "kind": "text", "n": 1024, "reps": 2}, timeout=600)
+    _, ri = oj.run("time", {"what": "image", "n": 1024, "reps": 2,
+                            "opts": {"map": "viridis", "percentile": 0.99}}, timeout=600)
     return {
         "value": round(gbs, 4),
         "unit": "GB/s",
@@ -78,6 +86,11 @@ def cpu_baseline(n: int = 2048, reps: int = 8) -> dict | None:
                    f"strided columns) on one node {res.get('node')} thread, 1 warm-up; "
                    f"{ms:.1f} ms per transform; GB/s counts the same 32 B/pt as the GPU value"),
         "host": platform.processor() or platform.machine(),
+        "targets": {"make_target_text_1024sq_ms": round(rt["ms"], 2),
+                    "plane_to_rgb_p99_1024sq_ms": round(ri["ms"], 2),
+                    "sample": "makeTarget('text') with its autoRoi, and planeToRGB (viridis, "
+                              "percentile 0.99), each on a 1024^2 grid by the JS oracle on one "
+                              "node thread, 2 reps after 1 warm-up"},
     }
 
 
@@ -231,6 +244,64 @@ def wgs_leg(ctx, train: int, steps: int, count: int, iters: int) -> dict:
     }
 
 
+TGT_KINDS = [("raster", None), ("disc", {}), ("ring", {"inner": 0.15, "outer": 0.35}),
+             ("dots", {"cols": 12, "rows": 9, "missing": [[2, 3], [7, 1]]}), ("zone", {"period": 40}),
+             ("checker", {"cells": 10}), ("lines", {"period": 24}), ("raster", "image")]
+
+
+def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int = 2048) -> dict:
+    """Device makeTarget + autoRoi over a mixed batch, and planeToRGB of a field.
+
+    Algorithmic bytes: a target's irradiance written (8 B/pt), read back for the
+    ROI and the ROI written (16 B/pt): 24 B/pt.  An image reads the FP32 field
+    (8 B/pt) per scan (min/max, 8 radix-select passes, the mapping) and writes
+    3 B/pt; its time includes the copy of the pixels to the host."""
+    rng = np.random.default_rng(3)
+    text = np.kron((rng.random((7, 17)) > 0.5).astype(np.float64), np.ones((6, 6)))
+    photo = rng.random((300, 400)) ** 2
+    specs = []
+    for q in range(count):
+        kind, opts = TGT_KINDS[q % len(TGT_KINDS)]
+        if kind == "raster":
+            specs.append(ctx.target_spec("raster", raster=photo if opts == "image" else text,
+                                         **({"gamma": 2.2} if opts == "image" else {})))
+        else:
+            specs.append(ctx.target_spec(kind, **opts))
+    out = ctx.field(n, n, count, "f64")
+    ctx.set_flags(asynchronous=True, profile=False)
+    ctx.target_make(specs, n, n, out)
+    ctx.target_auto_roi(out)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        ctx.target_make(specs, n, n, out)
+        ctx.target_auto_roi(out)
+    ms = ctx.timer_stop() / reps
+    out.free()
+    pts = count * n * n
+    # planeToRGB (viridis, percentile 0.99) of an FP32 field's intensity, PNG rows
+    f = ctx.field(img_n, img_n, 1, "f32")
+    re = rng.standard_normal((img_n, img_n))
+    f.upload(re, re[::-1].copy())
+    viridis = [(0, (68, 1, 84)), (0.25, (59, 82, 139)), (0.5, (33, 145, 140)), (0.75, (94, 201, 98)),
+               (1, (253, 231, 37))]
+    ctx.set_flags(asynchronous=False, profile=False)
+    ctx.field_image(f, quantity="intensity", stops=viridis, percentile=0.99, png_rows=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.field_image(f, quantity="intensity", stops=viridis, percentile=0.99, png_rows=True)
+    img_ms = (time.perf_counter() - t0) * 1e3 / reps
+    f.free()
+    ctx.set_flags(asynchronous=True, profile=False)
+    return {
+        "targets": count, "n": n, "kinds": sorted({k if o != "image" else "image" for k, o in TGT_KINDS}),
+        "ms": round(ms, 4), "us_per_target": round(ms * 1e3 / count, 2),
+        "gbs_alg": round(24 * pts / (ms * 1e-3) / 1e9, 1), "bytes_per_pt": 24,
+        "image": {"n": img_n, "quantity": "intensity", "map": "viridis", "percentile": 0.99,
+                  "ms_wall_incl_d2h": round(img_ms, 3)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +314,7 @@ def main() -> int:
                     help="skip the fft2d autotuner (built-in configuration)")
     ap.add_argument("--no-wgs", action="store_true", help="skip the WGS warm-start leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven GS leg")
+    ap.add_argument("--no-targets", action="store_true", help="skip the target / figure leg")
     ap.add_argument("--wgs-train", type=int, default=128)
     ap.add_argument("--wgs-steps", type=int, default=600)
     ap.add_argument("--wgs-eval", type=int, default=32)
@@ -338,6 +410,7 @@ def main() -> int:
     variants = storage_variants(ctx, field, args.steps)
     field.free()
     c3 = None if args.no_c3 else c3_leg(ctx)
+    targets = None if args.no_targets else targets_leg(ctx)
     wgs = None if args.no_wgs else wgs_leg(ctx, args.wgs_train, args.wgs_steps, args.wgs_eval,
                                            args.wgs_iters)
 
@@ -376,6 +449,7 @@ def main() -> int:
         "pipeline": pipeline,
         "storage_variants": variants,
         "gs_batch_uneven": c3,
+        "targets": targets,
         "wgs": wgs,
         "cpu_baseline": cpu,
     }

@@ -108,6 +108,15 @@ __device__ __forceinline__ double ld<double>(const double* p, long k) {
   return p[k];
 }
 
+// Column and row of a flat index k < nx * ny <= 2^32 (32-bit division: the
+// 64-bit one is a long software sequence).
+__device__ __forceinline__ void split_ij(long k, int nx, int& i, int& j) {
+  const unsigned kk = (unsigned)k, w = (unsigned)nx;
+  const unsigned q = kk / w;
+  j = (int)q;
+  i = (int)(kk - q * w);
+}
+
 // ---- makeTarget ------------------------------------------------------------
 
 struct TgtRec {
@@ -122,90 +131,129 @@ struct TgtRec {
   double gamma, background, gain;
 };
 
+// Irradiance of one sample of a target of kind K (projector.js:169-255).
+template <int K>
+__device__ __forceinline__ double target_at(const TgtRec& R, const double* __restrict__ rasters,
+                                            const unsigned char* __restrict__ dark, int i, int j,
+                                            int nx, int ny, double cx, double cy) {
+  if constexpr (K == DOE_TGT_DISC) {
+    return js_hypot(i - cx, j - cy) <= R.p0 ? 1 : 0;
+  } else if constexpr (K == DOE_TGT_RING) {
+    const double r = js_hypot(i - cx, j - cy);
+    return (r >= R.p0 && r <= R.p1) ? 1 : 0;
+  } else if constexpr (K == DOE_TGT_DOTS) {
+    const double px = R.p0, py = R.p1, sg = R.p2;
+    // x % 1 of a positive finite x is x - floor(x), exactly
+    const double qy = (j + 0.5) / py, qx = (i + 0.5) / px;
+    const double fy = (qy - floor(qy)) - 0.5;
+    const double fx = (qx - floor(qx)) - 0.5;
+    const double ddx = fx * px, ddy = fy * py;
+    double v = exp(-(ddx * ddx + ddy * ddy) / (2 * sg * sg));
+    if (R.mcols > 0) {
+      const long ci = (long)(i / px), cj = (long)(j / py);
+      if (ci < R.mcols && cj < R.mrows && dark[R.moff + cj * R.mcols + ci]) v = 0;
+    }
+    return v;
+  } else if constexpr (K == DOE_TGT_CHECKER) {
+    const double u = floor(((double)i / nx) * R.p0);
+    const double w = floor(((double)j / ny) * R.p0);
+    const double h = (u + w) * 0.5;  // (u + w) % 2 === 0: an even integer
+    return (isfinite(h) && h == floor(h)) ? 1 : 0;
+  } else if constexpr (K == DOE_TGT_LINES) {
+    return (fmod((double)i, R.p0) / R.p0) < R.p1 ? 1 : 0;
+  } else if constexpr (K == DOE_TGT_ZONE) {
+    const double r = js_hypot(i - cx, j - cy);
+    return sin((M_PI * r * r) / R.p0) > 0 ? 1 : 0;
+  } else {
+    double v = R.background;
+    const double fy = (j - R.oy) / R.s;
+    const double y0 = floor(fy);
+    if (y0 >= 0 && y0 < R.rh) {
+      const double fx = (i - R.ox) / R.s;
+      const double x0 = floor(fx);
+      if (x0 >= 0 && x0 < R.rw) {
+        const double ty = fy - y0, tx = fx - x0;
+        const long y1 = (long)fmin(y0 + 1, (double)(R.rh - 1));
+        const long x1 = (long)fmin(x0 + 1, (double)(R.rw - 1));
+        const double* src = rasters + R.roff;
+        const long r0 = (long)y0 * R.rw, r1 = y1 * R.rw;
+        const double a = src[(long)x0 + r0] * (1 - tx) * (1 - ty);
+        const double c = src[x1 + r0] * tx * (1 - ty);
+        const double d = src[(long)x0 + r1] * (1 - tx) * ty;
+        const double e = src[x1 + r1] * tx * ty;
+        v = (((a + c) + d) + e) * R.gain;
+      }
+    }
+    if (R.gamma_on) v = pow(js_max(0, v), R.gamma);
+    return v;
+  }
+}
+
+// Every sample of field b, walked by the block's threads; four samples per
+// thread per step, so their independent FP64 chains interleave.
+template <int K>
+__device__ __forceinline__ void target_field(const TgtRec& R, const double* __restrict__ rasters,
+                                             const unsigned char* __restrict__ dark, double* __restrict__ o,
+                                             int nx, int ny) {
+  const long n = (long)nx * ny;
+  const double cx = (nx - 1) / 2.0, cy = (ny - 1) / 2.0;
+  const long step = (long)gridDim.x * kTB;
+  long k = (long)blockIdx.x * kTB + threadIdx.x;
+  for (; k + 3 * step < n; k += 4 * step) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      int i, j;
+      split_ij(k + u * step, nx, i, j);
+      v[u] = target_at<K>(R, rasters, dark, i, j, nx, ny, cx, cy);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o[k + u * step] = v[u];
+  }
+  for (; k < n; k += step) {
+    int i, j;
+    split_ij(k, nx, i, j);
+    o[k] = target_at<K>(R, rasters, dark, i, j, nx, ny, cx, cy);
+  }
+}
+
 __global__ __launch_bounds__(kTB) void k_target(const TgtRec* __restrict__ recs,
                                                 const double* __restrict__ rasters,
                                                 const unsigned char* __restrict__ dark,
                                                 double* __restrict__ out, int nx, int ny) {
   const int b = blockIdx.y;
-  const long n = (long)nx * ny;
-  const long k = (long)blockIdx.x * kTB + threadIdx.x;
-  if (k >= n) return;
-  const TgtRec& R = recs[b];
-  const int i = int(k % nx), j = int(k / nx);
-  const double cx = (nx - 1) / 2.0, cy = (ny - 1) / 2.0;
-  double v = 0;
+  const TgtRec R = recs[b];
+  double* o = out + (long)b * nx * ny;
   switch (R.kind) {
-    case DOE_TGT_DISC:
-      v = js_hypot(i - cx, j - cy) <= R.p0 ? 1 : 0;
-      break;
-    case DOE_TGT_RING: {
-      const double r = js_hypot(i - cx, j - cy);
-      v = (r >= R.p0 && r <= R.p1) ? 1 : 0;
-      break;
-    }
-    case DOE_TGT_DOTS: {
-      const double px = R.p0, py = R.p1, sg = R.p2;
-      const double fy = fmod((j + 0.5) / py, 1.0) - 0.5;
-      const double fx = fmod((i + 0.5) / px, 1.0) - 0.5;
-      const double ddx = fx * px, ddy = fy * py;
-      v = exp(-(ddx * ddx + ddy * ddy) / (2 * sg * sg));
-      if (R.mcols > 0) {
-        const long ci = (long)(i / px), cj = (long)(j / py);
-        if (ci < R.mcols && cj < R.mrows && dark[R.moff + cj * R.mcols + ci]) v = 0;
-      }
-      break;
-    }
-    case DOE_TGT_CHECKER: {
-      const double u = floor(((double)i / nx) * R.p0);
-      const double w = floor(((double)j / ny) * R.p0);
-      v = fmod(u + w, 2.0) == 0 ? 1 : 0;
-      break;
-    }
-    case DOE_TGT_LINES:
-      v = (fmod((double)i, R.p0) / R.p0) < R.p1 ? 1 : 0;
-      break;
-    case DOE_TGT_ZONE: {
-      const double r = js_hypot(i - cx, j - cy);
-      v = sin((M_PI * r * r) / R.p0) > 0 ? 1 : 0;
-      break;
-    }
-    case DOE_TGT_RASTER: {
-      v = R.background;
-      const double fy = (j - R.oy) / R.s;
-      const double y0 = floor(fy);
-      if (y0 >= 0 && y0 < R.rh) {
-        const double fx = (i - R.ox) / R.s;
-        const double x0 = floor(fx);
-        if (x0 >= 0 && x0 < R.rw) {
-          const double ty = fy - y0, tx = fx - x0;
-          const long y1 = (long)fmin(y0 + 1, (double)(R.rh - 1));
-          const long x1 = (long)fmin(x0 + 1, (double)(R.rw - 1));
-          const double* src = rasters + R.roff;
-          const long r0 = (long)y0 * R.rw, r1 = y1 * R.rw;
-          const double a = src[(long)x0 + r0] * (1 - tx) * (1 - ty);
-          const double c = src[x1 + r0] * tx * (1 - ty);
-          const double d = src[(long)x0 + r1] * (1 - tx) * ty;
-          const double e = src[x1 + r1] * tx * ty;
-          v = (((a + c) + d) + e) * R.gain;
-        }
-      }
-      if (R.gamma_on) v = pow(js_max(0, v), R.gamma);
-      break;
-    }
+    case DOE_TGT_DISC: target_field<DOE_TGT_DISC>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_RING: target_field<DOE_TGT_RING>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_DOTS: target_field<DOE_TGT_DOTS>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_CHECKER: target_field<DOE_TGT_CHECKER>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_LINES: target_field<DOE_TGT_LINES>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_ZONE: target_field<DOE_TGT_ZONE>(R, rasters, dark, o, nx, ny); break;
+    default: target_field<DOE_TGT_RASTER>(R, rasters, dark, o, nx, ny); break;
   }
-  out[(long)b * n + k] = v;
 }
 
 // ---- autoRoi ---------------------------------------------------------------
 
-// Per-field maximum over irr > m (NaN never selected, start at +0): the
-// non-negative doubles order like their bit patterns.
+// Per-block maxima of irr over irr > m (NaN never selected, start at +0),
+// part[b * gridDim.x + block]; four independent loads in flight per thread.
 __global__ __launch_bounds__(kTB) void k_roi_max(const double* __restrict__ irr, long n,
-                                                 unsigned long long* __restrict__ mx) {
+                                                 double* __restrict__ part) {
   const int b = blockIdx.y;
   const double* p = irr + (long)b * n;
   double m = 0;
-  for (long k = (long)blockIdx.x * kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
+  const long step = (long)gridDim.x * kTB;
+  long k = (long)blockIdx.x * kTB + threadIdx.x;
+  for (; k + 3 * step < n; k += 4 * step) {
+    const double v0 = p[k], v1 = p[k + step], v2 = p[k + 2 * step], v3 = p[k + 3 * step];
+    if (v0 > m) m = v0;
+    if (v1 > m) m = v1;
+    if (v2 > m) m = v2;
+    if (v3 > m) m = v3;
+  }
+  for (; k < n; k += step) {
     const double v = p[k];
     if (v > m) m = v;
   }
@@ -215,8 +263,18 @@ __global__ __launch_bounds__(kTB) void k_roi_max(const double* __restrict__ irr,
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < kTB / 64; ++w) m = fmax(m, red[w]);
-    atomicMax(mx + b, (unsigned long long)__double_as_longlong(m));
+    part[(long)b * gridDim.x + blockIdx.x] = m;
   }
+}
+
+// thr[b] = threshold * (maximum of field b's partials).
+__global__ void k_roi_thr(const double* __restrict__ part, int nblk, double threshold,
+                          double* __restrict__ thr) {
+  const int b = blockIdx.x;
+  double m = 0;
+  for (int q = threadIdx.x; q < nblk; q += 64) m = fmax(m, part[(long)b * nblk + q]);
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if (threadIdx.x == 0) thr[b] = threshold * m;
 }
 
 __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
@@ -224,17 +282,137 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   if (k < n) p[k] = v;
 }
 
-// tmp = row-wise dilation (half-width r) of irr > threshold * max.
+// The whole ROI of a 64 x 32 output tile in LDS: threshold over the tile
+// plus a halo of h = r + s, row dilation, column dilation (zero outside the
+// plane), row box sums, column box sums / clipped window count.  Equal to
+// the separable passes below, for r + s <= kRoiHalo.
+constexpr int kRoiW = 64, kRoiH = 32, kRoiHalo = 16;
+constexpr int kRoiBuf = (kRoiW + 2 * kRoiHalo) * (kRoiH + 2 * kRoiHalo);
+
+// OR / sum of len <= 2 * WMAX + 1 LDS entries at stride `st`.  With WMAX > 0
+// the reads are unrolled (predicated), so they issue together instead of as
+// a chain of dependent waits; WMAX = 0 loops over any length.
+template <int WMAX>
+__device__ __forceinline__ unsigned char or_window(const unsigned char* p, int st, int len) {
+  unsigned char v = 0;
+  if constexpr (WMAX > 0) {
+#pragma unroll
+    for (int d = 0; d < 2 * WMAX + 1; ++d)
+      if (d < len) v |= p[d * st];
+  } else {
+    for (int d = 0; d < len; ++d) v |= p[d * st];
+  }
+  return v;
+}
+template <int WMAX, class T>
+__device__ __forceinline__ int sum_window(const T* p, int st, int len) {
+  int v = 0;
+  if constexpr (WMAX > 0) {
+#pragma unroll
+    for (int d = 0; d < 2 * WMAX + 1; ++d)
+      if (d < len) v += p[d * st];
+  } else {
+    for (int d = 0; d < len; ++d) v += p[d * st];
+  }
+  return v;
+}
+
+template <int WMAX>
+__global__ __launch_bounds__(kTB) void k_roi_tile(const double* __restrict__ irr,
+                                                  const double* __restrict__ thrs,
+                                                  double* __restrict__ roi, int nx, int ny, int r,
+                                                  int s) {
+  __shared__ unsigned char bin[kRoiBuf];
+  __shared__ unsigned char tmp[kRoiBuf];
+  __shared__ unsigned char dil[kRoiBuf];
+  __shared__ unsigned short rs[kRoiW * (kRoiH + 2 * kRoiHalo)];
+  const int b = blockIdx.z;
+  const long n = (long)nx * ny;
+  const double* p = irr + (long)b * n;
+  const double thr = thrs[b];
+  const int x0 = blockIdx.x * kRoiW, y0 = blockIdx.y * kRoiH;
+  const int h = r + s;
+  const int BW = kRoiW + 2 * h, BH = kRoiH + 2 * h;
+  const int TW = kRoiW + 2 * s, TH = BH;  // tmp: x from x0 - s, y from y0 - h
+  const int DW = TW, DH = kRoiH + 2 * s;  // dil: x from x0 - s, y from y0 - s
+  // 64 lanes along x, the block's 4 waves along y: no index division
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // every load of the halo'd tile in flight at once (registers), then the
+  // thresholds to LDS; NaN stands for samples outside the plane
+  constexpr int kRows = (kRoiH + 2 * kRoiHalo) / (kTB / 64);
+  double va[kRows], vb[kRows];
+  const int Xa = x0 - h + tx, Xb = Xa + 64;
+  const bool ina = Xa >= 0 && Xa < nx, inb = tx + 64 < BW && Xb >= 0 && Xb < nx;
+#pragma unroll
+  for (int it = 0; it < kRows; ++it) {
+    const int yy = ty + (kTB / 64) * it;
+    const int Y = y0 - h + yy;
+    const bool rowin = yy < BH && Y >= 0 && Y < ny;
+    const double* prow = p + (long)Y * nx;
+    va[it] = (rowin && ina) ? prow[Xa] : NAN;
+    vb[it] = (rowin && inb) ? prow[Xb] : NAN;
+  }
+#pragma unroll
+  for (int it = 0; it < kRows; ++it) {
+    const int yy = ty + (kTB / 64) * it;
+    if (yy < BH) {
+      bin[yy * BW + tx] = va[it] > thr;
+      if (tx + 64 < BW) bin[yy * BW + tx + 64] = vb[it] > thr;
+    }
+  }
+  __syncthreads();
+  for (int yy = ty; yy < TH; yy += kTB / 64) {
+    for (int xx = tx; xx < TW; xx += 64) {
+      tmp[yy * TW + xx] = or_window<WMAX>(bin + yy * BW + xx, 1, 2 * r + 1);
+    }
+  }
+  __syncthreads();
+  for (int yy = ty; yy < DH; yy += kTB / 64) {
+    const int Y = y0 - s + yy;
+    for (int xx = tx; xx < DW; xx += 64) {
+      const int X = x0 - s + xx;
+      const bool in = X >= 0 && X < nx && Y >= 0 && Y < ny;
+      dil[yy * DW + xx] = in ? or_window<WMAX>(tmp + yy * TW + xx, TW, 2 * r + 1) : 0;
+    }
+  }
+  __syncthreads();
+  const int X = x0 + tx;
+  if (s == 0) {
+    for (int yy = ty; yy < kRoiH; yy += kTB / 64) {
+      const int Y = y0 + yy;
+      if (X < nx && Y < ny) roi[(long)b * n + (long)Y * nx + X] = dil[yy * DW + tx];
+    }
+    return;
+  }
+  for (int yy = ty; yy < DH; yy += kTB / 64) {
+    rs[yy * kRoiW + tx] = (unsigned short)sum_window<WMAX>(dil + yy * DW + tx, 1, 2 * s + 1);
+  }
+  __syncthreads();
+  if (X >= nx) return;
+  const int cx = min(nx - 1, X + s) - max(0, X - s) + 1;
+  for (int yy = ty; yy < kRoiH; yy += kTB / 64) {
+    const int Y = y0 + yy;
+    if (Y >= ny) break;
+    const int acc = sum_window<WMAX>(rs + yy * kRoiW + tx, kRoiW, 2 * s + 1);
+    const int cy = min(ny - 1, Y + s) - max(0, Y - s) + 1;
+    roi[(long)b * n + (long)Y * nx + X] = (double)acc / (double)(cx * cy);
+  }
+}
+
+// Separable passes for any r, s (the tile kernel's halo is bounded).
+// tmp = row-wise dilation (half-width r) of irr > thr.
 __global__ __launch_bounds__(kTB) void k_roi_row(const double* __restrict__ irr,
-                                                 const unsigned long long* __restrict__ mx,
+                                                 const double* __restrict__ thrs,
                                                  unsigned char* __restrict__ tmp, int nx, int ny,
-                                                 int r, double threshold) {
+                                                 int r) {
   const int b = blockIdx.y;
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const double thr = threshold * __longlong_as_double((long long)mx[b]);
-  const int i = int(k % nx);
+  const double thr = thrs[b];
+  int i, j;
+  split_ij(k, nx, i, j);
+  (void)j;
   const double* row = irr + (long)b * n + (k - i);
   const int i0 = max(0, i - r), i1 = min(nx - 1, i + r);
   unsigned char v = 0;
@@ -251,7 +429,8 @@ __global__ __launch_bounds__(kTB) void k_roi_col(const unsigned char* __restrict
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const int i = int(k % nx), j = int(k / nx);
+  int i, j;
+  split_ij(k, nx, i, j);
   const unsigned char* col = tmp + (long)b * n + i;
   const int j0 = max(0, j - r), j1 = min(ny - 1, j + r);
   unsigned char v = 0;
@@ -269,7 +448,9 @@ __global__ __launch_bounds__(kTB) void k_box_row(const unsigned char* __restrict
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const int i = int(k % nx);
+  int i, j;
+  split_ij(k, nx, i, j);
+  (void)j;
   const unsigned char* row = dil + (long)b * n + (k - i);
   const int i0 = max(0, i - s), i1 = min(nx - 1, i + s);
   int acc = 0;
@@ -285,7 +466,8 @@ __global__ __launch_bounds__(kTB) void k_box_col(const int* __restrict__ rs, dou
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const int i = int(k % nx), j = int(k / nx);
+  int i, j;
+  split_ij(k, nx, i, j);
   const int* col = rs + (long)b * n + i;
   const int j0 = max(0, j - s), j1 = min(ny - 1, j + s);
   const int i0 = max(0, i - s), i1 = min(nx - 1, i + s);
@@ -303,7 +485,9 @@ __global__ __launch_bounds__(kTB) void k_blur_row(const double* __restrict__ src
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const int i = int(k % nx);
+  int i, j;
+  split_ij(k, nx, i, j);
+  (void)j;
   const double* row = src + (long)b * n + (k - i);
   double acc = 0;
   for (int d = -r; d <= r; ++d) acc += row[min(nx - 1, max(0, i + d))] * ker[d + r];
@@ -316,7 +500,8 @@ __global__ __launch_bounds__(kTB) void k_blur_col(const double* __restrict__ tmp
   const long n = (long)nx * ny;
   const long k = (long)blockIdx.x * kTB + threadIdx.x;
   if (k >= n) return;
-  const int i = int(k % nx), j = int(k / nx);
+  int i, j;
+  split_ij(k, nx, i, j);
   const double* col = tmp + (long)b * n + i;
   double acc = 0;
   for (int d = -r; d <= r; ++d) acc += col[(long)min(ny - 1, max(0, j + d)) * nx] * ker[d + r];
@@ -340,12 +525,12 @@ __device__ __forceinline__ double quantity(const S* re, const S* im, long k, int
   }
 }
 
-// mm[0] = min key, mm[1] = max key over the plane (NaN keys skipped unless
-// with_nan: the sort order's first element).
+// Per-block min / max keys of the plane (NaN keys skipped unless with_nan:
+// the sort order's first element), part[2 * block], part[2 * block + 1].
 template <class S>
 __global__ __launch_bounds__(kTB) void k_img_minmax(const S* __restrict__ re, const S* __restrict__ im,
                                                     long n, int q, int with_nan,
-                                                    unsigned long long* __restrict__ mm) {
+                                                    unsigned long long* __restrict__ part) {
   unsigned long long lo = ~0ull, hi = 0;
   for (long k = (long)blockIdx.x * kTB + threadIdx.x; k < n; k += (long)gridDim.x * kTB) {
     const double v = quantity(re, im, k, q);
@@ -359,9 +544,19 @@ __global__ __launch_bounds__(kTB) void k_img_minmax(const S* __restrict__ re, co
     lo = a < lo ? a : lo;
     hi = c > hi ? c : hi;
   }
+  __shared__ unsigned long long red[2][kTB / 64];
   if ((threadIdx.x & 63) == 0) {
-    atomicMin(mm, lo);
-    atomicMax(mm + 1, hi);
+    red[0][threadIdx.x >> 6] = lo;
+    red[1][threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kTB / 64; ++w) {
+      lo = red[0][w] < lo ? red[0][w] : lo;
+      hi = red[1][w] > hi ? red[1][w] : hi;
+    }
+    part[2 * blockIdx.x] = lo;
+    part[2 * blockIdx.x + 1] = hi;
   }
 }
 
@@ -385,23 +580,26 @@ __global__ __launch_bounds__(kTB) void k_sel_hist(const S* __restrict__ re, cons
 
 // st = {prefix, mask, rank}: pick the digit that holds the rank, clear the
 // histogram for the next pass.
-__global__ void k_sel_pick(unsigned long long* __restrict__ st, unsigned* __restrict__ hist, int shift) {
+__global__ __launch_bounds__(256) void k_sel_pick(unsigned long long* __restrict__ st,
+                                                  unsigned* __restrict__ hist, int shift) {
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = hist[threadIdx.x];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long rank = st[2], cum = 0;
     int d = 255;
     for (int q = 0; q < 256; ++q) {
-      if (cum + hist[q] > rank) {
+      if (cum + h[q] > rank) {
         d = q;
         break;
       }
-      cum += hist[q];
+      cum += h[q];
     }
     st[2] = rank - cum;
     st[0] |= (unsigned long long)d << shift;
     st[1] |= 255ull << shift;
   }
-  __syncthreads();
-  hist[threadIdx.x] = 0;
 }
 
 struct ImgParams {
@@ -411,12 +609,35 @@ struct ImgParams {
   double rgb[8][3];
 };
 
-// range[0] = lo, range[1] = hi as planeToRGB / planeToGray settle them.
-__global__ void k_img_range(const ImgParams P, const unsigned long long* __restrict__ mm,
-                            const unsigned long long* __restrict__ st, double* __restrict__ range) {
-  if (threadIdx.x != 0) return;
-  range[0] = P.has_min ? P.min : dval(mm[0]);
-  range[1] = P.has_max ? P.max : (P.pct ? dval(st[0]) : dval(mm[1]));
+// range[0] = lo, range[1] = hi as planeToRGB / planeToGray settle them, from
+// the given bounds, the folded block min / max keys, or the selected key.
+__global__ __launch_bounds__(256) void k_img_range(const ImgParams P, const unsigned long long* __restrict__ part,
+                                                   int nblk, int scanned,
+                                                   const unsigned long long* __restrict__ st,
+                                                   double* __restrict__ range) {
+  __shared__ unsigned long long red[2][256];
+  unsigned long long lo = ~0ull, hi = 0;
+  if (scanned) {
+    for (int q = threadIdx.x; q < nblk; q += 256) {
+      lo = part[2 * q] < lo ? part[2 * q] : lo;
+      hi = part[2 * q + 1] > hi ? part[2 * q + 1] : hi;
+    }
+  }
+  red[0][threadIdx.x] = lo;
+  red[1][threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (threadIdx.x < w) {
+      const unsigned long long a = red[0][threadIdx.x + w], c = red[1][threadIdx.x + w];
+      if (a < red[0][threadIdx.x]) red[0][threadIdx.x] = a;
+      if (c > red[1][threadIdx.x]) red[1][threadIdx.x] = c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    range[0] = P.has_min ? P.min : dval(red[0][0]);
+    range[1] = P.has_max ? P.max : (P.pct ? dval(st[0]) : dval(red[1][0]));
+  }
 }
 
 template <class S>
@@ -430,7 +651,9 @@ __global__ __launch_bounds__(kTB) void k_image(const S* __restrict__ re, const S
   if (span == 0 || span != span) span = 1;
   const double p = quantity(re, im, k, P.q);
   const int ch = P.gray ? 1 : 3;
-  const long x = k % nx, y = k / nx;
+  int xi, yi;
+  split_ij(k, nx, xi, yi);
+  const long x = xi, y = yi;
   const long stride = (long)P.png + (long)nx * ch;
   unsigned char* o = out + y * stride + P.png + x * ch;
   if (P.png && x == 0) out[y * stride] = 0;
@@ -586,7 +809,9 @@ int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) 
   if (rc) return rc;
   char* base = static_cast<char*>(dev);
   const long n = (long)nx * ny;
-  hipLaunchKernelGGL(k_target, dim3(blocks_for(n), B), dim3(kTB), 0, ctx->stream, (const TgtRec*)base,
+  // a few thousand workgroups in all, each walking its field's grid
+  const unsigned per = std::max(1u, std::min(blocks_for(n), (4096u + B - 1) / B));
+  hipLaunchKernelGGL(k_target, dim3(per, B), dim3(kTB), 0, ctx->stream, (const TgtRec*)base,
                      (const double*)(base + rb), (const unsigned char*)(base + rb + ab), (double*)out->re, nx,
                      ny);
   DOE_HIP(hipGetLastError());
@@ -607,25 +832,35 @@ int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, 
   const bool has_soft = !(soft <= 0);
   const int s = has_soft ? (int)std::min((double)big, js_max(1, js_round(soft))) : 0;
   DOE_HIP(hipSetDevice(ctx->device));
-  int rc = ensure_work(ctx, size_t(n) * B * sizeof(int));
+  const long nblk = std::max(1L, std::min(64L, (n + kTB * 16 - 1) / (kTB * 16)));
+  int rc = ensure_reduce(ctx, size_t(B) * (nblk + 1), 0);
   if (rc) return rc;
-  rc = ensure_reduce(ctx, size_t(B), 0);
-  if (rc) return rc;
-  unsigned char* tmp = static_cast<unsigned char*>(ctx->work_re);
-  unsigned char* dil = tmp + size_t(n) * B;
-  int* rs = static_cast<int*>(ctx->work_im);
-  unsigned long long* mx = reinterpret_cast<unsigned long long*>(ctx->partials);
+  double* part = ctx->partials;
+  double* thr = ctx->partials + size_t(B) * nblk;
   double* irr = static_cast<double*>(t->re);
   double* roi = static_cast<double*>(t->im);
   hipStream_t st = ctx->stream;
-  const dim3 g(blocks_for(n), B);
-  hipLaunchKernelGGL(k_fill_u64, dim3((B + kTB - 1) / kTB), dim3(kTB), 0, st, mx, B, 0ull);
-  hipLaunchKernelGGL(k_roi_max, dim3(red_blocks(n), B), dim3(kTB), 0, st, irr, n, mx);
-  hipLaunchKernelGGL(k_roi_row, g, dim3(kTB), 0, st, irr, mx, tmp, nx, ny, r, threshold);
-  hipLaunchKernelGGL(k_roi_col, g, dim3(kTB), 0, st, tmp, dil, has_soft ? nullptr : roi, nx, ny, r);
-  if (has_soft) {
-    hipLaunchKernelGGL(k_box_row, g, dim3(kTB), 0, st, dil, rs, nx, ny, s);
-    hipLaunchKernelGGL(k_box_col, g, dim3(kTB), 0, st, rs, roi, nx, ny, s);
+  hipLaunchKernelGGL(k_roi_max, dim3(unsigned(nblk), B), dim3(kTB), 0, st, irr, n, part);
+  hipLaunchKernelGGL(k_roi_thr, dim3(B), dim3(64), 0, st, (const double*)part, int(nblk), threshold, thr);
+  if (r + s <= kRoiHalo) {
+    const dim3 g((nx + kRoiW - 1) / kRoiW, (ny + kRoiH - 1) / kRoiH, B);
+    if (r <= 4 && s <= 4)
+      hipLaunchKernelGGL(k_roi_tile<4>, g, dim3(kTB), 0, st, irr, (const double*)thr, roi, nx, ny, r, s);
+    else
+      hipLaunchKernelGGL(k_roi_tile<0>, g, dim3(kTB), 0, st, irr, (const double*)thr, roi, nx, ny, r, s);
+  } else {
+    rc = ensure_work(ctx, size_t(n) * B * sizeof(int));
+    if (rc) return rc;
+    unsigned char* tmp = static_cast<unsigned char*>(ctx->work_re);
+    unsigned char* dil = tmp + size_t(n) * B;
+    int* rsum = static_cast<int*>(ctx->work_im);
+    const dim3 g(blocks_for(n), B);
+    hipLaunchKernelGGL(k_roi_row, g, dim3(kTB), 0, st, irr, (const double*)thr, tmp, nx, ny, r);
+    hipLaunchKernelGGL(k_roi_col, g, dim3(kTB), 0, st, tmp, dil, has_soft ? nullptr : roi, nx, ny, r);
+    if (has_soft) {
+      hipLaunchKernelGGL(k_box_row, g, dim3(kTB), 0, st, dil, rsum, nx, ny, s);
+      hipLaunchKernelGGL(k_box_col, g, dim3(kTB), 0, st, rsum, roi, nx, ny, s);
+    }
   }
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));
@@ -730,13 +965,15 @@ int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image
   DOE_HIP(hipSetDevice(ctx->device));
   int rc = ensure_work(ctx, size_t(need));
   if (rc) return rc;
-  // reduction scratch: mm[2], st[3], range[2] (doubles), hist[256] (u32)
-  rc = ensure_reduce(ctx, 8 + 128, 0);
+  // reduction scratch: block min / max keys (2 per block), st[3], range[2], hist[256] (u32)
+  const unsigned nblk = std::max(1u, std::min(1024u, (unsigned)((n + kTB * 8 - 1) / (kTB * 8))));
+  const size_t o_st = 2 * size_t(nblk), o_rg = o_st + 3, o_h = o_rg + 3;
+  rc = ensure_reduce(ctx, o_h + 128, 0);
   if (rc) return rc;
-  unsigned long long* mm = reinterpret_cast<unsigned long long*>(ctx->partials);
-  unsigned long long* st = mm + 2;
-  double* range = ctx->partials + 5;
-  unsigned* hist = reinterpret_cast<unsigned*>(ctx->partials + 8);
+  unsigned long long* part = reinterpret_cast<unsigned long long*>(ctx->partials);
+  unsigned long long* st = part + o_st;
+  double* range = ctx->partials + o_rg;
+  unsigned* hist = reinterpret_cast<unsigned*>(ctx->partials + o_h);
   unsigned char* img = static_cast<unsigned char*>(ctx->work_re);
   hipStream_t sm = ctx->stream;
   const size_t plane = size_t(n) * dtype_size(f->dtype);
@@ -748,9 +985,7 @@ int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image
     const S* r_ = static_cast<const S*>(re);
     const S* i_ = static_cast<const S*>(im);
     if (need_scan) {
-      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, mm, 1, ~0ull);
-      hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, mm + 1, 1, 0ull);
-      hipLaunchKernelGGL(k_img_minmax<S>, dim3(red_blocks(n)), dim3(kTB), 0, sm, r_, i_, n, P.q, P.pct, mm);
+      hipLaunchKernelGGL(k_img_minmax<S>, dim3(nblk), dim3(kTB), 0, sm, r_, i_, n, P.q, P.pct, part);
       if (P.pct && !P.has_max) {
         const double kk = std::floor((double)n * o->percentile);
         const unsigned long long rank = (unsigned long long)std::min((double)(n - 1), kk);
@@ -758,15 +993,16 @@ int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, sm, st + 2, 1, rank);
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(128), 0, sm,
                            reinterpret_cast<unsigned long long*>(hist), 128, 0ull);
+        const unsigned sblk = std::min(nblk, 512u);
         for (int shift = 56; shift >= 0; shift -= 8) {
-          hipLaunchKernelGGL(k_sel_hist<S>, dim3(red_blocks(n)), dim3(kTB), 0, sm, r_, i_, n, P.q,
+          hipLaunchKernelGGL(k_sel_hist<S>, dim3(sblk), dim3(kTB), 0, sm, r_, i_, n, P.q,
                              (const unsigned long long*)st, hist, shift);
           hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(256), 0, sm, st, hist, shift);
         }
       }
     }
-    hipLaunchKernelGGL(k_img_range, dim3(1), dim3(64), 0, sm, P, (const unsigned long long*)mm,
-                       (const unsigned long long*)st, range);
+    hipLaunchKernelGGL(k_img_range, dim3(1), dim3(256), 0, sm, P, (const unsigned long long*)part, int(nblk),
+                       need_scan ? 1 : 0, (const unsigned long long*)st, range);
     hipLaunchKernelGGL(k_image<S>, dim3(blocks_for(n)), dim3(kTB), 0, sm, r_, i_, nx, n, P,
                        (const double*)range, img);
     return DOE_OK;

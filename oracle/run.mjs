@@ -231,6 +231,13 @@ if (op === 'fft2d' || op === 'dft2d') {
     fn = function () {
       weightedGerchbergSaxton(ill, tgt, { nx: n, ny: n, iterations: a.iterations || 1, initPhase: randomPhase(n, n, 1) }, fourierCgh);
     };
+  } else if (a.what === 'target') {
+    // makeTarget of one kind (its autoRoi included) on an n x n grid
+    fn = function () { makeTarget(a.kind || 'text', n, n, a.opts || {}); };
+  } else if (a.what === 'image') {
+    // planeToRGB of an n x n plane (options as given, e.g. a percentile)
+    const plane = new Float64Array(n * n).map(function (_, k) { return Math.abs(Math.sin(k * 0.37)) ** 4; });
+    fn = function () { planeToRGB(plane, a.opts || {}); };
   } else {
     throw new Error('unknown timing target ' + a.what);
   }

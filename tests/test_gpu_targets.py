@@ -116,7 +116,9 @@ def test_mixed_batch_equals_single_calls(ctx):
         np.testing.assert_array_equal(im_b[q], im1[0])
 
 
-@pytest.mark.parametrize("dilate,soft,threshold", [(2, 2, 0.05), (0.3, 0, 0.05), (4.5, 3, 0.2), (40, 7, 0.01)])
+# (8, 8): the largest halo of the tiled ROI kernel; (9, 8) and (40, 7): the separable passes
+@pytest.mark.parametrize("dilate,soft,threshold", [(2, 2, 0.05), (0.3, 0, 0.05), (4.5, 3, 0.2), (8, 8, 0.05),
+                                                   (9, 8, 0.05), (16, 0, 0.05), (40, 7, 0.01)])
 def test_auto_roi_bit_exact_on_custom_irradiance(ctx, dilate, soft, threshold):
     rng = np.random.default_rng(13)
     nx, ny = 93, 70
