@@ -23,7 +23,10 @@ FP16 and BF16 (the half-width ones compute in FP32).
 
 "targets" times makeTarget + autoRoi on the device for 64 mixed 1024^2
 targets (text and image rasters, disc, ring, spot grid, zone plate, checker,
-lines) and planeToRGB (viridis, percentile 0.99) of a 2048^2 field.
+lines) and planeToRGB (viridis, percentile 0.99) of a 2048^2 field;
+"fabrication" times errorDiffusionQuantize (4 levels) on one 1024^2 image
+with plain rows (wavefront) and on 64 images of 512^2 with serpentine rows
+(one serial chain per image).
 
 The second half of the metric, WGS time to screen RMSE <= 1 % at 1024^2
 (learned warm start vs plain random start), is reported under "wgs": a
@@ -272,11 +275,17 @@ def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int 
     ctx.target_make(specs, n, n, out)
     ctx.target_auto_roi(out)
     ctx.sync()
+    ctx.set_flags(asynchronous=True, profile=True)
     ctx.timer_start()
     for _ in range(reps):
         ctx.target_make(specs, n, n, out)
         ctx.target_auto_roi(out)
-    ms = ctx.timer_stop() / reps
+    call_ms = ctx.timer_stop() / reps
+    prof = ctx.profile_report()
+    ctx.set_flags(asynchronous=True, profile=False)
+    # device time: the bracketed launches of both calls (the call time adds the
+    # host's per-call work, e.g. copying 1 MB of rasters into the parameter block)
+    ms = sum(tot for _, tot, _ in prof.values()) / reps
     out.free()
     pts = count * n * n
     # planeToRGB (viridis, percentile 0.99) of an FP32 field's intensity, PNG rows
@@ -295,11 +304,45 @@ def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int 
     ctx.set_flags(asynchronous=True, profile=False)
     return {
         "targets": count, "n": n, "kinds": sorted({k if o != "image" else "image" for k, o in TGT_KINDS}),
-        "ms": round(ms, 4), "us_per_target": round(ms * 1e3 / count, 2),
+        "ms": round(ms, 4), "call_ms": round(call_ms, 4), "us_per_target": round(ms * 1e3 / count, 2),
+        "kernels_ms": {t: round(tot / reps, 4) for t, (_, tot, _) in prof.items()},
         "gbs_alg": round(24 * pts / (ms * 1e-3) / 1e9, 1), "bytes_per_pt": 24,
         "image": {"n": img_n, "quantity": "intensity", "map": "viridis", "percentile": 0.99,
                   "ms_wall_incl_d2h": round(img_ms, 3)},
     }
+
+
+def fab_leg(ctx, n: int = 1024, batch: int = 64, nb: int = 512) -> dict:
+    """errorDiffusionQuantize (4 levels) on the device: one n^2 image with
+    plain rows (diagonal wavefront, one workgroup) and a batch of nb^2 images
+    with serpentine rows (one serial chain per image, one lane each)."""
+    rng = np.random.default_rng(5)
+    out = {}
+    f = ctx.upload(rng.uniform(-np.pi, np.pi, (1, n, n)), None, "f64")
+    o = ctx.field(n, n, 1, "f64")
+    ctx.set_flags(asynchronous=False, profile=False)
+    ctx.error_diffusion(f, o, 4, snake=False)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.error_diffusion(f, o, 4, snake=False)
+    out["wavefront_ms"] = round((time.perf_counter() - t0) * 1e3 / 3, 3)
+    out["wavefront_n"] = n
+    f.free()
+    o.free()
+    fb = ctx.field(nb, nb, batch, "f64")
+    ob = ctx.field(nb, nb, batch, "f64")
+    for b in range(batch):
+        v = ctx.wrap(fb.ptrs()[0] + b * nb * nb * 8, fb.ptrs()[1] + b * nb * nb * 8, nb, nb, 1, "f64")
+        ph = rng.uniform(-np.pi, np.pi, (nb, nb))
+        v.upload(ph, ph)
+        v.free()
+    t0 = time.perf_counter()
+    ctx.error_diffusion(fb, ob, 4, snake=True)
+    out["serpentine_batch"] = {"images": batch, "n": nb, "ms": round((time.perf_counter() - t0) * 1e3, 2)}
+    fb.free()
+    ob.free()
+    ctx.set_flags(asynchronous=True, profile=False)
+    return out
 
 
 def main() -> int:
@@ -314,7 +357,8 @@ def main() -> int:
                     help="skip the fft2d autotuner (built-in configuration)")
     ap.add_argument("--no-wgs", action="store_true", help="skip the WGS warm-start leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven GS leg")
-    ap.add_argument("--no-targets", action="store_true", help="skip the target / figure leg")
+    ap.add_argument("--no-targets", action="store_true",
+                    help="skip the target / figure and fabrication legs")
     ap.add_argument("--wgs-train", type=int, default=128)
     ap.add_argument("--wgs-steps", type=int, default=600)
     ap.add_argument("--wgs-eval", type=int, default=32)
@@ -411,6 +455,7 @@ def main() -> int:
     field.free()
     c3 = None if args.no_c3 else c3_leg(ctx)
     targets = None if args.no_targets else targets_leg(ctx)
+    fab = None if args.no_targets else fab_leg(ctx)
     wgs = None if args.no_wgs else wgs_leg(ctx, args.wgs_train, args.wgs_steps, args.wgs_eval,
                                            args.wgs_iters)
 
@@ -450,6 +495,7 @@ def main() -> int:
         "storage_variants": variants,
         "gs_batch_uneven": c3,
         "targets": targets,
+        "fabrication": fab,
         "wgs": wgs,
         "cpu_baseline": cpu,
     }

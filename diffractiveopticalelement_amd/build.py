@@ -47,6 +47,7 @@ SOURCES = [
     ("warmstart.hip", [None]),
     ("metrics.hip", [None]),
     ("targets.hip", [None]),
+    ("fab.hip", [None]),
 ]
 
 

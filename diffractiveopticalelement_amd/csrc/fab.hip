@@ -1,0 +1,226 @@
+// Fabrication output on the device: Floyd–Steinberg error diffusion of a
+// wrapped phase (errorDiffusionQuantize, doe.js:185-226) and the etch-depth
+// map (phaseToHeight, doe.js:967-976), in FP64 with the reference's operation
+// order and contraction off, so results equal the reference bit for bit.
+//
+// Error diffusion is a recurrence.  A sample's input holds, in order, the
+// 1/16, 5/16 and 3/16 shares of the three samples above it and then the 7/16
+// share of its predecessor in the row.  Two schedules:
+//   serpentine rows (the reference's default): a row runs in the direction
+//     opposite to the previous one, so its first sample needs the previous
+//     row complete and every sample of an image is one step of one chain.
+//     One lane per image: a batch of images runs as parallel chains;
+//   left-to-right rows: sample (i, j) needs (i+1, j-1) and (i-1, j), so row
+//     j can trail row j-1 by two samples — a diagonal wavefront of
+//     nx + 2 (ny - 1) steps per image, one workgroup per image, one barrier
+//     per step.
+// Both keep the pending shares of the next row in registers (a sample's
+// three shares arrive from three consecutive samples of the row above) and
+// read each sample's original value once.
+#include <algorithm>
+#include <cmath>
+
+#include "ctx.hpp"
+
+#pragma clang fp contract(off)
+
+namespace doe {
+namespace {
+
+constexpr int kFB = 256;
+
+// wrapPhase of one value (propagate.js:410-418): p % 2pi, then into (-pi, pi].
+__device__ __forceinline__ double wrap1(double p) {
+  const double two_pi = 2 * M_PI;
+  p = fmod(p, two_pi);
+  if (p > M_PI)
+    p -= two_pi;
+  else if (p <= -M_PI)
+    p += two_pi;
+  return p;
+}
+
+// Math.round: nearest integer, ties toward +Infinity.
+__device__ __forceinline__ double js_round(double x) {
+  if (x != x || x == INFINITY || x == -INFINITY) return x;
+  const double f = floor(x);
+  return (x - f >= 0.5) ? f + 1 : f;
+}
+
+// One sample: quantise v, return the wrapped error; q is the level.
+__device__ __forceinline__ double quantise(double v, double step, double* q) {
+  *q = js_round(v / step) * step;
+  double e = v - *q;
+  if (e > M_PI)
+    e -= 2 * M_PI;
+  else if (e <= -M_PI)
+    e += 2 * M_PI;
+  return e;
+}
+
+// Serpentine (or plain, with snake = 0) scan of one image per lane.  out
+// holds the working values: row j+1's sums are written as its samples
+// complete, and the levels replace the values as the scan passes; wrapped at
+// the end.
+__global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ in, double* __restrict__ out,
+                                                     int nx, int ny, int batch, double step, int snake,
+                                                     int wrap_first) {
+  const int b = blockIdx.x * 64 + threadIdx.x;
+  if (b >= batch) return;
+  const long n = (long)nx * ny;
+  const double* src = in + (long)b * n;
+  double* o = out + (long)b * n;
+  // out doubles as the working buffer of the row below the current one:
+  // o[next row] holds that row's partial sums until the row is processed.
+  for (long k = 0; k < nx; ++k) o[k] = wrap_first ? wrap1(src[k]) : src[k];
+  for (int j = 0; j < ny; ++j) {
+    const int dir = (snake && (j & 1)) ? -1 : 1;
+    const bool below = j + 1 < ny;
+    double* row = o + (long)j * nx;
+    double* nrow = o + (long)(j + 1) * nx;
+    const double* nsrc = src + (long)(j + 1) * nx;
+    double carry = 0;  // 7/16 of the previous sample of this row
+    // pending sums of the row below: p_next (the current sample's: initial
+    // value + 1/16 of the sample behind), p_cur (the sample behind's: + 5/16),
+    // which completes with the current sample's 3/16
+    const int i0 = dir == 1 ? 0 : nx - 1;
+    double p_next = below ? (wrap_first ? wrap1(nsrc[i0]) : nsrc[i0]) : 0, p_cur = 0;
+    // blocks of 8 samples: the block's inputs are loaded together, then the
+    // chain runs on registers
+    for (int c0 = 0; c0 < nx; c0 += 8) {
+      double rv[8], av[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        const int i = dir == 1 ? c : nx - 1 - c;
+        rv[u] = c < nx ? row[i] : 0;
+        av[u] = 0;
+        if (below && c + 1 < nx) av[u] = wrap_first ? wrap1(nsrc[i + dir]) : nsrc[i + dir];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int c = c0 + u;
+        if (c >= nx) break;
+        const int i = dir == 1 ? c : nx - 1 - c;
+        // the first sample of a row has no 7/16 share (and keeps a -0)
+        const double v = c ? rv[u] + carry : rv[u];
+        double q;
+        const double e = quantise(v, step, &q);
+        row[i] = q;
+        carry = e * (7.0 / 16);
+        if (below) {
+          if (c > 0) nrow[i - dir] = p_cur + e * (3.0 / 16);
+          p_cur = p_next + e * (5.0 / 16);
+          p_next = av[u] + e * (1.0 / 16);
+        }
+      }
+    }
+    if (below) nrow[dir == 1 ? nx - 1 : 0] = p_cur;
+  }
+  for (long k = 0; k < n; ++k) o[k] = wrap1(o[k]);
+}
+
+// Plain (left-to-right) rows on a diagonal wavefront: step t processes sample
+// i = t - 2j of every row j in flight.  One workgroup per image; rows are
+// dealt to threads cyclically; out holds the working values in place.
+__global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict__ in, double* __restrict__ out,
+                                                       int nx, int ny, double step, int wrap_first) {
+  const int b = blockIdx.x;
+  const long n = (long)nx * ny;
+  const double* src = in + (long)b * n;
+  double* o = out + (long)b * n;
+  for (long k = threadIdx.x; k < n; k += blockDim.x) o[k] = wrap_first ? wrap1(src[k]) : src[k];
+  __syncthreads();
+  // per owned row: the pending 7/16 share (rows beyond kMaxRows per thread
+  // are not launched: ny <= 8 * blockDim.x)
+  constexpr int kMaxRows = 8;
+  double carry[kMaxRows];
+#pragma unroll
+  for (int q = 0; q < kMaxRows; ++q) carry[q] = 0;
+  const int steps = nx + 2 * (ny - 1);
+  for (int t = 0; t < steps; ++t) {
+#pragma unroll
+    for (int q = 0; q < kMaxRows; ++q) {
+      const int j = threadIdx.x + q * blockDim.x;
+      const int i = t - 2 * j;
+      if (j < ny && i >= 0 && i < nx) {
+        double* row = o + (long)j * nx;
+        const double v = i ? row[i] + carry[q] : row[i];
+        double qv;
+        const double e = quantise(v, step, &qv);
+        row[i] = qv;
+        carry[q] = e * (7.0 / 16);
+        if (j + 1 < ny) {
+          double* nrow = row + nx;
+          // the order of the shares into a sample below matches the serial
+          // scan: (i+1, j+1) gets 1/16 now, (i, j+1) its 5/16, (i-1, j+1) its 3/16
+          if (i + 1 < nx) nrow[i + 1] += e * (1.0 / 16);
+          nrow[i] += e * (5.0 / 16);
+          if (i >= 1) nrow[i - 1] += e * (3.0 / 16);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (long k = threadIdx.x; k < n; k += blockDim.x) o[k] = wrap1(o[k]);
+}
+
+// phaseToHeight: ((phase * s) % step + step) % step, s = lambda / (2 pi (n - 1)),
+// step = lambda / (n - 1).
+__global__ __launch_bounds__(kFB) void k_height(const double* __restrict__ ph, double* __restrict__ h, long total,
+                                                double s, double step) {
+  const long k = (long)blockIdx.x * kFB + threadIdx.x;
+  if (k < total) h[k] = fmod(fmod(ph[k] * s, step) + step, step);
+}
+
+}  // namespace
+}  // namespace doe
+
+using namespace doe;
+
+extern "C" {
+
+int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, int levels, int snake,
+                        int wrap_first) {
+  DOE_REQUIRE(ctx && phase && out, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(phase->dtype == DOE_F64 && out->dtype == DOE_F64, DOE_ERR_INVALID,
+              "errorDiffusionQuantize: FP64 fields");
+  DOE_REQUIRE(phase->nx == out->nx && phase->ny == out->ny && phase->batch == out->batch, DOE_ERR_SHAPE,
+              "errorDiffusionQuantize: phase and output grids differ");
+  DOE_REQUIRE(phase->re != out->re, DOE_ERR_INVALID, "errorDiffusionQuantize: output overlaps the phase");
+  DOE_REQUIRE(levels != 0, DOE_ERR_INVALID, "errorDiffusionQuantize: levels must be non-zero");
+  const int nx = phase->nx, ny = phase->ny, B = phase->batch;
+  const double step = (2 * M_PI) / levels;
+  DOE_HIP(hipSetDevice(ctx->device));
+  const double* in = static_cast<const double*>(phase->re);
+  double* o = static_cast<double*>(out->re);
+  if (snake) {
+    hipLaunchKernelGGL(k_diffuse_lane, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step,
+                       1, wrap_first ? 1 : 0);
+  } else {
+    DOE_REQUIRE(ny <= 8 * 1024, DOE_ERR_UNSUPPORTED, "errorDiffusionQuantize: at most 8192 rows");
+    hipLaunchKernelGGL(k_diffuse_wave, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step,
+                       wrap_first ? 1 : 0);
+  }
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_phase_to_height(doe_ctx* ctx, const doe_field* phase, doe_field* out, double lambda, double n_index) {
+  DOE_REQUIRE(ctx && phase && out, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(phase->dtype == DOE_F64 && out->dtype == DOE_F64, DOE_ERR_INVALID, "phaseToHeight: FP64 fields");
+  DOE_REQUIRE(phase->nx == out->nx && phase->ny == out->ny && phase->batch == out->batch, DOE_ERR_SHAPE,
+              "phaseToHeight: phase and output grids differ");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const double s = lambda / (2 * M_PI * (n_index - 1));
+  const double step = lambda / (n_index - 1);
+  const long total = (long)phase->count();
+  hipLaunchKernelGGL(k_height, dim3(unsigned((total + kFB - 1) / kFB)), dim3(kFB), 0, ctx->stream,
+                     static_cast<const double*>(phase->re), static_cast<double*>(out->re), total, s, step);
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+}  // extern "C"

@@ -811,9 +811,11 @@ int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) 
   const long n = (long)nx * ny;
   // a few thousand workgroups in all, each walking its field's grid
   const unsigned per = std::max(1u, std::min(blocks_for(n), (4096u + B - 1) / B));
+  prof_begin(ctx, "target_make", n * B);
   hipLaunchKernelGGL(k_target, dim3(per, B), dim3(kTB), 0, ctx->stream, (const TgtRec*)base,
                      (const double*)(base + rb), (const unsigned char*)(base + rb + ab), (double*)out->re, nx,
                      ny);
+  prof_end(ctx);
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));
   return DOE_OK;
@@ -840,6 +842,7 @@ int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, 
   double* irr = static_cast<double*>(t->re);
   double* roi = static_cast<double*>(t->im);
   hipStream_t st = ctx->stream;
+  prof_begin(ctx, "target_roi", n * B);
   hipLaunchKernelGGL(k_roi_max, dim3(unsigned(nblk), B), dim3(kTB), 0, st, irr, n, part);
   hipLaunchKernelGGL(k_roi_thr, dim3(B), dim3(64), 0, st, (const double*)part, int(nblk), threshold, thr);
   if (r + s <= kRoiHalo) {
@@ -862,6 +865,7 @@ int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, 
       hipLaunchKernelGGL(k_box_col, g, dim3(kTB), 0, st, rsum, roi, nx, ny, s);
     }
   }
+  prof_end(ctx);
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));
   return DOE_OK;

@@ -65,8 +65,34 @@ export function quantizePhase(phi, levels = 4) {
   return wrapPhase(out);
 }
 
-// Serpentine Floyd–Steinberg on the wrapped phase (ref: doe.js:185-226).
+// Floyd–Steinberg on the wrapped phase (ref: doe.js:185-226).  Plain
+// left-to-right rows (snake: false) run on the GPU as a diagonal wavefront
+// (doe_error_diffusion; row j trails row j-1 by two samples).  Serpentine
+// rows (the default) make the whole image one serial chain — every row needs
+// the previous one complete — so a single image runs here on the host, where
+// one core outpaces one GPU lane on a dependent FP64 chain; batches of
+// serpentine images run on the device one lane each through the C ABI.
 export function errorDiffusionQuantize(phi, levels = 4, { snake = true, wrapFirst = true } = {}) {
+  if (!snake) {
+    const n = phi.length;
+    const nx = Math.round(Math.sqrt(n));
+    const ny = n / nx;
+    // a non-integer row count passes the reference's check and reads past
+    // the array there; the device path refuses it with the same message
+    if (nx * ny !== n || !Number.isInteger(ny)) throw new Error('errorDiffusionQuantize: expected a square grid');
+    const plane = phi instanceof Float64Array ? phi : Float64Array.from(phi);
+    const src = native.fieldCreate(nx, ny, 1, 1);
+    native.fieldUpload(src, plane, plane);
+    const dst = native.fieldCreate(nx, ny, 1, 1);
+    native.errorDiffusion(src, dst, levels, false, wrapFirst);
+    const out = new Float64Array(n);
+    native.fieldDownload(dst, out, new Float64Array(n));
+    return out;
+  }
+  return errorDiffusionHost(phi, levels, wrapFirst);
+}
+
+function errorDiffusionHost(phi, levels, wrapFirst) {
   const n = phi.length;
   const buf = wrapFirst ? wrapPhase(Float64Array.from(phi)) : Float64Array.from(phi);
   const out = new Float64Array(n);
@@ -76,7 +102,7 @@ export function errorDiffusionQuantize(phi, levels = 4, { snake = true, wrapFirs
   if (nx * ny !== n) throw new Error('errorDiffusionQuantize: expected a square grid');
   const taps = [[1, 0, 7 / 16], [-1, 1, 3 / 16], [0, 1, 5 / 16], [1, 1, 1 / 16]];
   for (let j = 0; j < ny; j++) {
-    const dir = snake && j % 2 === 1 ? -1 : 1;
+    const dir = j % 2 === 1 ? -1 : 1;
     for (let c = 0; c < nx; c++) {
       const i = dir === 1 ? c : nx - 1 - c;
       const k = i + j * nx;
@@ -93,6 +119,17 @@ export function errorDiffusionQuantize(phi, levels = 4, { snake = true, wrapFirs
     }
   }
   return wrapPhase(out);
+}
+
+// Etch depth in [0, lambda / (n - 1)) of a host phase array (ref:
+// doe.js:967-976).  An elementwise map of a host array is cheaper here than
+// a PCIe round trip; device-resident phases use doe_phase_to_height.
+export function phaseToHeight(phase, lambda, n = 1.52) {
+  const s = lambda / (2 * Math.PI * (n - 1));
+  const step = lambda / (n - 1);
+  const h = new Float64Array(phase.length);
+  for (let k = 0; k < phase.length; k++) h[k] = (((phase[k] * s) % step) + step) % step;
+  return h;
 }
 
 // Annular mask (ref: doe.js:288-305), used by illuminationProfile('annulus').

@@ -813,6 +813,21 @@ napi_value FieldImage(napi_env env, napi_callback_info info) {
   return obj;
 }
 
+// errorDiffusion(src, dst, levels, snake, wrapFirst): re planes, FP64
+napi_value ErrorDiffusion(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  doe_field* d = a.n > 1 ? field_of(env, a.v[1]) : nullptr;
+  if (!s || !d || a.n < 5) {
+    napi_throw_type_error(env, nullptr, "errorDiffusion(src, dst, levels, snake, wrapFirst)");
+    return nullptr;
+  }
+  throw_status(env, doe_error_diffusion(ctx(env), s, d, (int)num(env, a.v[2], 4),
+                                        boolean(env, a.v[3], true) ? 1 : 0, boolean(env, a.v[4], true) ? 1 : 0));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -836,6 +851,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"targetAutoRoi", nullptr, TargetAutoRoi, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"rasterBlur", nullptr, RasterBlur, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fieldImage", nullptr, FieldImage, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"errorDiffusion", nullptr, ErrorDiffusion, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

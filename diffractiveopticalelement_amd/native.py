@@ -173,6 +173,8 @@ SIGNATURES = [
     ("doe_target_auto_roi", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double]),
     ("doe_raster_blur", C.c_int, [_VP, _VP, _VP, C.c_double, c_double_p]),
     ("doe_image_opts_init", C.c_int, [C.POINTER(ImageOpts)]),
+    ("doe_error_diffusion", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int]),
+    ("doe_phase_to_height", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double]),
     ("doe_field_image", C.c_int, [_VP, _VP, C.c_int, C.POINTER(ImageOpts), _VP, C.c_long,
                                   c_double_p]),
     ("doe_gs_multiplane_run", C.c_int, [_VP, C.POINTER(GsPlan), c_double_p, C.c_int,
@@ -487,6 +489,14 @@ class Context:
         check(self.lib.doe_field_image(self.h, f.h, index, C.byref(o), out.ctypes.data, out.size,
                                        _dptr(rng)))
         return out, (rng[0], rng[1])
+
+    # ---- fabrication output ----------------------------------------
+    def error_diffusion(self, phase: "Field", out: "Field", levels: int = 4, snake: bool = True,
+                        wrap_first: bool = True) -> None:
+        check(self.lib.doe_error_diffusion(self.h, phase.h, out.h, levels, int(snake), int(wrap_first)))
+
+    def phase_to_height(self, phase: "Field", out: "Field", lam: float, n_index: float = 1.52) -> None:
+        check(self.lib.doe_phase_to_height(self.h, phase.h, out.h, lam, n_index))
 
     def gs_run(self, plan: GsPlan, illumination, target_amp, roi=None, init_phase=None,
                seed: int = 12345):

@@ -417,6 +417,20 @@ int doe_image_opts_init(doe_image_opts* opts);
 int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image_opts* opts,
                     unsigned char* out, long out_len, double* range_out);
 
+/* ---- fabrication output (ref: src/doe.js:185-226, 967-976) ------------- *
+ * errorDiffusionQuantize of the re plane of every field (FP64, out re gets
+ * the wrapped levels; out must not be the input): Floyd–Steinberg with the
+ * error taken on the circle, levels uniformly spaced, rows serpentine
+ * (snake = 1, the reference's default: each image is one serial chain, the
+ * images of a batch run in parallel, one lane each) or all left to right
+ * (snake = 0: a diagonal wavefront per image, at most 8192 rows); wrap_first
+ * wraps the input first.  Equal to the reference bit for bit.
+ * phaseToHeight: out re = ((phase * s) % step + step) % step with
+ * s = lambda / (2 pi (n - 1)), step = lambda / (n - 1). */
+int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, int levels, int snake,
+                        int wrap_first);
+int doe_phase_to_height(doe_ctx* ctx, const doe_field* phase, doe_field* out, double lambda, double n_index);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,7 @@
 // identities).  Iterate-level values beyond those are "parity unpinned".
 
 import { ComplexField, discMask } from './field.js';
-import { propagateAngularSpectrumInPlace, lensPhase } from './propagate.js';
+import { propagateAngularSpectrumInPlace, lensPhase, wrapPhase } from './propagate.js';
 import { fft2d, fftshift2d, ifftshift2d } from './fft.js';
 
 export const DEFAULT_DESIGN = {
@@ -358,6 +358,61 @@ export function fourierCgh(illumination, targetAmp, opts) {
     }
   }
   return { phase: phase, error: error, efficiency: efficiency, amplitude: amp, intensity: inten, iterations: iterations, field: null };
+}
+
+// Nearest of `levels` uniform phase levels, wrapped (ref: doe.js:167-174).
+export function quantizePhase(phi, levels) {
+  levels = pick(levels, 4);
+  const step = (2 * Math.PI) / levels;
+  const out = new Float64Array(phi.length);
+  for (let k = 0; k < phi.length; k++) out[k] = Math.round(phi[k] / step) * step;
+  return wrapPhase(out);
+}
+
+// Floyd–Steinberg on the circle (ref: doe.js:185-226): each sample is
+// rounded to a level, its wrapped error is scattered 7/16 ahead in the row
+// and 3/16, 5/16, 1/16 to the row below (mirrored on right-to-left rows).
+export function errorDiffusionQuantize(phi, levels, opts) {
+  levels = pick(levels, 4);
+  const o = opts || {};
+  const snake = pick(o.snake, true);
+  const wrapFirst = pick(o.wrapFirst, true);
+  const n = phi.length;
+  const nx = Math.round(Math.sqrt(n));
+  const ny = n / nx;
+  if (nx * ny !== n) throw new Error('errorDiffusionQuantize: expected a square grid');
+  const buf = wrapFirst ? wrapPhase(Float64Array.from(phi)) : Float64Array.from(phi);
+  const out = new Float64Array(n);
+  const step = (2 * Math.PI) / levels;
+  const scatter = [[1, 0, 7 / 16], [-1, 1, 3 / 16], [0, 1, 5 / 16], [1, 1, 1 / 16]];
+  for (let j = 0; j < ny; j++) {
+    const dir = snake && j % 2 === 1 ? -1 : 1;
+    for (let c = 0; c < nx; c++) {
+      const i = dir > 0 ? c : nx - 1 - c;
+      const k = i + j * nx;
+      const q = Math.round(buf[k] / step) * step;
+      out[k] = q;
+      let e = buf[k] - q;
+      if (e > Math.PI) e -= 2 * Math.PI;
+      else if (e <= -Math.PI) e += 2 * Math.PI;
+      for (let t = 0; t < scatter.length; t++) {
+        const ii = i + dir * scatter[t][0];
+        const jj = j + scatter[t][1];
+        if (ii >= 0 && ii < nx && jj < ny) buf[ii + jj * nx] += e * scatter[t][2];
+      }
+    }
+  }
+  return wrapPhase(out);
+}
+
+// Etch depth in [0, lambda / (n - 1)) (ref: doe.js:967-976).
+export function phaseToHeight(phase, lambda, nIndex) {
+  nIndex = pick(nIndex, 1.52);
+  const s = lambda / (2 * Math.PI * (nIndex - 1));
+  const step = lambda / (nIndex - 1);
+  const h = new Float64Array(phase.length);
+  for (let k = 0; k < phase.length; k++) h[k] = (((phase[k] * s) % step) + step) % step;
+  return h;
 }
 
 // Separable Gaussian blur with clamped edges (ref: doe.js:761-794).

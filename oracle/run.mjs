@@ -15,7 +15,7 @@ import {
   focalPlaneSpectrum,
   focusSweep,
 } from './js/propagate.js';
-import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane, blurRaster } from './js/doe.js';
+import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane, blurRaster, errorDiffusionQuantize, quantizePhase, phaseToHeight } from './js/doe.js';
 import { ProjectorSystem, makeTarget, autoRoi, resampleRaster, renderText } from './js/projector.js';
 import { planeToRGB, planeToGray } from './js/color.js';
 import { ssim, rmsError, correlation, regionStats } from './js/metrics.js';
@@ -199,6 +199,12 @@ if (op === 'fft2d' || op === 'dft2d') {
   out('bytes', a.gray ? r.gray : r.rgb);
   res.lo = r.lo;
   res.hi = r.hi;
+} else if (op === 'error_diffusion') {
+  out('out', errorDiffusionQuantize(inp('phase'), a.levels, a.opts || {}));
+} else if (op === 'quantize') {
+  out('out', quantizePhase(inp('phase'), a.levels));
+} else if (op === 'phase_to_height') {
+  out('out', phaseToHeight(inp('phase'), a.lambda, a.n));
 } else if (op === 'metrics') {
   const p = inp('plane');
   const t = inp('target');

@@ -6,7 +6,7 @@ import * as offt from '../../oracle/js/fft.js';
 import * as oprop from '../../oracle/js/propagate.js';
 import { ComplexField as OField } from '../../oracle/js/field.js';
 import { ProjectorSystem as OProjector } from '../../oracle/js/projector.js';
-import { designDOE as oDesign, gerchbergSaxtonMultiPlane as oMultiPlane, blurRaster as oBlur } from '../../oracle/js/doe.js';
+import { designDOE as oDesign, gerchbergSaxtonMultiPlane as oMultiPlane, blurRaster as oBlur, errorDiffusionQuantize as oFS, phaseToHeight as oHeight } from '../../oracle/js/doe.js';
 import { makeTarget as oMakeTarget, autoRoi as oAutoRoi, resampleRaster as oResample } from '../../oracle/js/projector.js';
 import { planeToRGB as oRGB, planeToGray as oGray } from '../../oracle/js/color.js';
 import zlib from 'zlib';
@@ -330,6 +330,29 @@ function maxRel(a, b) {
   const osys = new OProjector({ n: 64, target: { kind: 'dots', cols: 4, rows: 4 } });
   check('ProjectorSystem target', maxRel(sys.target.irradiance, osys.target.irradiance) < 1e-14 &&
     same(sys.target.roi, osys.target.roi));
+}
+
+// fabrication output (ref: doe.js:185-226, 967-976)
+{
+  const ph = lcg(128 * 128, 51).map((v) => v * 12);
+  for (const snake of [false, true]) {
+    for (const wrapFirst of [true, false]) {
+      const g = gpu.errorDiffusionQuantize(ph, 4, { snake, wrapFirst });
+      const o = oFS(ph, 4, { snake, wrapFirst });
+      check(`errorDiffusionQuantize snake=${snake} wrapFirst=${wrapFirst}`, same(g, o));
+    }
+  }
+  check('phaseToHeight', same(gpu.phaseToHeight(ph, 633e-9, 1.5), oHeight(ph, 633e-9, 1.5)));
+  // a 3 x 4 'square' grid by the reference's rule (round(sqrt(12)) = 3 divides 12)
+  const p12 = lcg(12, 3).map((v) => v * 9);
+  check('errorDiffusionQuantize 3x4', same(gpu.errorDiffusionQuantize(p12, 3, { snake: false }), oFS(p12, 3, { snake: false })));
+  let threw = false;
+  try {
+    gpu.errorDiffusionQuantize(new Float64Array(10), 4, { snake: false });  // 10 / round(sqrt(10)) is not a row count
+  } catch (err) {
+    threw = /expected a square grid/.test(err.message);
+  }
+  check('errorDiffusionQuantize non-square throws', threw);
 }
 
 // figure pixels and PNG output (ref: color.js:87-160, png.js:89-180)

@@ -1,0 +1,109 @@
+"""Fabrication output on the device against the oracle: error diffusion
+(both scan orders, one lane per image for serpentine rows, a wavefront for
+plain rows) and the etch-depth map (ref: src/doe.js:185-226, 967-976).
+Gate: equal bit for bit (every step is exact IEEE arithmetic in the
+reference's order)."""
+import time
+
+import numpy as np
+import pytest
+
+from diffractiveopticalelement_amd import native as nat
+import oracle_js as oj
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not oj.node_available(), reason="node (the oracle's runtime) missing")]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = nat.Context(0)
+    yield c
+    c.close()
+
+
+def device_fs(ctx, phases, n, levels, snake, wrap_first=True):
+    f = ctx.upload(np.stack([p.reshape(n, n) for p in phases]), None, "f64")
+    o = ctx.field(n, n, len(phases), "f64")
+    ctx.error_diffusion(f, o, levels, snake, wrap_first)
+    re, _ = o.download()
+    f.free()
+    o.free()
+    return re.reshape(len(phases), -1)
+
+
+@pytest.mark.parametrize("n", [16, 64, 256])
+@pytest.mark.parametrize("snake", [True, False])
+@pytest.mark.parametrize("levels", [2, 4, 3])
+def test_error_diffusion_bit_exact(ctx, n, snake, levels):
+    rng = np.random.default_rng(n * 10 + levels)
+    phi = rng.uniform(-6, 6, n * n)
+    print("error diffusion", n, snake, levels, flush=True)
+    out, _ = oj.run("error_diffusion", {"levels": levels, "opts": {"snake": snake}}, {"phase": phi})
+    got = device_fs(ctx, [phi], n, levels, snake)[0]
+    np.testing.assert_array_equal(got, out["out"])
+
+
+@pytest.mark.parametrize("snake", [True, False])
+def test_error_diffusion_batch_and_no_wrap(ctx, snake):
+    rng = np.random.default_rng(3)
+    n = 48
+    phis = [rng.uniform(-3, 3, n * n) for _ in range(5)]
+    got = device_fs(ctx, phis, n, 8, snake, wrap_first=False)
+    for q, phi in enumerate(phis):
+        out, _ = oj.run("error_diffusion", {"levels": 8, "opts": {"snake": snake, "wrapFirst": False}},
+                        {"phase": phi})
+        np.testing.assert_array_equal(got[q], out["out"])
+
+
+def test_error_diffusion_smooth_phase_and_timing(ctx):
+    n = 1024
+    j, i = np.mgrid[0:n, 0:n].astype(float)
+    phi = (np.sin(i / 37) * 3 + np.cos(j / 23) * 2).ravel()
+    for snake in (False, True):
+        f = ctx.upload(phi.reshape(1, n, n), None, "f64")
+        o = ctx.field(n, n, 1, "f64")
+        ctx.error_diffusion(f, o, 4, snake)  # warm
+        t0 = time.perf_counter()
+        ctx.error_diffusion(f, o, 4, snake)
+        ms = (time.perf_counter() - t0) * 1e3
+        re, _ = o.download()
+        print(f"errorDiffusionQuantize 1024^2 snake={snake}: {ms:.2f} ms", flush=True)
+        st = 2 * np.pi / 4
+        m = np.mod(re.ravel() + np.pi, st)
+        assert np.all(np.minimum(m, st - m) < 1e-9)
+        f.free()
+        o.free()
+    out, _ = oj.run("error_diffusion", {"levels": 4, "opts": {"snake": False}}, {"phase": phi})
+    f = ctx.upload(phi.reshape(1, n, n), None, "f64")
+    o = ctx.field(n, n, 1, "f64")
+    ctx.error_diffusion(f, o, 4, False)
+    re, _ = o.download()
+    np.testing.assert_array_equal(re.ravel(), out["out"])
+
+
+def test_phase_to_height_bit_exact(ctx):
+    rng = np.random.default_rng(8)
+    phase = rng.uniform(-50, 50, (3, 40, 24))
+    for lam, nidx in ((633e-9, 1.52), (532e-9, 1.46)):
+        out, _ = oj.run("phase_to_height", {"lambda": lam, "n": nidx}, {"phase": phase})
+        f = ctx.upload(phase, None, "f64")
+        o = ctx.field(24, 40, 3, "f64")
+        ctx.phase_to_height(f, o, lam, nidx)
+        re, _ = o.download()
+        np.testing.assert_array_equal(re.ravel(), out["out"])
+        f.free()
+        o.free()
+
+
+def test_argument_errors(ctx):
+    f = ctx.field(8, 8, 1, "f64")
+    g = ctx.field(8, 8, 1, "f32")
+    with pytest.raises(nat.DoeError, match="FP64"):
+        ctx.error_diffusion(f, g)
+    with pytest.raises(nat.DoeError, match="overlaps"):
+        ctx.error_diffusion(f, f)
+    with pytest.raises(nat.DoeError, match="non-zero"):
+        ctx.error_diffusion(f, ctx.field(8, 8, 1, "f64"), levels=0)
+    f.free()
+    g.free()
