@@ -129,6 +129,23 @@ __global__ void k_ws_pred_field(const float* __restrict__ z, const float* __rest
   im[i] = cb * t[i] * pi;
 }
 
+// In place on the forward far field F of the best element field: Y = t .*
+// phasor(F) at full resolution, times the checkerboard c (the input of the
+// inverse transform) — one GS projection of the best iterate.
+__global__ void k_ws_restart(float* __restrict__ re, float* __restrict__ im, const float* __restrict__ t,
+                             int n, long total, float g) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long r = i % ((long)n * n);
+  const int y = int(r / n), x = int(r - (long)y * n);
+  const float cb = ((x + y) & 1) ? -1.f : 1.f;
+  const float zr = cb * g * re[i], zi = cb * g * im[i];
+  const float m = sqrtf(zr * zr + zi * zi);
+  const float pr = m > 0 ? zr / m : 1.f, pi = m > 0 ? zi / m : 0.f;
+  re[i] = cb * t[i] * pr;
+  im[i] = cb * t[i] * pi;
+}
+
 // phi0 = arg(c .* F^-1(c .* Y)) (the global sign g is a global phase)
 __global__ void k_ws_phase(const float* __restrict__ re, const float* __restrict__ im,
                            float* __restrict__ phi, int n, long total) {
@@ -292,6 +309,35 @@ int doe_warmstart_apply_phasor(doe_gs_batch* B, int res, const float* z) {
   DOE_HIP(hipSetDevice(B->ctx->device));
   rc = ws_apply_phasor(B, res, z);
   if (rc) return rc;
+  DOE_HIP(finish(B->ctx));
+  return DOE_OK;
+}
+
+// Restart from the best iterate: every design's initial phase becomes
+// arg(F^-1(t * phasor(F(best element field)))); the next doe_gs_batch_run
+// starts from it with fresh weights.
+int doe_gs_batch_restart_best(doe_gs_batch* B) {
+  DOE_REQUIRE(B, DOE_ERR_INVALID, "null argument");
+  DOE_HIP(hipSetDevice(B->ctx->device));
+  hipStream_t st = B->ctx->stream;
+  for (const auto& c : B->cls) {
+    const int nd = int(c.users.size());
+    const long total = long(c.n) * c.n * nd;
+    hipLaunchKernelGGL(k_ws_element, dim3(blk(total)), dim3(256), 0, st, B->illum + c.off,
+                       B->ph[0] + c.off, B->re + c.off, B->im + c.off, c.n, total);
+    DOE_HIP(hipGetLastError());
+    int rc = fft_class(B, c, 0);
+    if (rc) return rc;
+    const float g = (((c.n >> 1) + (c.n >> 1)) & 1) ? -1.f : 1.f;
+    hipLaunchKernelGGL(k_ws_restart, dim3(blk(total)), dim3(256), 0, st, B->re + c.off, B->im + c.off,
+                       B->tgt + c.off, c.n, total, g);
+    DOE_HIP(hipGetLastError());
+    rc = fft_class(B, c, 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ws_phase, dim3(blk(total)), dim3(256), 0, st, B->re + c.off, B->im + c.off,
+                       B->phi0 + c.off, c.n, total);
+  }
+  DOE_HIP(hipGetLastError());
   DOE_HIP(finish(B->ctx));
   return DOE_OK;
 }

@@ -163,6 +163,7 @@ SIGNATURES = [
     ("doe_warmstart_labels", C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
     ("doe_warmstart_apply_phasor", C.c_int, [_VP, C.c_int, _VP]),
+    ("doe_gs_batch_restart_best", C.c_int, [_VP]),
     ("doe_screen_metrics", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts), C.POINTER(Metrics)]),
     ("doe_screen_metrics_batch", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
                                            C.POINTER(Metrics)]),
@@ -579,6 +580,10 @@ class GsBatch:
 
     def warm_apply_phasor(self, res: int, phasor_ptr: int) -> None:
         check(self.ctx.lib.doe_warmstart_apply_phasor(self.h, res, phasor_ptr))
+
+    def restart_best(self) -> None:
+        """next run starts from one GS projection of each design's best iterate"""
+        check(self.ctx.lib.doe_gs_batch_restart_best(self.h))
 
     def get_screen(self, index: int):
         out = np.zeros(self.iterations)

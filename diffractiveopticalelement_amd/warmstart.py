@@ -236,6 +236,9 @@ def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0
                and handed off like the network's output: what a network that
                reproduced its training labels exactly at resolution r would
                give (an upper bound for the learned start at that resolution)
+    restart@k  plain for k iterations, then a restart from each design's best
+               iterate (doe_gs_batch_restart_best) for the remaining
+               iterations; its history and GPU time cover both stages
 
     Per design: the screen rmse history, the first iteration at or below
     `thr`, the best rmse, and for every mode but plain the iterations to the
@@ -253,6 +256,8 @@ def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0
             b.random_init(seed)
         elif mode == "quadratic":
             start_quadratic(ctx, b, count, n, net.res)
+        elif mode.startswith("restart@"):
+            pass  # the stages run below
         elif mode == "warm":
             scratch = ctx.field(net.res, net.res, 4 * count, "f32")  # >= 3 count res^2 floats
             ctx.sync()
@@ -265,10 +270,24 @@ def evaluate(ctx, net, count: int, n: int, iters: int, seed: int, thr: float = 0
             zf = _phasor_field(ctx, labels[r])
             b.warm_apply_phasor(r, zf.ptrs()[0])
             zf.free()
-        ctx.timer_start()
-        b.run(iters, graph=True)
-        ms = ctx.timer_stop()
-        scr = np.array([b.get_screen(i) for i in range(count)])
+        if mode.startswith("restart@"):
+            k = int(mode.split("@")[1])
+            stages = [k, iters - k]
+            b.random_init(seed)
+            ms, parts = 0.0, []
+            for q, m in enumerate(stages):
+                ctx.timer_start()
+                if q:
+                    b.restart_best()
+                b.run(m, graph=True)
+                ms += ctx.timer_stop()
+                parts.append(np.array([b.get_screen(i) for i in range(count)]))  # the next run overwrites them
+            scr = np.concatenate(parts, axis=1)
+        else:
+            ctx.timer_start()
+            b.run(iters, graph=True)
+            ms = ctx.timer_stop()
+            scr = np.array([b.get_screen(i) for i in range(count)])
         if mode == "plain":
             scr_plain = scr
             for r in label_res:

@@ -286,6 +286,13 @@ int doe_warmstart_apply(doe_gs_batch* batch, doe_wsnet* net, float* scratch);
  * floats interleaved re, im, schedule order) instead of a network, e.g. a
  * quadratic target-plane phase. */
 int doe_warmstart_apply_phasor(doe_gs_batch* batch, int res, const float* phasor);
+/* Restart from the best iterate (no counterpart in the reference): every
+ * design's initial phase becomes arg(F^-1(t * phasor(F(illumination *
+ * e^{i best phase})))) — one GS projection of its best iterate — so the next
+ * doe_gs_batch_run continues from it with fresh weights.  WGS on extended
+ * targets lets its weights run away after its best early iterate; restarting
+ * there converges where the uninterrupted run stalls. */
+int doe_gs_batch_restart_best(doe_gs_batch* batch);
 
 /* Projection-quality metrics of a screen field on the device, in FP64.
  * Replaces ProjectorSystem#screenMetrics (projector.js:554-614) and the

@@ -43,3 +43,43 @@ def test_trained_start_against_plain_and_label_bounds():
     assert np.median(full_its) < 0.75 * np.median(plain_its), (plain_its, full_its)
     net.close()
     ctx.close()
+
+
+def test_restart_from_best_equals_the_full_resolution_handoff():
+    """doe_gs_batch_restart_best = the labels@n hand-off (each design's best
+    target-plane phasor at full resolution, applied through
+    doe_warmstart_apply_phasor), done on the device in one call."""
+    ctx = nat.Context(0)
+    n, count, seed = 256, 8, 77
+    a, kinds = ws.fill_batch(ctx, count, n, seed)
+    a.random_init(seed)
+    a.run(25)
+    lab = ctx.field(n, n, count, "f32")
+    wts = ctx.field(n, n, count, "f32")
+    a.warm_labels(n, lab.ptrs()[0], lab.ptrs()[1], wts.ptrs()[0])
+    lr_, li_ = lab.download()
+    z = np.stack([lr_, li_], axis=-1).astype(np.float32)
+    lab.free()
+    wts.free()
+    a.restart_best()
+    a.run(5)
+    sa = np.array([a.get_screen(i) for i in range(count)])
+    b, _ = ws.fill_batch(ctx, count, n, seed)
+    b.random_init(seed)
+    b.run(25)
+    zf = ws._phasor_field(ctx, z)
+    b.warm_apply_phasor(n, zf.ptrs()[0])
+    zf.free()
+    b.run(5)
+    sb = np.array([b.get_screen(i) for i in range(count)])
+    assert np.all(np.isfinite(sa))
+    np.testing.assert_allclose(sa, sb, rtol=1e-3, atol=1e-5)
+    a.close()
+    b.close()
+    # the evaluation's restart mode: both stages in the history and the time
+    out = ws.evaluate(ctx, type("N", (), {"res": 64})(), count, n, 60, seed=seed, thr=0.01, log=print,
+                      modes=("plain", "restart@25"))
+    r = out["restart@25"]
+    assert len(r["best_screen_rmse"]) == count and r["ms_per_design_iteration"] > 0
+    assert np.all(np.isfinite(r["best_screen_rmse"]))
+    ctx.close()
