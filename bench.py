@@ -257,7 +257,8 @@ TGT_KINDS = [("raster", None), ("disc", {}), ("ring", {"inner": 0.15, "outer": 0
 
 
 def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int = 2048) -> dict:
-    """Device makeTarget + autoRoi over a mixed batch, and planeToRGB of a field.
+    """Device makeTarget + autoRoi (one doe_target_make_roi call) over a mixed
+    batch, and planeToRGB of a field.
 
     Algorithmic bytes: a target's irradiance written (8 B/pt), read back for the
     ROI and the ROI written (16 B/pt): 24 B/pt.  An image reads the FP32 field
@@ -276,14 +277,12 @@ def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int 
             specs.append(ctx.target_spec(kind, **opts))
     out = ctx.field(n, n, count, "f64")
     ctx.set_flags(asynchronous=True, profile=False)
-    ctx.target_make(specs, n, n, out)
-    ctx.target_auto_roi(out)
+    ctx.target_make_roi(specs, n, n, out)
     ctx.sync()
     ctx.set_flags(asynchronous=True, profile=True)
     ctx.timer_start()
     for _ in range(reps):
-        ctx.target_make(specs, n, n, out)
-        ctx.target_auto_roi(out)
+        ctx.target_make_roi(specs, n, n, out)
     call_ms = ctx.timer_stop() / reps
     prof = ctx.profile_report()
     ctx.set_flags(asynchronous=True, profile=False)

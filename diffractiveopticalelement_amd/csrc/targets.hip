@@ -127,8 +127,8 @@ struct TgtRec {
   long roff;         // raster offset (doubles) in the raster block
   long moff;         // dark-cell table offset (bytes) in the table block
   double p0, p1, p2; // kind parameters (radii, pitches and sigma, cells, period and duty)
-  double s, ox, oy;  // raster placement: scale and offsets
-  double gamma, background, gain;
+  double s, ox, oy;  // raster placement: scale and offsets (disc, ring: squared-radius bounds)
+  double gamma, background, gain;  // (ring: squared-radius bounds in gamma, background)
 };
 
 // Irradiance of one sample of a target of kind K (projector.js:169-255).
@@ -136,10 +136,19 @@ template <int K>
 __device__ __forceinline__ double target_at(const TgtRec& R, const double* __restrict__ rasters,
                                             const unsigned char* __restrict__ dark, int i, int j,
                                             int nx, int ny, double cx, double cy) {
+  // Disc, ring, zone: a = i - cx and b = j - cy are multiples of 1/2, so
+  // d2 = a^2 + b^2 is exact; away from a boundary it decides the sample,
+  // near one (relative 1e-9) the engine's hypot does, as in the reference.
   if constexpr (K == DOE_TGT_DISC) {
-    return js_hypot(i - cx, j - cy) <= R.p0 ? 1 : 0;
+    const double a = i - cx, b = j - cy, d2 = a * a + b * b;
+    if (d2 < R.s) return 1;
+    if (d2 > R.ox) return 0;
+    return js_hypot(a, b) <= R.p0 ? 1 : 0;
   } else if constexpr (K == DOE_TGT_RING) {
-    const double r = js_hypot(i - cx, j - cy);
+    const double a = i - cx, b = j - cy, d2 = a * a + b * b;
+    if (d2 > R.oy && d2 < R.gamma) return 1;  // clearly between r0 and r1
+    if (d2 < R.s || d2 > R.background) return 0;  // clearly inside r0 or outside r1
+    const double r = js_hypot(a, b);
     return (r >= R.p0 && r <= R.p1) ? 1 : 0;
   } else if constexpr (K == DOE_TGT_DOTS) {
     const double px = R.p0, py = R.p1, sg = R.p2;
@@ -162,7 +171,15 @@ __device__ __forceinline__ double target_at(const TgtRec& R, const double* __res
   } else if constexpr (K == DOE_TGT_LINES) {
     return (fmod((double)i, R.p0) / R.p0) < R.p1 ? 1 : 0;
   } else if constexpr (K == DOE_TGT_ZONE) {
-    const double r = js_hypot(i - cx, j - cy);
+    // sin(pi t) > 0 with t = r^2 / period iff floor(t) is even; decided from
+    // the exact d2 unless t is within 1e-6 of an integer
+    const double a = i - cx, b = j - cy, d2 = a * a + b * b;
+    const double t = d2 / R.p0;
+    if (fabs(t) < 1e7) {
+      const double fl = floor(t), fr = t - fl;
+      if (fr > 1e-6 && fr < 1 - 1e-6) return (fl * 0.5 == floor(fl * 0.5)) ? 1 : 0;
+    }
+    const double r = js_hypot(a, b);
     return sin((M_PI * r * r) / R.p0) > 0 ? 1 : 0;
   } else {
     double v = R.background;
@@ -190,15 +207,17 @@ __device__ __forceinline__ double target_at(const TgtRec& R, const double* __res
 }
 
 // Every sample of field b, walked by the block's threads; four samples per
-// thread per step, so their independent FP64 chains interleave.
+// thread per step, so their independent FP64 chains interleave.  Returns the
+// thread's maximum over v > m from +0 (NaN never selected), for the ROI.
 template <int K>
-__device__ __forceinline__ void target_field(const TgtRec& R, const double* __restrict__ rasters,
-                                             const unsigned char* __restrict__ dark, double* __restrict__ o,
-                                             int nx, int ny) {
+__device__ __forceinline__ double target_field(const TgtRec& R, const double* __restrict__ rasters,
+                                               const unsigned char* __restrict__ dark, double* __restrict__ o,
+                                               int nx, int ny) {
   const long n = (long)nx * ny;
   const double cx = (nx - 1) / 2.0, cy = (ny - 1) / 2.0;
   const long step = (long)gridDim.x * kTB;
   long k = (long)blockIdx.x * kTB + threadIdx.x;
+  double m = 0;
   for (; k + 3 * step < n; k += 4 * step) {
     double v[4];
 #pragma unroll
@@ -208,37 +227,56 @@ __device__ __forceinline__ void target_field(const TgtRec& R, const double* __re
       v[u] = target_at<K>(R, rasters, dark, i, j, nx, ny, cx, cy);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) o[k + u * step] = v[u];
+    for (int u = 0; u < 4; ++u) {
+      o[k + u * step] = v[u];
+      if (v[u] > m) m = v[u];
+    }
   }
   for (; k < n; k += step) {
     int i, j;
     split_ij(k, nx, i, j);
-    o[k] = target_at<K>(R, rasters, dark, i, j, nx, ny, cx, cy);
+    const double v = target_at<K>(R, rasters, dark, i, j, nx, ny, cx, cy);
+    o[k] = v;
+    if (v > m) m = v;
   }
+  return m;
 }
 
+// part (nullable): per-block maxima of the irradiance, part[b * gridDim.x +
+// block], the first half of autoRoi's maximum.
 __global__ __launch_bounds__(kTB) void k_target(const TgtRec* __restrict__ recs,
                                                 const double* __restrict__ rasters,
                                                 const unsigned char* __restrict__ dark,
-                                                double* __restrict__ out, int nx, int ny) {
+                                                double* __restrict__ out, int nx, int ny,
+                                                double* __restrict__ part) {
   const int b = blockIdx.y;
   const TgtRec R = recs[b];
   double* o = out + (long)b * nx * ny;
+  double m;
   switch (R.kind) {
-    case DOE_TGT_DISC: target_field<DOE_TGT_DISC>(R, rasters, dark, o, nx, ny); break;
-    case DOE_TGT_RING: target_field<DOE_TGT_RING>(R, rasters, dark, o, nx, ny); break;
-    case DOE_TGT_DOTS: target_field<DOE_TGT_DOTS>(R, rasters, dark, o, nx, ny); break;
-    case DOE_TGT_CHECKER: target_field<DOE_TGT_CHECKER>(R, rasters, dark, o, nx, ny); break;
-    case DOE_TGT_LINES: target_field<DOE_TGT_LINES>(R, rasters, dark, o, nx, ny); break;
-    case DOE_TGT_ZONE: target_field<DOE_TGT_ZONE>(R, rasters, dark, o, nx, ny); break;
-    default: target_field<DOE_TGT_RASTER>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_DISC: m = target_field<DOE_TGT_DISC>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_RING: m = target_field<DOE_TGT_RING>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_DOTS: m = target_field<DOE_TGT_DOTS>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_CHECKER: m = target_field<DOE_TGT_CHECKER>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_LINES: m = target_field<DOE_TGT_LINES>(R, rasters, dark, o, nx, ny); break;
+    case DOE_TGT_ZONE: m = target_field<DOE_TGT_ZONE>(R, rasters, dark, o, nx, ny); break;
+    default: m = target_field<DOE_TGT_RASTER>(R, rasters, dark, o, nx, ny); break;
+  }
+  if (!part) return;
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  __shared__ double red[kTB / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kTB / 64; ++w) m = fmax(m, red[w]);
+    part[(long)b * gridDim.x + blockIdx.x] = m;
   }
 }
 
 // ---- autoRoi ---------------------------------------------------------------
 
 // Per-block maxima of irr over irr > m (NaN never selected, start at +0),
-// part[b * gridDim.x + block]; four independent loads in flight per thread.
+// part[b * gridDim.x + block]; eight independent loads in flight per thread.
 __global__ __launch_bounds__(kTB) void k_roi_max(const double* __restrict__ irr, long n,
                                                  double* __restrict__ part) {
   const int b = blockIdx.y;
@@ -246,12 +284,13 @@ __global__ __launch_bounds__(kTB) void k_roi_max(const double* __restrict__ irr,
   double m = 0;
   const long step = (long)gridDim.x * kTB;
   long k = (long)blockIdx.x * kTB + threadIdx.x;
-  for (; k + 3 * step < n; k += 4 * step) {
-    const double v0 = p[k], v1 = p[k + step], v2 = p[k + 2 * step], v3 = p[k + 3 * step];
-    if (v0 > m) m = v0;
-    if (v1 > m) m = v1;
-    if (v2 > m) m = v2;
-    if (v3 > m) m = v3;
+  for (; k + 7 * step < n; k += 8 * step) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[k + u * step];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (v[u] > m) m = v[u];
   }
   for (; k < n; k += step) {
     const double v = p[k];
@@ -282,70 +321,76 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
   if (k < n) p[k] = v;
 }
 
-// The whole ROI of a 64 x 32 output tile in LDS: threshold over the tile
-// plus a halo of h = r + s, row dilation, column dilation (zero outside the
-// plane), row box sums, column box sums / clipped window count.  Equal to
-// the separable passes below, for r + s <= kRoiHalo.
+// The whole ROI of a 64 x 32 output tile, bit-packed.  The tile plus a halo
+// of h = r + s is thresholded into 128-bit row masks (two wave ballots per
+// row); the square dilation is an OR of 2r+1 row masks followed by 2r+1
+// shifted copies (the max filter is separable, the order is free), cleared
+// outside the plane; the box sum along x is a popcount of a (2s+1)-bit
+// window; the sum along y a running window down each column; interior
+// samples divide through a table of acc / (2s+1)^2 (the same correctly
+// rounded quotients).  Integer-exact throughout, so equal to the separable
+// passes below, for r + s <= kRoiHalo.
 constexpr int kRoiW = 64, kRoiH = 32, kRoiHalo = 16;
-constexpr int kRoiBuf = (kRoiW + 2 * kRoiHalo) * (kRoiH + 2 * kRoiHalo);
+constexpr int kRoiRows = kRoiH + 2 * kRoiHalo;  // 64: one lane per halo'd row
 
-// OR / sum of len <= 2 * WMAX + 1 LDS entries at stride `st`.  With WMAX > 0
-// the reads are unrolled (predicated), so they issue together instead of as
-// a chain of dependent waits; WMAX = 0 loops over any length.
-template <int WMAX>
-__device__ __forceinline__ unsigned char or_window(const unsigned char* p, int st, int len) {
-  unsigned char v = 0;
-  if constexpr (WMAX > 0) {
-#pragma unroll
-    for (int d = 0; d < 2 * WMAX + 1; ++d)
-      if (d < len) v |= p[d * st];
-  } else {
-    for (int d = 0; d < len; ++d) v |= p[d * st];
-  }
-  return v;
+struct Bits128 {
+  unsigned long long lo, hi;
+};
+__device__ __forceinline__ Bits128 shl128(Bits128 v, int d) {  // toward higher bit indices
+  if (d == 0) return v;
+  return {v.lo << d, (v.hi << d) | (v.lo >> (64 - d))};
 }
-template <int WMAX, class T>
-__device__ __forceinline__ int sum_window(const T* p, int st, int len) {
-  int v = 0;
-  if constexpr (WMAX > 0) {
-#pragma unroll
-    for (int d = 0; d < 2 * WMAX + 1; ++d)
-      if (d < len) v += p[d * st];
-  } else {
-    for (int d = 0; d < len; ++d) v += p[d * st];
-  }
-  return v;
+__device__ __forceinline__ Bits128 shr128(Bits128 v, int d) {
+  if (d == 0) return v;
+  return {(v.lo >> d) | (v.hi << (64 - d)), v.hi >> d};
+}
+// bits [p, p + w) of v, w <= 64, p + w <= 128
+__device__ __forceinline__ unsigned long long window128(Bits128 v, int p, int w) {
+  const unsigned long long m = w >= 64 ? ~0ull : ((1ull << w) - 1);
+  if (p >= 64) return (v.hi >> (p - 64)) & m;
+  const unsigned long long x = p ? ((v.lo >> p) | (v.hi << (64 - p))) : v.lo;
+  return x & m;
+}
+// mask of bits [a, b) of a 128-bit row, 0 <= a <= b <= 128
+__device__ __forceinline__ Bits128 range128(int a, int b) {
+  auto upto = [](int k) -> Bits128 {  // bits [0, k)
+    if (k <= 0) return {0, 0};
+    if (k < 64) return {(1ull << k) - 1, 0};
+    if (k == 64) return {~0ull, 0};
+    if (k < 128) return {~0ull, (1ull << (k - 64)) - 1};
+    return {~0ull, ~0ull};
+  };
+  const Bits128 hb = upto(b), ha = upto(a);
+  return {hb.lo & ~ha.lo, hb.hi & ~ha.hi};
 }
 
-template <int WMAX>
 __global__ __launch_bounds__(kTB) void k_roi_tile(const double* __restrict__ irr,
                                                   const double* __restrict__ thrs,
                                                   double* __restrict__ roi, int nx, int ny, int r,
                                                   int s) {
-  __shared__ unsigned char bin[kRoiBuf];
-  __shared__ unsigned char tmp[kRoiBuf];
-  __shared__ unsigned char dil[kRoiBuf];
-  __shared__ unsigned short rs[kRoiW * (kRoiH + 2 * kRoiHalo)];
+  __shared__ Bits128 binw[kRoiRows];
+  __shared__ Bits128 dilw[kRoiRows];
+  __shared__ unsigned char rs[kRoiRows][kRoiW];
+  __shared__ double quot[(2 * kRoiHalo + 1) * (2 * kRoiHalo + 1) + 1];
   const int b = blockIdx.z;
   const long n = (long)nx * ny;
   const double* p = irr + (long)b * n;
   const double thr = thrs[b];
   const int x0 = blockIdx.x * kRoiW, y0 = blockIdx.y * kRoiH;
   const int h = r + s;
-  const int BW = kRoiW + 2 * h, BH = kRoiH + 2 * h;
-  const int TW = kRoiW + 2 * s, TH = BH;  // tmp: x from x0 - s, y from y0 - h
-  const int DW = TW, DH = kRoiH + 2 * s;  // dil: x from x0 - s, y from y0 - s
-  // 64 lanes along x, the block's 4 waves along y: no index division
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  // every load of the halo'd tile in flight at once (registers), then the
-  // thresholds to LDS; NaN stands for samples outside the plane
-  constexpr int kRows = (kRoiH + 2 * kRoiHalo) / (kTB / 64);
-  double va[kRows], vb[kRows];
+  const int BW = kRoiW + 2 * h, BH = kRoiH + 2 * h, DH = kRoiH + 2 * s;
+  const int tx = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int win = 2 * s + 1, full = win * win;
+  // exact interior quotients acc / (2s+1)^2
+  for (int q = threadIdx.x; q <= full; q += kTB) quot[q] = (double)q / (double)full;
+  // thresholds: all loads of the wave's rows in flight, then two ballots a row
+  constexpr int kPer = kRoiRows / (kTB / 64);
+  double va[kPer], vb[kPer];
   const int Xa = x0 - h + tx, Xb = Xa + 64;
   const bool ina = Xa >= 0 && Xa < nx, inb = tx + 64 < BW && Xb >= 0 && Xb < nx;
 #pragma unroll
-  for (int it = 0; it < kRows; ++it) {
-    const int yy = ty + (kTB / 64) * it;
+  for (int it = 0; it < kPer; ++it) {
+    const int yy = w + (kTB / 64) * it;
     const int Y = y0 - h + yy;
     const bool rowin = yy < BH && Y >= 0 && Y < ny;
     const double* prow = p + (long)Y * nx;
@@ -353,49 +398,58 @@ __global__ __launch_bounds__(kTB) void k_roi_tile(const double* __restrict__ irr
     vb[it] = (rowin && inb) ? prow[Xb] : NAN;
   }
 #pragma unroll
-  for (int it = 0; it < kRows; ++it) {
-    const int yy = ty + (kTB / 64) * it;
-    if (yy < BH) {
-      bin[yy * BW + tx] = va[it] > thr;
-      if (tx + 64 < BW) bin[yy * BW + tx + 64] = vb[it] > thr;
-    }
+  for (int it = 0; it < kPer; ++it) {
+    const int yy = w + (kTB / 64) * it;
+    const unsigned long long lo = __ballot(va[it] > thr), hi = __ballot(vb[it] > thr);
+    if (tx == 0 && yy < BH) binw[yy] = {lo, hi};
   }
   __syncthreads();
-  for (int yy = ty; yy < TH; yy += kTB / 64) {
-    for (int xx = tx; xx < TW; xx += 64) {
-      tmp[yy * TW + xx] = or_window<WMAX>(bin + yy * BW + xx, 1, 2 * r + 1);
+  // dilation, one lane per row of the dilated band (rows y0 - s ..)
+  if (w == 0 && tx < DH) {
+    const int Y = y0 - s + tx;
+    Bits128 v = {0, 0};
+    if (Y >= 0 && Y < ny) {
+      for (int d = 0; d <= 2 * r; ++d) {  // bin rows tx .. tx + 2r
+        const Bits128 u = binw[tx + d];
+        v.lo |= u.lo;
+        v.hi |= u.hi;
+      }
+      Bits128 dd = v;
+      for (int d = 1; d <= r; ++d) {
+        const Bits128 a = shl128(v, d), c = shr128(v, d);
+        dd.lo |= a.lo | c.lo;
+        dd.hi |= a.hi | c.hi;
+      }
+      // bits xx with x0 - h + xx inside the plane
+      const Bits128 m = range128(max(0, h - x0), min(BW, nx - x0 + h));
+      v = {dd.lo & m.lo, dd.hi & m.hi};
     }
-  }
-  __syncthreads();
-  for (int yy = ty; yy < DH; yy += kTB / 64) {
-    const int Y = y0 - s + yy;
-    for (int xx = tx; xx < DW; xx += 64) {
-      const int X = x0 - s + xx;
-      const bool in = X >= 0 && X < nx && Y >= 0 && Y < ny;
-      dil[yy * DW + xx] = in ? or_window<WMAX>(tmp + yy * TW + xx, TW, 2 * r + 1) : 0;
-    }
+    dilw[tx] = v;
   }
   __syncthreads();
   const int X = x0 + tx;
   if (s == 0) {
-    for (int yy = ty; yy < kRoiH; yy += kTB / 64) {
-      const int Y = y0 + yy;
-      if (X < nx && Y < ny) roi[(long)b * n + (long)Y * nx + X] = dil[yy * DW + tx];
-    }
+    if (X < nx)
+      for (int yy = w; yy < kRoiH && y0 + yy < ny; yy += kTB / 64)
+        roi[(long)b * n + (long)(y0 + yy) * nx + X] = (double)window128(dilw[yy], tx + h, 1);
     return;
   }
-  for (int yy = ty; yy < DH; yy += kTB / 64) {
-    rs[yy * kRoiW + tx] = (unsigned short)sum_window<WMAX>(dil + yy * DW + tx, 1, 2 * s + 1);
-  }
+  // box sums along x: popcount of bits [tx + r, tx + r + 2s] of each dilated row
+  for (int yy = w; yy < DH; yy += kTB / 64) rs[yy][tx] = (unsigned char)__popcll(window128(dilw[yy], tx + r, win));
   __syncthreads();
   if (X >= nx) return;
+  // sums along y: a running window over the wave's 8 output rows
   const int cx = min(nx - 1, X + s) - max(0, X - s) + 1;
-  for (int yy = ty; yy < kRoiH; yy += kTB / 64) {
+  const int ya = w * (kRoiH / (kTB / 64));
+  int acc = 0;
+  for (int d = 0; d < win; ++d) acc += rs[ya + d][tx];
+  for (int yy = ya; yy < ya + kRoiH / (kTB / 64); ++yy) {
     const int Y = y0 + yy;
     if (Y >= ny) break;
-    const int acc = sum_window<WMAX>(rs + yy * kRoiW + tx, kRoiW, 2 * s + 1);
+    if (yy > ya) acc += rs[yy + 2 * s][tx] - rs[yy - 1][tx];
     const int cy = min(ny - 1, Y + s) - max(0, Y - s) + 1;
-    roi[(long)b * n + (long)Y * nx + X] = (double)acc / (double)(cx * cy);
+    roi[(long)b * n + (long)Y * nx + X] =
+        (cx == win && cy == win) ? quot[acc] : (double)acc / (double)(cx * cy);
   }
 }
 
@@ -731,7 +785,14 @@ int doe_target_spec_init(doe_target_spec* s, int kind) {
   return DOE_OK;
 }
 
-int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) {
+}  // extern "C"
+
+namespace doe {
+namespace {
+
+// makeTarget for every field of out; with `maxima`, the kernel also leaves
+// per-block irradiance maxima in ctx->partials ([b * per + block], *per set).
+int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsigned* maxima) {
   DOE_REQUIRE(ctx && specs && out, DOE_ERR_INVALID, "null argument");
   DOE_REQUIRE(out->dtype == DOE_F64, DOE_ERR_INVALID, "makeTarget: the target field is FP64");
   DOE_REQUIRE(out->nx >= 1 && out->ny >= 1 && out->batch >= 1, DOE_ERR_SHAPE, "makeTarget: empty grid");
@@ -746,11 +807,25 @@ int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) 
     std::memset(&R, 0, sizeof(R));
     R.kind = S.kind;
     switch (S.kind) {
-      case DOE_TGT_DISC: R.p0 = S.radius * mn * 0.5; break;
-      case DOE_TGT_RING:
+      case DOE_TGT_DISC:
+        R.p0 = S.radius * mn * 0.5;
+        // squared radius with a relative margin: below s inside, above ox outside
+        // (NaN bounds send every sample to the exact test)
+        R.s = R.p0 >= 0 ? R.p0 * R.p0 * (1 - 1e-9) : -1;
+        R.ox = R.p0 >= 0 ? R.p0 * R.p0 * (1 + 1e-9) : -1;
+        break;
+      case DOE_TGT_RING: {
         R.p0 = S.inner * mn * 0.5;
         R.p1 = S.outer * mn * 0.5;
+        // fast decisions only for 0 <= r0 <= r1; otherwise every sample takes the exact test
+        const bool ok = R.p0 >= 0 && R.p1 >= R.p0;
+        const double q0 = R.p0 * R.p0, q1 = R.p1 * R.p1;
+        R.s = ok ? q0 * (1 - 1e-9) : -1;                  // d2 below: inside r0
+        R.oy = ok ? q0 * (1 + 1e-9) : INFINITY;           // d2 above: beyond r0
+        R.gamma = ok ? q1 * (1 - 1e-9) : -INFINITY;       // d2 below: within r1
+        R.background = ok ? q1 * (1 + 1e-9) : INFINITY;   // d2 above: outside r1
         break;
+      }
       case DOE_TGT_DOTS: {
         DOE_REQUIRE(S.cols > 0 && S.rows > 0 && std::isfinite(S.cols) && std::isfinite(S.rows),
                     DOE_ERR_INVALID, "makeTarget: dots/grid need positive cols and rows");
@@ -811,31 +886,34 @@ int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) 
   const long n = (long)nx * ny;
   // a few thousand workgroups in all, each walking its field's grid
   const unsigned per = std::max(1u, std::min(blocks_for(n), (4096u + B - 1) / B));
+  double* part = nullptr;
+  if (maxima) {
+    // room for the maxima and, after them, the per-field thresholds of the ROI
+    rc = ensure_reduce(ctx, size_t(B) * (per + 1), 0);
+    if (rc) return rc;
+    part = ctx->partials;
+    *maxima = per;
+  }
   prof_begin(ctx, "target_make", n * B);
   hipLaunchKernelGGL(k_target, dim3(per, B), dim3(kTB), 0, ctx->stream, (const TgtRec*)base,
                      (const double*)(base + rb), (const unsigned char*)(base + rb + ab), (double*)out->re, nx,
-                     ny);
+                     ny, part);
   prof_end(ctx);
   DOE_HIP(hipGetLastError());
-  DOE_HIP(finish(ctx));
   return DOE_OK;
 }
 
-int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold) {
-  DOE_REQUIRE(ctx && t, DOE_ERR_INVALID, "null argument");
-  DOE_REQUIRE(t->dtype == DOE_F64, DOE_ERR_INVALID, "autoRoi: the target field is FP64");
-  DOE_REQUIRE(t->nx >= 1 && t->ny >= 1 && t->batch >= 1, DOE_ERR_SHAPE, "autoRoi: empty grid");
-  DOE_REQUIRE(dilate == dilate && soft == soft && threshold == threshold, DOE_ERR_INVALID,
-              "autoRoi: dilate, soft and threshold must be numbers");
+// autoRoi of every field of t; `known` > 0: ctx->partials already holds
+// `known` per-block maxima per field (left by make_impl).
+int roi_impl(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold, unsigned known) {
   const int nx = t->nx, ny = t->ny, B = t->batch;
   const long n = (long)nx * ny;
   const int big = std::max(nx, ny);
   const int r = (int)std::min((double)big, js_max(1, js_round(dilate)));
   const bool has_soft = !(soft <= 0);
   const int s = has_soft ? (int)std::min((double)big, js_max(1, js_round(soft))) : 0;
-  DOE_HIP(hipSetDevice(ctx->device));
-  const long nblk = std::max(1L, std::min(64L, (n + kTB * 16 - 1) / (kTB * 16)));
-  int rc = ensure_reduce(ctx, size_t(B) * (nblk + 1), 0);
+  const long nblk = known ? long(known) : std::max(1L, std::min(64L, (n + kTB * 16 - 1) / (kTB * 16)));
+  int rc = ensure_reduce(ctx, size_t(B) * (nblk + 1), 0);  // no reallocation when `known`
   if (rc) return rc;
   double* part = ctx->partials;
   double* thr = ctx->partials + size_t(B) * nblk;
@@ -843,14 +921,11 @@ int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, 
   double* roi = static_cast<double*>(t->im);
   hipStream_t st = ctx->stream;
   prof_begin(ctx, "target_roi", n * B);
-  hipLaunchKernelGGL(k_roi_max, dim3(unsigned(nblk), B), dim3(kTB), 0, st, irr, n, part);
+  if (!known) hipLaunchKernelGGL(k_roi_max, dim3(unsigned(nblk), B), dim3(kTB), 0, st, irr, n, part);
   hipLaunchKernelGGL(k_roi_thr, dim3(B), dim3(64), 0, st, (const double*)part, int(nblk), threshold, thr);
   if (r + s <= kRoiHalo) {
     const dim3 g((nx + kRoiW - 1) / kRoiW, (ny + kRoiH - 1) / kRoiH, B);
-    if (r <= 4 && s <= 4)
-      hipLaunchKernelGGL(k_roi_tile<4>, g, dim3(kTB), 0, st, irr, (const double*)thr, roi, nx, ny, r, s);
-    else
-      hipLaunchKernelGGL(k_roi_tile<0>, g, dim3(kTB), 0, st, irr, (const double*)thr, roi, nx, ny, r, s);
+    hipLaunchKernelGGL(k_roi_tile, g, dim3(kTB), 0, st, irr, (const double*)thr, roi, nx, ny, r, s);
   } else {
     rc = ensure_work(ctx, size_t(n) * B * sizeof(int));
     if (rc) return rc;
@@ -867,6 +942,51 @@ int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, 
   }
   prof_end(ctx);
   DOE_HIP(hipGetLastError());
+  return DOE_OK;
+}
+
+int check_roi_args(const doe_field* t, double dilate, double soft, double threshold) {
+  DOE_REQUIRE(t->dtype == DOE_F64, DOE_ERR_INVALID, "autoRoi: the target field is FP64");
+  DOE_REQUIRE(t->nx >= 1 && t->ny >= 1 && t->batch >= 1, DOE_ERR_SHAPE, "autoRoi: empty grid");
+  DOE_REQUIRE(dilate == dilate && soft == soft && threshold == threshold, DOE_ERR_INVALID,
+              "autoRoi: dilate, soft and threshold must be numbers");
+  return DOE_OK;
+}
+
+}  // namespace
+}  // namespace doe
+
+extern "C" {
+
+int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out) {
+  DOE_REQUIRE(ctx && specs && out, DOE_ERR_INVALID, "null argument");
+  int rc = make_impl(ctx, specs, out, nullptr);
+  if (rc) return rc;
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold) {
+  DOE_REQUIRE(ctx && t, DOE_ERR_INVALID, "null argument");
+  int rc = check_roi_args(t, dilate, soft, threshold);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(ctx->device));
+  rc = roi_impl(ctx, t, dilate, soft, threshold, 0);
+  if (rc) return rc;
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_target_make_roi(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, double dilate,
+                        double soft, double threshold) {
+  DOE_REQUIRE(ctx && specs && out, DOE_ERR_INVALID, "null argument");
+  int rc = check_roi_args(out, dilate, soft, threshold);
+  if (rc) return rc;
+  unsigned per = 0;
+  rc = make_impl(ctx, specs, out, &per);
+  if (rc) return rc;
+  rc = roi_impl(ctx, out, dilate, soft, threshold, per);
+  if (rc) return rc;
   DOE_HIP(finish(ctx));
   return DOE_OK;
 }

@@ -168,11 +168,13 @@ export function makeTarget(kind, nx, ny, opts = {}) {
   } else {
     throw new Error(`makeTarget: unknown kind "${kind}"`);
   }
-  const dev = native.targetMake([spec], nx, ny);
-  if (!opts.roi) {
-    native.targetAutoRoi(dev, Number(opts.dilate === undefined ? 2 : opts.dilate),
-      Number(opts.softEdge === undefined ? 2 : opts.softEdge), 0.05);
-  }
+  const dev = opts.roi
+    ? native.targetMake([spec], nx, ny)
+    : native.targetMake([spec], nx, ny, {
+      dilate: Number(opts.dilate === undefined ? 2 : opts.dilate),
+      soft: Number(opts.softEdge === undefined ? 2 : opts.softEdge),
+      threshold: 0.05,
+    });
   const irr = new Float64Array(nx * ny);
   const devRoi = new Float64Array(nx * ny);
   native.fieldDownload(dev, irr, devRoi);

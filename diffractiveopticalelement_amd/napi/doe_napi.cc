@@ -625,8 +625,9 @@ std::string str(napi_env env, napi_value v) {
   return std::string(buf, n);
 }
 
-// targetMake(specs[], nx, ny) -> FP64 field handle (batch = specs.length,
-// re = irradiance).  spec: {kind: 'disc'|'ring'|'dots'|'grid'|'checker'|
+// targetMake(specs[], nx, ny[, {dilate, soft, threshold}]) -> FP64 field
+// handle (batch = specs.length, re = irradiance; with the ROI options, im =
+// the autoRoi of the same call).  spec: {kind: 'disc'|'ring'|'dots'|'grid'|'checker'|
 // 'lines'|'zone'|'raster', radius, inner, outer, cols, rows, sigma,
 // missing: Int32Array of (col, row) pairs, cells, period, duty,
 // raster: Float64Array, width, height, cover, gamma, background, gain};
@@ -695,7 +696,15 @@ napi_value TargetMake(napi_env env, napi_callback_info info) {
   doe_field* f = nullptr;
   if (throw_status(env, doe_field_create(c, (int)num(env, a.v[1]), (int)num(env, a.v[2]), (int)count, DOE_F64, &f)))
     return nullptr;
-  if (throw_status(env, doe_target_make(c, specs.data(), f))) {
+  // optional 4th argument {dilate, soft, threshold}: the ROI in the same call
+  int rc;
+  if (a.n > 3 && present(env, a.v[3])) {
+    rc = doe_target_make_roi(c, specs.data(), f, opt_num(env, a.v[3], "dilate", NAN),
+                             opt_num(env, a.v[3], "soft", NAN), opt_num(env, a.v[3], "threshold", NAN));
+  } else {
+    rc = doe_target_make(c, specs.data(), f);
+  }
+  if (throw_status(env, rc)) {
     doe_field_free(f);
     return nullptr;
   }

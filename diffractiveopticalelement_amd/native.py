@@ -172,6 +172,8 @@ SIGNATURES = [
     ("doe_target_spec_init", C.c_int, [C.POINTER(TargetSpec), C.c_int]),
     ("doe_target_make", C.c_int, [_VP, C.POINTER(TargetSpec), _VP]),
     ("doe_target_auto_roi", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double]),
+    ("doe_target_make_roi", C.c_int, [_VP, C.POINTER(TargetSpec), _VP, C.c_double, C.c_double,
+                                      C.c_double]),
     ("doe_raster_blur", C.c_int, [_VP, _VP, _VP, C.c_double, c_double_p]),
     ("doe_image_opts_init", C.c_int, [C.POINTER(ImageOpts)]),
     ("doe_error_diffusion", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int]),
@@ -454,6 +456,16 @@ class Context:
         if out is None:
             out = self.field(nx, ny, len(specs), "f64")
         check(self.lib.doe_target_make(self.h, arr, out.h))
+        return out
+
+    def target_make_roi(self, specs, nx: int, ny: int, out: "Field" = None, dilate=2.0, soft=2.0,
+                        threshold=0.05) -> "Field":
+        """makeTarget + autoRoi in one call (re = irradiance, im = ROI)."""
+        specs = list(specs)
+        arr = (TargetSpec * len(specs))(*specs)
+        if out is None:
+            out = self.field(nx, ny, len(specs), "f64")
+        check(self.lib.doe_target_make_roi(self.h, arr, out.h, dilate, soft, threshold))
         return out
 
     def target_auto_roi(self, t: "Field", dilate=2.0, soft=2.0, threshold=0.05) -> None:

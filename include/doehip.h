@@ -382,6 +382,12 @@ int doe_target_make(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out);
  * then a (2s+1)^2 box blur with s = max(1, round(soft)) unless soft <= 0.
  * Exact: equals the reference's values bit for bit for the same irradiance. */
 int doe_target_auto_roi(doe_ctx* ctx, doe_field* t, double dilate, double soft, double threshold);
+/* makeTarget followed by autoRoi in one call (makeTarget without a caller
+ * ROI, projector.js:251): the same results as doe_target_make then
+ * doe_target_auto_roi, with the irradiance maximum taken while the targets
+ * are written instead of by a second read. */
+int doe_target_make_roi(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, double dilate,
+                        double soft, double threshold);
 /* blurRaster (src/doe.js:761-794) of the re plane of every field, src to dst
  * (FP64, may be the same field): separable Gaussian, half-width
  * r = max(1, ceil(3 sigma)), clamped edges; sigma <= 0 copies.  taps: the

@@ -104,6 +104,20 @@ def test_image_target_and_resample_options(ctx):
         np.testing.assert_array_equal(got[0], arr["out"].reshape(90, 70))
 
 
+@pytest.mark.parametrize("roi", [(2.0, 2.0, 0.05), (3.0, 0.0, 0.2), (12.0, 9.0, 0.05)])
+def test_make_roi_equals_make_then_auto_roi(ctx, roi):
+    nx, ny = 200, 136
+    kinds = [("disc", {}), ("dots", {"cols": 5, "rows": 4, "missing": [[1, 1]]}),
+             ("raster", {"raster": text_raster("XY")}), ("zone", {"period": 7}), ("ring", {})]
+    specs = [ctx.target_spec(k, **o) for k, o in kinds]
+    re1, im1 = device_target(ctx, specs, nx, ny, roi=roi)
+    f = ctx.target_make_roi(specs, nx, ny, None, *roi)
+    re2, im2 = f.download()
+    f.free()
+    np.testing.assert_array_equal(re2, re1)
+    np.testing.assert_array_equal(im2, im1)
+
+
 def test_mixed_batch_equals_single_calls(ctx):
     nx, ny = 128, 96
     kinds = [("disc", {}), ("dots", {"cols": 4, "rows": 3}), ("raster", {"raster": text_raster("AB")}),
