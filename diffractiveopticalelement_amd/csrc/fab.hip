@@ -87,15 +87,22 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
     double p_next = below ? (wrap_first ? wrap1(nsrc[i0]) : nsrc[i0]) : 0, p_cur = 0;
     // blocks of 8 samples: the block's inputs are loaded together, then the
     // chain runs on registers
+    // the row below, or (on the last row) any valid row: loads stay in bounds
+    const double* nsafe = below ? nsrc : src;
     for (int c0 = 0; c0 < nx; c0 += 8) {
+      // the block's 16 loads at clamped (valid) positions, all in flight
+      // together; entries past the row's end are never used
       double rv[8], av[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int c = c0 + u;
+        const int c = min(c0 + u, nx - 1);
         const int i = dir == 1 ? c : nx - 1 - c;
-        rv[u] = c < nx ? row[i] : 0;
-        av[u] = 0;
-        if (below && c + 1 < nx) av[u] = wrap_first ? wrap1(nsrc[i + dir]) : nsrc[i + dir];
+        rv[u] = row[i];
+        av[u] = nsafe[min(max(i + dir, 0), nx - 1)];
+      }
+      if (wrap_first) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) av[u] = wrap1(av[u]);
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
