@@ -245,13 +245,14 @@ static int convert_out(doe_ctx* ctx, const void* src, double* host, long n, int 
   return DOE_OK;
 }
 
-// Storage: any grid of at most 65536 x 65536 points per field.  Transforms
-// and propagators additionally need power-of-two sides (check_field).
+// Storage: any grid of at most 65536 rows and 2^32 points per field (a row
+// may be long: a host plane travels as one row).  Transforms and propagators
+// additionally need power-of-two sides (check_field).
 int check_storage(const doe_ctx* ctx, const doe_field* f) {
   DOE_REQUIRE(ctx != nullptr, DOE_ERR_INVALID, "null context");
   DOE_REQUIRE(f != nullptr, DOE_ERR_INVALID, "null field");
-  DOE_REQUIRE(f->nx >= 1 && f->ny >= 1 && f->nx <= 65536 && f->ny <= 65536, DOE_ERR_SHAPE,
-              "grid sides must be in [1, 65536]");
+  DOE_REQUIRE(f->nx >= 1 && f->ny >= 1 && f->ny <= 65536 && (long)f->nx * f->ny <= (1L << 32), DOE_ERR_SHAPE,
+              "grids need 1..65536 rows and at most 2^32 points");
   return DOE_OK;
 }
 

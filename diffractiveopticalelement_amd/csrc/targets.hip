@@ -796,6 +796,7 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
   DOE_REQUIRE(ctx && specs && out, DOE_ERR_INVALID, "null argument");
   DOE_REQUIRE(out->dtype == DOE_F64, DOE_ERR_INVALID, "makeTarget: the target field is FP64");
   DOE_REQUIRE(out->nx >= 1 && out->ny >= 1 && out->batch >= 1, DOE_ERR_SHAPE, "makeTarget: empty grid");
+  DOE_REQUIRE(out->batch <= 65535, DOE_ERR_SHAPE, "makeTarget: at most 65535 targets per call");
   const int nx = out->nx, ny = out->ny, B = out->batch;
   std::vector<TgtRec> recs(B);
   std::vector<double> rasters;
@@ -948,6 +949,7 @@ int roi_impl(doe_ctx* ctx, doe_field* t, double dilate, double soft, double thre
 int check_roi_args(const doe_field* t, double dilate, double soft, double threshold) {
   DOE_REQUIRE(t->dtype == DOE_F64, DOE_ERR_INVALID, "autoRoi: the target field is FP64");
   DOE_REQUIRE(t->nx >= 1 && t->ny >= 1 && t->batch >= 1, DOE_ERR_SHAPE, "autoRoi: empty grid");
+  DOE_REQUIRE(t->batch <= 65535, DOE_ERR_SHAPE, "autoRoi: at most 65535 fields per call");
   DOE_REQUIRE(dilate == dilate && soft == soft && threshold == threshold, DOE_ERR_INVALID,
               "autoRoi: dilate, soft and threshold must be numbers");
   return DOE_OK;
@@ -997,6 +999,7 @@ int doe_raster_blur(doe_ctx* ctx, const doe_field* src, doe_field* dst, double s
   DOE_REQUIRE(src->nx == dst->nx && src->ny == dst->ny && src->batch == dst->batch, DOE_ERR_SHAPE,
               "blurRaster: source and destination grids differ");
   DOE_REQUIRE(sigma == sigma, DOE_ERR_INVALID, "blurRaster: sigma must be a number");
+  DOE_REQUIRE(src->batch <= 65535, DOE_ERR_SHAPE, "blurRaster: at most 65535 fields per call");
   const int nx = src->nx, ny = src->ny, B = src->batch;
   const long n = (long)nx * ny;
   DOE_HIP(hipSetDevice(ctx->device));

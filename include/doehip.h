@@ -15,7 +15,7 @@
  * two planar arrays (real, imaginary), row-major with idx = x + y*nx, fields
  * back to back — the reference's ComplexField layout (src/field.js:22-40).
  * Transform grids have power-of-two sides in [2, 8192]; storage, targets,
- * metrics and images take any side in [1, 65536].
+ * metrics and images take any grid of 1..65536 rows and up to 2^32 points.
  *
  * Citations "ref:" point into the reference toolkit's sources.
  */
@@ -80,9 +80,10 @@ int doe_timer_start(doe_ctx* ctx);
 int doe_timer_stop(doe_ctx* ctx, double* ms);
 
 /* ---- device-resident planar fields (ref: src/field.js:22-196) --------- *
- * Any grid of sides 1..65536 can be stored, uploaded, downloaded, copied,
- * resampled, measured and imaged; transforms, propagators and phase
- * retrieval need power-of-two sides in [2, 8192] (DOE_ERR_UNSUPPORTED). */
+ * Any grid of 1..65536 rows and at most 2^32 points can be stored,
+ * uploaded, downloaded, copied, resampled, measured and imaged; transforms,
+ * propagators and phase retrieval need power-of-two sides in [2, 8192]
+ * (DOE_ERR_UNSUPPORTED). */
 int doe_field_create(doe_ctx* ctx, int nx, int ny, int batch, int dtype, doe_field** out);
 /* Non-owning view of caller device memory (re, im: batch*nx*ny elements). */
 int doe_field_wrap(doe_ctx* ctx, void* re, void* im, int nx, int ny, int batch, int dtype,

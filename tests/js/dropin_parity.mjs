@@ -366,6 +366,9 @@ async function figureChecks() {
   for (const opts of [{}, { invert: true }, { min: 0, max: 2 }]) {
     check(`planeToGray ${JSON.stringify(opts)}`, same(gpu.planeToGray(plane, opts), oGray(plane, opts).gray));
   }
+  // a plane longer than 65536 samples travels as one long row
+  const big = lcg(512 * 512, 61);
+  check('planeToRGB 512^2 plane', same(gpu.planeToRGB(big, { percentile: 0.9 }), oRGB(big, { percentile: 0.9 }).rgb));
   const f = new gpu.ComplexField(64, 32, 8e-6, 8e-6);
   f.re.set(lcg(2048, 41));
   f.im.set(lcg(2048, 43));

@@ -83,7 +83,7 @@ def test_fft2d_host_dropin_and_errors(ctx):
     with pytest.raises(nat.DoeError, match="powers of two"):
         ctx.fft2d(f)
     f.free()
-    with pytest.raises(nat.DoeError, match="in \\[1, 65536\\]"):
+    with pytest.raises(nat.DoeError, match="65536 rows"):
         ctx.field(0, 16)
 
 
