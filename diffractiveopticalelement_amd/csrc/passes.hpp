@@ -140,7 +140,11 @@ struct OpWide<Op, std::void_t<decltype(Op::COL_WIDE)>> {
 // while not filling the caches with read-once data.  Stores stay ordinary,
 // so the next pass finds them there.  Measured on 16 x 2048^2 FP32 fft2d:
 // 412 us against 430 us with ordinary loads; non-temporal stores as well:
-// 433 us.
+// 433 us.  Column tiles whose row segment is narrower than a 128-byte line
+// (16 FP32 columns) load ordinarily: the neighbouring tile reads the other
+// half of each line from L2 (the 16 x 16 ASM column pass fetched 1.15x its
+// planes with non-temporal loads, 1.00x with ordinary ones, in the same
+// time).
 template <typename T, typename S>
 __device__ __forceinline__ T ld_field(const S* p) {
   if constexpr (std::is_same_v<S, float> || std::is_same_v<S, double>)
@@ -622,8 +626,15 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
       const int p = PL::template in_pos<false>(t, i);
       const int row = NAT_IN ? k1 + n1cnt * p : N2 * k1 + p;
       const long a = fbase + (long)row * nx + x;
-      vr[i] = ld_field<T>(ire + a);
-      vi[i] = ld_field<T>(iim + a);
+      // a row segment narrower than a 128-byte line shares its line with the
+      // neighbouring tile, which finds it in L2 only after an ordinary load
+      if constexpr (C * sizeof(S) < 128) {
+        vr[i] = ld<T>(ire + a);
+        vi[i] = ld<T>(iim + a);
+      } else {
+        vr[i] = ld_field<T>(ire + a);
+        vi[i] = ld_field<T>(iim + a);
+      }
       op.pre(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
     }
   } else {
