@@ -25,8 +25,8 @@ FP16 and BF16 (the half-width ones compute in FP32).
 targets (text and image rasters, disc, ring, spot grid, zone plate, checker,
 lines) and planeToRGB (viridis, percentile 0.99) of a 2048^2 field;
 "fabrication" times errorDiffusionQuantize (4 levels) on one 1024^2 image
-with plain rows (wavefront) and on 64 images of 512^2 with serpentine rows
-(one serial chain per image).
+with plain rows (wavefront) and with serpentine rows (one serial chain), and
+on 64 images of 512^2 with serpentine rows (one chain per image).
 
 The second half of the metric, WGS time to screen RMSE <= 1 % at 1024^2
 (learned warm start vs plain random start, and a restart of WGS from its own
@@ -317,8 +317,9 @@ def targets_leg(ctx, n: int = 1024, count: int = 64, reps: int = 10, img_n: int 
 
 def fab_leg(ctx, n: int = 1024, batch: int = 64, nb: int = 512) -> dict:
     """errorDiffusionQuantize (4 levels) on the device: one n^2 image with
-    plain rows (diagonal wavefront, one workgroup) and a batch of nb^2 images
-    with serpentine rows (one serial chain per image, one lane each)."""
+    plain rows (diagonal wavefront, one workgroup), the same image with
+    serpentine rows (one serial chain, rows staged through LDS) and a batch
+    of nb^2 serpentine images (one chain per image, one workgroup each)."""
     rng = np.random.default_rng(5)
     out = {}
     f = ctx.upload(rng.uniform(-np.pi, np.pi, (1, n, n)), None, "f64")
@@ -330,6 +331,10 @@ def fab_leg(ctx, n: int = 1024, batch: int = 64, nb: int = 512) -> dict:
         ctx.error_diffusion(f, o, 4, snake=False)
     out["wavefront_ms"] = round((time.perf_counter() - t0) * 1e3 / 3, 3)
     out["wavefront_n"] = n
+    ctx.error_diffusion(f, o, 4, snake=True)
+    t0 = time.perf_counter()
+    ctx.error_diffusion(f, o, 4, snake=True)
+    out["serpentine_single_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
     f.free()
     o.free()
     fb = ctx.field(nb, nb, batch, "f64")

@@ -9,14 +9,16 @@
 //   serpentine rows (the reference's default): a row runs in the direction
 //     opposite to the previous one, so its first sample needs the previous
 //     row complete and every sample of an image is one step of one chain.
-//     One lane per image: a batch of images runs as parallel chains;
+//     Rows of at most 4096 samples: one workgroup per image, the chain on
+//     rows staged in LDS while the other waves stream rows in and out;
+//     longer rows: one lane per image, on HBM;
 //   left-to-right rows: sample (i, j) needs (i+1, j-1) and (i-1, j), so row
 //     j can trail row j-1 by two samples — a diagonal wavefront of
 //     nx + 2 (ny - 1) steps per image, one workgroup per image, one barrier
 //     per step.
-// Both keep the pending shares of the next row in registers (a sample's
-// three shares arrive from three consecutive samples of the row above) and
-// read each sample's original value once.
+// The chains keep the pending shares of the next row in registers (a
+// sample's three shares arrive from three consecutive samples of the row
+// above) and read each sample's original value once.
 #include <algorithm>
 #include <cmath>
 
@@ -47,9 +49,22 @@ __device__ __forceinline__ double js_round(double x) {
   return (x - f >= 0.5) ? f + 1 : f;
 }
 
+// Math.round(v / step) with the quotient taken as v * inv (inv = 1 / step):
+// the two quotients differ by a few ulps, which changes the rounding only
+// when the quotient lies that close to a half-integer (an integer boundary
+// gives the same result from either side), so within 2^-40 of one the IEEE
+// division decides.
+__device__ __forceinline__ double round_quot(double v, double step, double inv) {
+  const double y = v * inv;
+  const double f = floor(y);
+  const double d = (y - f) - 0.5;
+  if (fabs(d) > 0x1p-40 * fmax(1.0, fabs(y))) return d >= 0 ? f + 1 : f;
+  return js_round(v / step);
+}
+
 // One sample: quantise v, return the wrapped error; q is the level.
-__device__ __forceinline__ double quantise(double v, double step, double* q) {
-  *q = js_round(v / step) * step;
+__device__ __forceinline__ double quantise(double v, double step, double inv, double* q) {
+  *q = round_quot(v, step, inv) * step;
   double e = v - *q;
   if (e > M_PI)
     e -= 2 * M_PI;
@@ -63,8 +78,8 @@ __device__ __forceinline__ double quantise(double v, double step, double* q) {
 // complete, and the levels replace the values as the scan passes; wrapped at
 // the end.
 __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ in, double* __restrict__ out,
-                                                     int nx, int ny, int batch, double step, int snake,
-                                                     int wrap_first) {
+                                                     int nx, int ny, int batch, double step, double inv,
+                                                     int snake, int wrap_first) {
   const int b = blockIdx.x * 64 + threadIdx.x;
   if (b >= batch) return;
   const long n = (long)nx * ny;
@@ -112,7 +127,7 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
         // the first sample of a row has no 7/16 share (and keeps a -0)
         const double v = c ? rv[u] + carry : rv[u];
         double q;
-        const double e = quantise(v, step, &q);
+        const double e = quantise(v, step, inv, &q);
         row[i] = q;
         carry = e * (7.0 / 16);
         if (below) {
@@ -127,11 +142,89 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
   for (long k = 0; k < n; ++k) o[k] = wrap1(o[k]);
 }
 
+// Serpentine rows of one image per workgroup, rows staged through LDS: three
+// row buffers rotate.  At row j, lane 0 runs the chain on row j (its sums,
+// replaced by the levels) and row j+1 (its wrapped inputs ahead of the scan,
+// its sums behind), while waves 1.. write row j-1's levels (wrapped) out and
+// load row j+2's inputs into the buffer that frees.  One barrier per row.
+constexpr int kRowThreads = 256;
+constexpr int kRowMax = 4096;  // longest row staged (3 x 32 KiB of LDS)
+__global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __restrict__ in, double* __restrict__ out,
+                                                             int nx, int ny, double step, double inv,
+                                                             int wrap_first) {
+  extern __shared__ double R[];  // 3 rows of nx
+  const long n = (long)nx * ny;
+  const double* src = in + (long)blockIdx.x * n;
+  double* o = out + (long)blockIdx.x * n;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < nx; k += kRowThreads) {
+    const double a = src[k];
+    R[k] = wrap_first ? wrap1(a) : a;
+    if (ny > 1) {
+      const double b = src[nx + k];
+      R[nx + k] = wrap_first ? wrap1(b) : b;
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < ny; ++j) {
+    if (tid == 0) {
+      double* cur = R + (j % 3) * nx;
+      double* nxt = R + ((j + 1) % 3) * nx;  // unused past the last row
+      const int dir = (j & 1) ? -1 : 1;
+      const bool below = j + 1 < ny;
+      double carry = 0;
+      const int i0 = dir == 1 ? 0 : nx - 1;
+      double p_next = below ? nxt[i0] : 0, p_cur = 0;
+      for (int c0 = 0; c0 < nx; c0 += 8) {
+        double rv[8], av[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = min(c0 + u, nx - 1);
+          const int i = dir == 1 ? c : nx - 1 - c;
+          rv[u] = cur[i];
+          av[u] = nxt[min(max(i + dir, 0), nx - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = c0 + u;
+          if (c >= nx) break;
+          const int i = dir == 1 ? c : nx - 1 - c;
+          const double v = c ? rv[u] + carry : rv[u];
+          double q;
+          const double e = quantise(v, step, inv, &q);
+          cur[i] = q;
+          carry = e * (7.0 / 16);
+          if (below) {
+            if (c > 0) nxt[i - dir] = p_cur + e * (3.0 / 16);
+            p_cur = p_next + e * (5.0 / 16);
+            p_next = av[u] + e * (1.0 / 16);
+          }
+        }
+      }
+      if (below) nxt[dir == 1 ? nx - 1 : 0] = p_cur;
+    } else if (tid >= 64) {
+      // the buffer of row j-1 (levels) becomes row j+2's
+      double* old = R + ((j + 2) % 3) * nx;
+      for (int k = tid - 64; k < nx; k += kRowThreads - 64) {
+        if (j >= 1) o[(long)(j - 1) * nx + k] = wrap1(old[k]);
+        if (j + 2 < ny) {
+          const double a = src[(long)(j + 2) * nx + k];
+          old[k] = wrap_first ? wrap1(a) : a;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double* last = R + ((ny - 1) % 3) * nx;
+  for (int k = tid; k < nx; k += kRowThreads) o[(long)(ny - 1) * nx + k] = wrap1(last[k]);
+}
+
 // Plain (left-to-right) rows on a diagonal wavefront: step t processes sample
 // i = t - 2j of every row j in flight.  One workgroup per image; rows are
 // dealt to threads cyclically; out holds the working values in place.
 __global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict__ in, double* __restrict__ out,
-                                                       int nx, int ny, double step, int wrap_first) {
+                                                       int nx, int ny, double step, double inv,
+                                                       int wrap_first) {
   const int b = blockIdx.x;
   const long n = (long)nx * ny;
   const double* src = in + (long)b * n;
@@ -154,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict_
         double* row = o + (long)j * nx;
         const double v = i ? row[i] + carry[q] : row[i];
         double qv;
-        const double e = quantise(v, step, &qv);
+        const double e = quantise(v, step, inv, &qv);
         row[i] = qv;
         carry[q] = e * (7.0 / 16);
         if (j + 1 < ny) {
@@ -198,15 +291,19 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
   DOE_REQUIRE(levels != 0, DOE_ERR_INVALID, "errorDiffusionQuantize: levels must be non-zero");
   const int nx = phase->nx, ny = phase->ny, B = phase->batch;
   const double step = (2 * M_PI) / levels;
+  const double inv = 1 / step;
   DOE_HIP(hipSetDevice(ctx->device));
   const double* in = static_cast<const double*>(phase->re);
   double* o = static_cast<double*>(out->re);
-  if (snake) {
+  if (snake && nx <= kRowMax) {
+    hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 3 * nx * sizeof(double), ctx->stream, in, o,
+                       nx, ny, step, inv, wrap_first ? 1 : 0);
+  } else if (snake) {
     hipLaunchKernelGGL(k_diffuse_lane, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step,
-                       1, wrap_first ? 1 : 0);
+                       inv, 1, wrap_first ? 1 : 0);
   } else {
     DOE_REQUIRE(ny <= 8 * 1024, DOE_ERR_UNSUPPORTED, "errorDiffusionQuantize: at most 8192 rows");
-    hipLaunchKernelGGL(k_diffuse_wave, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step,
+    hipLaunchKernelGGL(k_diffuse_wave, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step, inv,
                        wrap_first ? 1 : 0);
   }
   DOE_HIP(hipGetLastError());

@@ -65,70 +65,40 @@ export function quantizePhase(phi, levels = 4) {
   return wrapPhase(out);
 }
 
-// Floyd–Steinberg on the wrapped phase (ref: doe.js:185-226).  Plain
-// left-to-right rows (snake: false) run on the GPU as a diagonal wavefront
-// (doe_error_diffusion; row j trails row j-1 by two samples).  Serpentine
-// rows (the default) make the whole image one serial chain — every row needs
-// the previous one complete — so a single image runs here on the host, where
-// one core outpaces one GPU lane on a dependent FP64 chain; batches of
-// serpentine images run on the device one lane each through the C ABI.
+// Floyd–Steinberg on the wrapped phase (ref: doe.js:185-226), on the GPU
+// (doe_error_diffusion).  Plain left-to-right rows (snake: false) run as a
+// diagonal wavefront (row j trails row j-1 by two samples); serpentine rows
+// (the default) make the whole image one serial chain, which runs in one
+// workgroup that stages rows through LDS.
 export function errorDiffusionQuantize(phi, levels = 4, { snake = true, wrapFirst = true } = {}) {
-  if (!snake) {
-    const n = phi.length;
-    const nx = Math.round(Math.sqrt(n));
-    const ny = n / nx;
-    // a non-integer row count passes the reference's check and reads past
-    // the array there; the device path refuses it with the same message
-    if (nx * ny !== n || !Number.isInteger(ny)) throw new Error('errorDiffusionQuantize: expected a square grid');
-    const plane = phi instanceof Float64Array ? phi : Float64Array.from(phi);
-    const src = native.fieldCreate(nx, ny, 1, 1);
-    native.fieldUpload(src, plane, plane);
-    const dst = native.fieldCreate(nx, ny, 1, 1);
-    native.errorDiffusion(src, dst, levels, false, wrapFirst);
-    const out = new Float64Array(n);
-    native.fieldDownload(dst, out, new Float64Array(n));
-    return out;
-  }
-  return errorDiffusionHost(phi, levels, wrapFirst);
-}
-
-function errorDiffusionHost(phi, levels, wrapFirst) {
   const n = phi.length;
-  const buf = wrapFirst ? wrapPhase(Float64Array.from(phi)) : Float64Array.from(phi);
-  const out = new Float64Array(n);
-  const step = (2 * Math.PI) / levels;
   const nx = Math.round(Math.sqrt(n));
   const ny = n / nx;
-  if (nx * ny !== n) throw new Error('errorDiffusionQuantize: expected a square grid');
-  const taps = [[1, 0, 7 / 16], [-1, 1, 3 / 16], [0, 1, 5 / 16], [1, 1, 1 / 16]];
-  for (let j = 0; j < ny; j++) {
-    const dir = j % 2 === 1 ? -1 : 1;
-    for (let c = 0; c < nx; c++) {
-      const i = dir === 1 ? c : nx - 1 - c;
-      const k = i + j * nx;
-      const q = Math.round(buf[k] / step) * step;
-      out[k] = q;
-      let e = buf[k] - q;
-      if (e > Math.PI) e -= 2 * Math.PI;
-      else if (e <= -Math.PI) e += 2 * Math.PI;
-      for (const [di, dj, w] of taps) {
-        const ii = i + dir * di;
-        const jj = j + dj;
-        if (ii >= 0 && ii < nx && jj < ny) buf[ii + jj * nx] += e * w;
-      }
-    }
-  }
-  return wrapPhase(out);
+  // a non-integer row count passes the reference's check and reads past the
+  // array there; here it throws with the reference's message
+  if (nx * ny !== n || !Number.isInteger(ny)) throw new Error('errorDiffusionQuantize: expected a square grid');
+  const plane = phi instanceof Float64Array ? phi : Float64Array.from(phi);
+  const src = native.fieldCreate(nx, ny, 1, 1);
+  native.fieldUpload(src, plane, plane);
+  const dst = native.fieldCreate(nx, ny, 1, 1);
+  native.errorDiffusion(src, dst, levels, snake, wrapFirst);
+  const out = new Float64Array(n);
+  native.fieldDownload(dst, out, new Float64Array(n));
+  return out;
 }
 
-// Etch depth in [0, lambda / (n - 1)) of a host phase array (ref:
-// doe.js:967-976).  An elementwise map of a host array is cheaper here than
-// a PCIe round trip; device-resident phases use doe_phase_to_height.
+// Etch depth in [0, lambda / (n - 1)) (ref: doe.js:967-976), on the GPU
+// (doe_phase_to_height); the phase travels as one row.
 export function phaseToHeight(phase, lambda, n = 1.52) {
-  const s = lambda / (2 * Math.PI * (n - 1));
-  const step = lambda / (n - 1);
-  const h = new Float64Array(phase.length);
-  for (let k = 0; k < phase.length; k++) h[k] = (((phase[k] * s) % step) + step) % step;
+  const len = phase.length;
+  const h = new Float64Array(len);
+  if (len === 0) return h;
+  const plane = phase instanceof Float64Array ? phase : Float64Array.from(phase);
+  const src = native.fieldCreate(len, 1, 1, 1);
+  native.fieldUpload(src, plane, plane);
+  const dst = native.fieldCreate(len, 1, 1, 1);
+  native.phaseToHeight(src, dst, lambda, n);
+  native.fieldDownload(dst, h, new Float64Array(len));
   return h;
 }
 

@@ -837,6 +837,20 @@ napi_value ErrorDiffusion(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// phaseToHeight(src, dst, lambda, n): re planes, FP64
+napi_value PhaseToHeight(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  doe_field* d = a.n > 1 ? field_of(env, a.v[1]) : nullptr;
+  if (!s || !d || a.n < 4) {
+    napi_throw_type_error(env, nullptr, "phaseToHeight(src, dst, lambda, n)");
+    return nullptr;
+  }
+  throw_status(env, doe_phase_to_height(ctx(env), s, d, num(env, a.v[2], 0), num(env, a.v[3], 1.52)));
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   napi_property_descriptor d[] = {
       {"version", nullptr, Version, nullptr, nullptr, nullptr, napi_default, nullptr},
@@ -861,6 +875,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"rasterBlur", nullptr, RasterBlur, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fieldImage", nullptr, FieldImage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"errorDiffusion", nullptr, ErrorDiffusion, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"phaseToHeight", nullptr, PhaseToHeight, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

@@ -3,6 +3,7 @@
 plain rows) and the etch-depth map (ref: src/doe.js:185-226, 967-976).
 Gate: equal bit for bit (every step is exact IEEE arithmetic in the
 reference's order)."""
+import math
 import time
 
 import numpy as np
@@ -54,6 +55,82 @@ def test_error_diffusion_batch_and_no_wrap(ctx, snake):
         out, _ = oj.run("error_diffusion", {"levels": 8, "opts": {"snake": snake, "wrapFirst": False}},
                         {"phase": phi})
         np.testing.assert_array_equal(got[q], out["out"])
+
+
+@pytest.mark.parametrize("snake", [True, False])
+@pytest.mark.parametrize("levels", [4, 3])
+def test_error_diffusion_half_level_inputs(ctx, snake, levels):
+    """Quotients on and next to half-integers: the level decision there goes
+    through the IEEE division (the fast reciprocal quotient is only trusted
+    away from them)."""
+    n = 64
+    st = 2 * np.pi / levels
+    rng = np.random.default_rng(levels)
+    k = rng.integers(-levels, levels, n * n)
+    for phi in ((k + 0.5) * st, np.full(n * n, 0.5 * st),
+                np.nextafter((k + 0.5) * st, np.where(k % 2 == 0, np.inf, -np.inf))):
+        for wrap_first in (True, False):
+            out, _ = oj.run("error_diffusion", {"levels": levels, "opts": {"snake": snake, "wrapFirst": wrap_first}},
+                            {"phase": phi})
+            got = device_fs(ctx, [phi], n, levels, snake, wrap_first)[0]
+            np.testing.assert_array_equal(got, out["out"])
+
+
+def _wrap(p):
+    p = math.fmod(p, 2 * math.pi)
+    if p > math.pi:
+        p -= 2 * math.pi
+    elif p <= -math.pi:
+        p += 2 * math.pi
+    return p
+
+
+def _gather_fs(x, levels, wrap_first=True):
+    """Serpentine Floyd–Steinberg of an ny x nx grid in gather form (each
+    sample pulls its 1/16, 5/16, 3/16 shares from the row above in that row's
+    scan order, then 7/16 of its predecessor); equal to the oracle's push
+    form on square grids (tests/test_fab_cpu.py)."""
+    ny, nx = x.shape
+    step = 2 * math.pi / levels
+    err = np.zeros((ny, nx))
+    out = np.zeros((ny, nx))
+    for j in range(ny):
+        d = -1 if j % 2 == 1 else 1
+        du = -1 if (j - 1) % 2 == 1 else 1
+        prev = None
+        for c in range(nx):
+            i = c if d == 1 else nx - 1 - c
+            v = _wrap(x[j, i]) if wrap_first else x[j, i]
+            if j > 0:
+                for src, w in ((i - du, 1 / 16), (i, 5 / 16), (i + du, 3 / 16)):
+                    if 0 <= src < nx:
+                        v += err[j - 1, src] * w
+            if prev is not None:
+                v += prev * 7 / 16
+            f = math.floor(v / step)
+            q = (f + 1 if v / step - f >= 0.5 else f) * step
+            e = v - q
+            if e > math.pi:
+                e -= 2 * math.pi
+            elif e <= -math.pi:
+                e += 2 * math.pi
+            out[j, i], err[j, i], prev = q, e, e
+    return np.vectorize(_wrap)(out)
+
+
+def test_error_diffusion_long_rows_one_lane_per_image(ctx):
+    """Rows longer than 4096 samples take the one-lane-per-image schedule."""
+    nx, ny, B = 4100, 3, 2
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-5, 5, (B, ny, nx))
+    f = ctx.upload(x, None, "f64")
+    o = ctx.field(nx, ny, B, "f64")
+    ctx.error_diffusion(f, o, 4, True, True)
+    re, _ = o.download()
+    for b in range(B):
+        np.testing.assert_array_equal(re[b], _gather_fs(x[b], 4))
+    f.free()
+    o.free()
 
 
 def test_error_diffusion_smooth_phase_and_timing(ctx):
