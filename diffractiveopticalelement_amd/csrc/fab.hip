@@ -149,62 +149,72 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
 // load row j+2's inputs into the buffer that frees.  One barrier per row.
 constexpr int kRowThreads = 256;
 constexpr int kRowMax = 4096;  // longest row staged (3 x 32 KiB of LDS)
+constexpr int kRowPad = 8;     // LDS rows padded on both sides: the chain's reads
+                               // and stores past a row's ends land in the padding
+
+// One serpentine row in scan direction DIR.  pc / pn point at the row's and
+// the next row's first sample in scan order.  The 7/16 share starts as -0
+// (x + -0 == x for every x, so the first sample keeps its value and sign);
+// the share behind the first sample, and the next row's shares on the last
+// row, go to padding or to a free buffer.
+template <int DIR>
+__device__ __forceinline__ void chain_row(double* pc, double* pn, int nx, double step, double inv) {
+  double carry = -0.0;
+  double p_next = pn[0], p_cur = 0;
+  auto sample = [&](int c, double r, double a) {
+    double q;
+    const double e = quantise(r + carry, step, inv, &q);
+    pc[DIR * c] = q;
+    carry = e * (7.0 / 16);
+    pn[DIR * (c - 1)] = p_cur + e * (3.0 / 16);
+    p_cur = p_next + e * (5.0 / 16);
+    p_next = a + e * (1.0 / 16);
+  };
+  int c0 = 0;
+  for (; c0 + 8 <= nx; c0 += 8) {
+    double rv[8], av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      rv[u] = pc[DIR * (c0 + u)];
+      av[u] = pn[DIR * (c0 + u + 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) sample(c0 + u, rv[u], av[u]);
+  }
+  for (int c = c0; c < nx; ++c) sample(c, pc[DIR * c], pn[DIR * (c + 1)]);
+  pn[DIR * (nx - 1)] = p_cur;
+}
+
 __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __restrict__ in, double* __restrict__ out,
                                                              int nx, int ny, double step, double inv,
                                                              int wrap_first) {
-  extern __shared__ double R[];  // 3 rows of nx
+  extern __shared__ double R[];  // 3 rows of nx + 2 kRowPad
+  const int ld = nx + 2 * kRowPad;
+  auto row = [&](int slot) { return R + slot * ld + kRowPad; };
   const long n = (long)nx * ny;
   const double* src = in + (long)blockIdx.x * n;
   double* o = out + (long)blockIdx.x * n;
   const int tid = threadIdx.x;
   for (int k = tid; k < nx; k += kRowThreads) {
     const double a = src[k];
-    R[k] = wrap_first ? wrap1(a) : a;
+    row(0)[k] = wrap_first ? wrap1(a) : a;
     if (ny > 1) {
       const double b = src[nx + k];
-      R[nx + k] = wrap_first ? wrap1(b) : b;
+      row(1)[k] = wrap_first ? wrap1(b) : b;
     }
   }
   __syncthreads();
   for (int j = 0; j < ny; ++j) {
     if (tid == 0) {
-      double* cur = R + (j % 3) * nx;
-      double* nxt = R + ((j + 1) % 3) * nx;  // unused past the last row
-      const int dir = (j & 1) ? -1 : 1;
-      const bool below = j + 1 < ny;
-      double carry = 0;
-      const int i0 = dir == 1 ? 0 : nx - 1;
-      double p_next = below ? nxt[i0] : 0, p_cur = 0;
-      for (int c0 = 0; c0 < nx; c0 += 8) {
-        double rv[8], av[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = min(c0 + u, nx - 1);
-          const int i = dir == 1 ? c : nx - 1 - c;
-          rv[u] = cur[i];
-          av[u] = nxt[min(max(i + dir, 0), nx - 1)];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = c0 + u;
-          if (c >= nx) break;
-          const int i = dir == 1 ? c : nx - 1 - c;
-          const double v = c ? rv[u] + carry : rv[u];
-          double q;
-          const double e = quantise(v, step, inv, &q);
-          cur[i] = q;
-          carry = e * (7.0 / 16);
-          if (below) {
-            if (c > 0) nxt[i - dir] = p_cur + e * (3.0 / 16);
-            p_cur = p_next + e * (5.0 / 16);
-            p_next = av[u] + e * (1.0 / 16);
-          }
-        }
-      }
-      if (below) nxt[dir == 1 ? nx - 1 : 0] = p_cur;
+      double* cur = row(j % 3);
+      double* nxt = row((j + 1) % 3);  // past the last row: a free buffer
+      if (j & 1)
+        chain_row<-1>(cur + nx - 1, nxt + nx - 1, nx, step, inv);
+      else
+        chain_row<1>(cur, nxt, nx, step, inv);
     } else if (tid >= 64) {
       // the buffer of row j-1 (levels) becomes row j+2's
-      double* old = R + ((j + 2) % 3) * nx;
+      double* old = row((j + 2) % 3);
       for (int k = tid - 64; k < nx; k += kRowThreads - 64) {
         if (j >= 1) o[(long)(j - 1) * nx + k] = wrap1(old[k]);
         if (j + 2 < ny) {
@@ -215,7 +225,7 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
     }
     __syncthreads();
   }
-  const double* last = R + ((ny - 1) % 3) * nx;
+  const double* last = row((ny - 1) % 3);
   for (int k = tid; k < nx; k += kRowThreads) o[(long)(ny - 1) * nx + k] = wrap1(last[k]);
 }
 
@@ -296,8 +306,8 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
   const double* in = static_cast<const double*>(phase->re);
   double* o = static_cast<double*>(out->re);
   if (snake && nx <= kRowMax) {
-    hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 3 * nx * sizeof(double), ctx->stream, in, o,
-                       nx, ny, step, inv, wrap_first ? 1 : 0);
+    hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 3 * (nx + 2 * kRowPad) * sizeof(double),
+                       ctx->stream, in, o, nx, ny, step, inv, wrap_first ? 1 : 0);
   } else if (snake) {
     hipLaunchKernelGGL(k_diffuse_lane, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step,
                        inv, 1, wrap_first ? 1 : 0);

@@ -7,17 +7,7 @@ import { ComplexField } from './field.js';
 import { native, dtypeCode } from './native.js';
 import { propagateAngularSpectrumInPlace, propagateFresnelFFT, focalPlaneSpectrum, lensPhase } from './propagate.js';
 import { designDOE, illuminationProfile, gerchbergSaxtonMultiPlane, errorDiffusionQuantize } from './doe.js';
-import {
-  correlation,
-  maskedSum,
-  regionStats,
-  rmsError,
-  rmsRadius,
-  spotUniformity,
-  ssim,
-  zeroOrderFraction,
-  summarizeMetrics,
-} from './metrics.js';
+import { summarizeMetrics } from './metrics.js';
 
 function pick(v, d) {
   return v === undefined || v === null ? d : v;
@@ -419,50 +409,6 @@ export class ProjectorSystem {
       cols: grid ? (grid.cols === undefined ? 8 : grid.cols) : 0,
       rows: grid ? (grid.rows === undefined ? 8 : grid.rows) : 0,
       minPeak: grid && grid.minPeak !== undefined ? grid.minPeak : 0.05,
-    };
-  }
-
-  // The same figures on the host from the downloaded irradiance (FP64 loops
-  // of metrics.js); kept for callers that hold host-only fields.
-  screenMetricsHost(field, { target = null, roi = null, phase = null } = {}) {
-    const tgt = target || this.target;
-    const irr = field.intensity();
-    const mask = roi || tgt.roi;
-    const element = new ComplexField(this.nx, this.ny, this.dx, this.dy);
-    element.setAmplitudePhase(this.result.illumination, phase || this.result.phase);
-    const cell = field.dx * field.dy;
-    const scaled = Float64Array.from(tgt.irradiance);
-    const pt = maskedSum(irr, mask) * cell;
-    const tt = maskedSum(scaled, null) * cell;
-    const s = tt > 0 ? pt / tt : 0;
-    for (let k = 0; k < scaled.length; k++) scaled[k] *= s;
-    const provisional = regionStats(irr, mask);
-    const stats = regionStats(irr, mask, { threshold: 0.1 * provisional.max });
-    const spots = tgt.grid ? spotUniformity(irr, this.nx, this.ny, tgt.grid) : null;
-    const incident = maskedSum(this.illumination(), null) * this.dx * this.dy;
-    const total = maskedSum(irr, null) * cell;
-    const inSignal = maskedSum(irr, mask) * cell;
-    const rms = rmsError(irr, scaled, mask);
-    const corr = correlation(irr, scaled, mask);
-    return {
-      efficiency: incident > 0 ? inSignal / incident : 0,
-      fluxFraction: total > 0 ? inSignal / total : 0,
-      uniformity: spots ? spots.uniformity : stats.uniformity,
-      spotUniformity: spots ? spots.uniformity : null,
-      flatness: stats.flatness,
-      contrast: stats.cv,
-      rmse: rms,
-      fidelity: (corr + 1) / 2,
-      correlation: corr,
-      ssim: ssim(irr, scaled, this.nx, this.ny, { window: 8, weight: 'target' }),
-      snr: rms > 0 ? 20 * Math.log10(1 / rms) : Infinity,
-      zeroOrder: zeroOrderFraction(element),
-      rmsRadius: rmsRadius(irr, this.nx, field.dx, 0, 0, field.dy).r,
-      meanIntensity: stats.mean,
-      peakIntensity: stats.max,
-      powerInSignal: inSignal,
-      powerAtScreen: total,
-      powerIncident: incident,
     };
   }
 
