@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "rng.hpp"
 
 namespace doe {
 
@@ -563,6 +564,22 @@ int doe_field_copy(doe_ctx* ctx, const doe_field* src, doe_field* dst) {
   DOE_HIP(hipMemcpyAsync(dst->re, src->re, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   DOE_HIP(hipMemcpyAsync(dst->im, src->im, bytes, hipMemcpyDeviceToDevice, ctx->stream));
   DOE_HIP(hipStreamSynchronize(ctx->stream));
+  return DOE_OK;
+}
+
+int doe_random_phase(doe_ctx* ctx, doe_field* out, unsigned seed) {
+  int rc = check_storage(ctx, out);
+  if (rc) return rc;
+  DOE_REQUIRE(out->dtype == DOE_F64 || out->dtype == DOE_F32, DOE_ERR_INVALID,
+              "randomPhase: FP64 or FP32 fields");
+  const long n = (long)out->count();
+  DOE_REQUIRE(n <= (1L << kRngLevels), DOE_ERR_SHAPE, "randomPhase: at most 2^40 samples per call");
+  DOE_HIP(hipSetDevice(ctx->device));
+  if (out->dtype == DOE_F64)
+    DOE_HIP(launch_random_phase(ctx, static_cast<double*>(out->re), n, seed));
+  else
+    DOE_HIP(launch_random_phase(ctx, static_cast<float*>(out->re), n, seed));
+  DOE_HIP(finish(ctx));
   return DOE_OK;
 }
 

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "gs_ops.hpp"
+#include "rng.hpp"
 
 namespace doe {
 
@@ -279,13 +280,14 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   rc = upload_t<T>(ctx, illum, illum_d, n);
   if (!rc) rc = upload_t<T>(ctx, tgt, tgt_d, n);
   if (!rc && roi) rc = upload_t<T>(ctx, roi, W.roi_d, n);
-  std::vector<double> phi;
-  if (!init_phase) {
-    phi.assign(n, 0.0);
-    if (far) random_phase(phi, seed);  // fourierCgh default (doe.js:651)
-  }
-  if (!rc) rc = upload_t<T>(ctx, init_phase ? init_phase : phi.data(), W.phi0, n);
+  if (!rc && init_phase) rc = upload_t<T>(ctx, init_phase, W.phi0, n);
   if (rc) return rc;
+  if (!init_phase) {
+    if (far)
+      DOE_HIP(launch_random_phase(ctx, W.phi0, n, seed));  // fourierCgh default (doe.js:651)
+    else
+      DOE_HIP(hipMemsetAsync(W.phi0, 0, tb, ctx->stream));  // flat start
+  }
   DOE_HIP(hipMemsetAsync(ph[0], 0, tb, ctx->stream));
   DOE_HIP(hipMemsetAsync(ph[1], 0, tb, ctx->stream));
   DOE_HIP(hipMemsetAsync(W.hist, 0, sizeof(double) * 2 * size_t(K > 0 ? K : 1), ctx->stream));

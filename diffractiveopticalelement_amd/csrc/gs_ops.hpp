@@ -334,17 +334,6 @@ __global__ void k_to_f64_t(const T* __restrict__ a, double* __restrict__ b, long
   if (i < n) b[i] = double(a[i]);
 }
 
-// xorshift32 random phase of doe.js:65-86: seed>>>0 || 1; phi = (u*2-1)*pi.
-inline void random_phase(std::vector<double>& p, unsigned seed) {
-  uint32_t s = seed ? seed : 1u;
-  for (double& v : p) {
-    s ^= s << 13;
-    s ^= s >> 17;
-    s ^= s << 5;
-    v = (double(s) / 4294967296.0 * 2 - 1) * kPi;
-  }
-}
-
 template <typename T>
 inline int upload_t(doe_ctx* ctx, const double* host, T* dst, long n) {
   int rc = ensure_stage(ctx, size_t(n) * sizeof(double));

@@ -41,11 +41,16 @@ export function makeRng(seed = 1) {
   };
 }
 
-// (u*2 - 1)*pi from the xorshift stream (ref: doe.js:81-86)
+// (u*2 - 1)*pi from the xorshift stream (ref: doe.js:81-86), generated on
+// the GPU (doe_random_phase: each thread jumps ahead to its chunk of the
+// stream), equal to the host stream bit for bit.
 export function randomPhase(nx, ny, seed = 1) {
-  const next = makeRng(seed);
-  const p = new Float64Array(nx * ny);
-  for (let k = 0; k < p.length; k++) p[k] = (next() * 2 - 1) * Math.PI;
+  const n = nx * ny;
+  const p = new Float64Array(n);
+  if (n === 0) return p;
+  const f = native.fieldCreate(n, 1, 1, 1);
+  native.randomPhase(f, seed >>> 0);
+  native.fieldDownload(f, p, new Float64Array(n));
   return p;
 }
 

@@ -837,6 +837,21 @@ napi_value ErrorDiffusion(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// randomPhase(dst, seed): re plane, FP64 or FP32; seed already ToUint32
+napi_value RandomPhase(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* d = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  if (!d || a.n < 2) {
+    napi_throw_type_error(env, nullptr, "randomPhase(dst, seed)");
+    return nullptr;
+  }
+  uint32_t seed = 0;
+  CHECK_NAPI(napi_get_value_uint32(env, a.v[1], &seed));
+  throw_status(env, doe_random_phase(ctx(env), d, seed));
+  return nullptr;
+}
+
 // phaseToHeight(src, dst, lambda, n): re planes, FP64
 napi_value PhaseToHeight(napi_env env, napi_callback_info info) {
   Args a;
@@ -876,6 +891,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"fieldImage", nullptr, FieldImage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"errorDiffusion", nullptr, ErrorDiffusion, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"phaseToHeight", nullptr, PhaseToHeight, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"randomPhase", nullptr, RandomPhase, nullptr, nullptr, nullptr, napi_default, nullptr},
   };
   napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d);
   return exports;

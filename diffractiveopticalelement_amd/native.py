@@ -176,6 +176,7 @@ SIGNATURES = [
                                       C.c_double]),
     ("doe_raster_blur", C.c_int, [_VP, _VP, _VP, C.c_double, c_double_p]),
     ("doe_image_opts_init", C.c_int, [C.POINTER(ImageOpts)]),
+    ("doe_random_phase", C.c_int, [_VP, _VP, C.c_uint]),
     ("doe_error_diffusion", C.c_int, [_VP, _VP, _VP, C.c_int, C.c_int, C.c_int]),
     ("doe_phase_to_height", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double]),
     ("doe_field_image", C.c_int, [_VP, _VP, C.c_int, C.POINTER(ImageOpts), _VP, C.c_long,
@@ -502,6 +503,11 @@ class Context:
         check(self.lib.doe_field_image(self.h, f.h, index, C.byref(o), out.ctypes.data, out.size,
                                        _dptr(rng)))
         return out, (rng[0], rng[1])
+
+    # ---- random phase ----------------------------------------------
+    def random_phase(self, out: "Field", seed: int = 1) -> None:
+        """out.re = the reference's randomPhase stream of seed (ToUint32, 0 -> 1)."""
+        check(self.lib.doe_random_phase(self.h, out.h, int(seed) & 0xFFFFFFFF))
 
     # ---- fabrication output ----------------------------------------
     def error_diffusion(self, phase: "Field", out: "Field", levels: int = 4, snake: bool = True,

@@ -431,12 +431,19 @@ int doe_image_opts_init(doe_image_opts* opts);
 int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image_opts* opts,
                     unsigned char* out, long out_len, double* range_out);
 
+/* ---- random phase (ref: src/doe.js:65-86 makeRng, randomPhase) --------- *
+ * out re (FP64 or FP32, every point of the batch as one stream) = the
+ * reference's randomPhase(nx, ny * batch, seed): xorshift32 (13, 17, 5) from
+ * seed || 1, sample k = (s / 2^32 * 2 - 1) * pi with s the state after k + 1
+ * steps; equal bit for bit (FP32: rounded once).  im is untouched. */
+int doe_random_phase(doe_ctx* ctx, doe_field* out, unsigned seed);
+
 /* ---- fabrication output (ref: src/doe.js:185-226, 967-976) ------------- *
  * errorDiffusionQuantize of the re plane of every field (FP64, out re gets
  * the wrapped levels; out must not be the input): Floyd–Steinberg with the
  * error taken on the circle, levels uniformly spaced, rows serpentine
  * (snake = 1, the reference's default: each image is one serial chain, the
- * images of a batch run in parallel, one lane each) or all left to right
+ * images of a batch run in parallel, one workgroup each) or all left to right
  * (snake = 0: a diagonal wavefront per image, at most 8192 rows); wrap_first
  * wraps the input first.  Equal to the reference bit for bit.
  * phaseToHeight: out re = ((phase * s) % step + step) % step with
