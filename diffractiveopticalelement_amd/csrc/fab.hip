@@ -10,15 +10,17 @@
 //     opposite to the previous one, so its first sample needs the previous
 //     row complete and every sample of an image is one step of one chain.
 //     Rows of at most 4096 samples: one workgroup per image, the chain on
-//     rows staged in LDS while the other waves stream rows in and out;
+//     rows staged in LDS while the other waves stream rows in and out and
+//     then add each row's shares into the next in parallel;
 //     longer rows: one lane per image, on HBM;
 //   left-to-right rows: sample (i, j) needs (i+1, j-1) and (i-1, j), so row
 //     j can trail row j-1 by two samples — a diagonal wavefront of
 //     nx + 2 (ny - 1) steps per image, one workgroup per image, one barrier
 //     per step.
-// The chains keep the pending shares of the next row in registers (a
-// sample's three shares arrive from three consecutive samples of the row
-// above) and read each sample's original value once.
+// The one-lane and wavefront schedules keep the pending shares of the next
+// row in registers (a sample's three shares arrive from three consecutive
+// samples of the row above); the workgroup schedule gathers them after each
+// row.  Every schedule reads each sample's original value once.
 #include <algorithm>
 #include <cmath>
 
@@ -143,54 +145,50 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
 }
 
 // Serpentine rows of one image per workgroup, rows staged through LDS: three
-// row buffers rotate.  At row j, lane 0 runs the chain on row j (its sums,
-// replaced by the levels) and row j+1 (its wrapped inputs ahead of the scan,
-// its sums behind), while waves 1.. write row j-1's levels (wrapped) out and
-// load row j+2's inputs into the buffer that frees.  One barrier per row.
+// row buffers rotate, plus a row of errors.  At row j, lane 0 runs the chain
+// along row j (its sums, replaced by the levels; the errors into the error
+// row) while waves 1.. write row j-1's levels (wrapped) out and load row
+// j+2's inputs into the buffer that frees; after a barrier every thread adds
+// row j's shares into row j+1 — column m gets, in the order the scan would
+// deliver them, 1/16 of the error behind m, 5/16 of m's, 3/16 of the one
+// ahead (behind and ahead in row j's direction).  Two barriers per row.
 constexpr int kRowThreads = 256;
-constexpr int kRowMax = 4096;  // longest row staged (3 x 32 KiB of LDS)
-constexpr int kRowPad = 8;     // LDS rows padded on both sides: the chain's reads
-                               // and stores past a row's ends land in the padding
+constexpr int kRowMax = 4096;  // longest row staged (4 x 32 KiB of LDS)
+constexpr int kRowPad = 8;     // LDS rows padded on both sides: the chain's block
+                               // reads past a row's end land in the padding
 
-// One serpentine row in scan direction DIR.  pc / pn point at the row's and
-// the next row's first sample in scan order.  The 7/16 share starts as -0
-// (x + -0 == x for every x, so the first sample keeps its value and sign);
-// the share behind the first sample, and the next row's shares on the last
-// row, go to padding or to a free buffer.
+// The chain along one serpentine row in scan direction DIR; pc / pe point at
+// the row's first sample (in scan order) and at its error slot.  The 7/16
+// share starts as -0 (x + -0 == x for every x: the first sample keeps its
+// value and sign).
 template <int DIR>
-__device__ __forceinline__ void chain_row(double* pc, double* pn, int nx, double step, double inv) {
+__device__ __forceinline__ void chain_row(double* pc, double* pe, int nx, double step, double inv) {
   double carry = -0.0;
-  double p_next = pn[0], p_cur = 0;
-  auto sample = [&](int c, double r, double a) {
+  auto sample = [&](int c, double r) {
     double q;
     const double e = quantise(r + carry, step, inv, &q);
     pc[DIR * c] = q;
+    pe[DIR * c] = e;
     carry = e * (7.0 / 16);
-    pn[DIR * (c - 1)] = p_cur + e * (3.0 / 16);
-    p_cur = p_next + e * (5.0 / 16);
-    p_next = a + e * (1.0 / 16);
   };
   int c0 = 0;
   for (; c0 + 8 <= nx; c0 += 8) {
-    double rv[8], av[8];
+    double rv[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      rv[u] = pc[DIR * (c0 + u)];
-      av[u] = pn[DIR * (c0 + u + 1)];
-    }
+    for (int u = 0; u < 8; ++u) rv[u] = pc[DIR * (c0 + u)];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) sample(c0 + u, rv[u], av[u]);
+    for (int u = 0; u < 8; ++u) sample(c0 + u, rv[u]);
   }
-  for (int c = c0; c < nx; ++c) sample(c, pc[DIR * c], pn[DIR * (c + 1)]);
-  pn[DIR * (nx - 1)] = p_cur;
+  for (int c = c0; c < nx; ++c) sample(c, pc[DIR * c]);
 }
 
 __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __restrict__ in, double* __restrict__ out,
                                                              int nx, int ny, double step, double inv,
                                                              int wrap_first) {
-  extern __shared__ double R[];  // 3 rows of nx + 2 kRowPad
+  extern __shared__ double R[];  // 3 rows + the error row, each nx + 2 kRowPad
   const int ld = nx + 2 * kRowPad;
   auto row = [&](int slot) { return R + slot * ld + kRowPad; };
+  double* err = row(3);
   const long n = (long)nx * ny;
   const double* src = in + (long)blockIdx.x * n;
   double* o = out + (long)blockIdx.x * n;
@@ -205,13 +203,13 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
   }
   __syncthreads();
   for (int j = 0; j < ny; ++j) {
+    const int dir = (j & 1) ? -1 : 1;
     if (tid == 0) {
       double* cur = row(j % 3);
-      double* nxt = row((j + 1) % 3);  // past the last row: a free buffer
-      if (j & 1)
-        chain_row<-1>(cur + nx - 1, nxt + nx - 1, nx, step, inv);
+      if (dir == 1)
+        chain_row<1>(cur, err, nx, step, inv);
       else
-        chain_row<1>(cur, nxt, nx, step, inv);
+        chain_row<-1>(cur + nx - 1, err + nx - 1, nx, step, inv);
     } else if (tid >= 64) {
       // the buffer of row j-1 (levels) becomes row j+2's
       double* old = row((j + 2) % 3);
@@ -221,6 +219,18 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
           const double a = src[(long)(j + 2) * nx + k];
           old[k] = wrap_first ? wrap1(a) : a;
         }
+      }
+    }
+    __syncthreads();
+    if (j + 1 < ny) {
+      double* nxt = row((j + 1) % 3);
+      for (int m = tid; m < nx; m += kRowThreads) {
+        double v = nxt[m];
+        const int mb = m - dir, ma = m + dir;  // behind and ahead in row j's scan
+        if (mb >= 0 && mb < nx) v = v + err[mb] * (1.0 / 16);
+        v = v + err[m] * (5.0 / 16);
+        if (ma >= 0 && ma < nx) v = v + err[ma] * (3.0 / 16);
+        nxt[m] = v;
       }
     }
     __syncthreads();
@@ -306,7 +316,7 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
   const double* in = static_cast<const double*>(phase->re);
   double* o = static_cast<double*>(out->re);
   if (snake && nx <= kRowMax) {
-    hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 3 * (nx + 2 * kRowPad) * sizeof(double),
+    hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 4 * (nx + 2 * kRowPad) * sizeof(double),
                        ctx->stream, in, o, nx, ny, step, inv, wrap_first ? 1 : 0);
   } else if (snake) {
     hipLaunchKernelGGL(k_diffuse_lane, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step,

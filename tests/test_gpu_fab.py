@@ -118,6 +118,22 @@ def _gather_fs(x, levels, wrap_first=True):
     return np.vectorize(_wrap)(out)
 
 
+def test_error_diffusion_widest_lds_rows(ctx):
+    """Rows of 4096 samples: the widest the workgroup schedule stages (four
+    LDS rows of 4112 doubles)."""
+    nx, ny, B = 4096, 3, 2
+    rng = np.random.default_rng(13)
+    x = rng.uniform(-5, 5, (B, ny, nx))
+    f = ctx.upload(x, None, "f64")
+    o = ctx.field(nx, ny, B, "f64")
+    ctx.error_diffusion(f, o, 3, True, True)
+    re, _ = o.download()
+    for b in range(B):
+        np.testing.assert_array_equal(re[b], _gather_fs(x[b], 3))
+    f.free()
+    o.free()
+
+
 def test_error_diffusion_long_rows_one_lane_per_image(ctx):
     """Rows longer than 4096 samples take the one-lane-per-image schedule."""
     nx, ny, B = 4100, 3, 2
