@@ -64,14 +64,20 @@ __device__ __forceinline__ double round_quot(double v, double step, double inv) 
   return js_round(v / step);
 }
 
-// One sample: quantise v, return the wrapped error; q is the level.
+// One sample: quantise v, return the wrapped error; q is the level.  With
+// |levels| >= 2 the error is at most |step| / 2 (1 + 2^-50) <= pi / 2 + ulps
+// (NaN and infinite inputs give NaN), so WRAP = false skips a wrap that
+// could not change it.
+template <bool WRAP = true>
 __device__ __forceinline__ double quantise(double v, double step, double inv, double* q) {
   *q = round_quot(v, step, inv) * step;
   double e = v - *q;
-  if (e > M_PI)
-    e -= 2 * M_PI;
-  else if (e <= -M_PI)
-    e += 2 * M_PI;
+  if constexpr (WRAP) {
+    if (e > M_PI)
+      e -= 2 * M_PI;
+    else if (e <= -M_PI)
+      e += 2 * M_PI;
+  }
   return e;
 }
 
@@ -161,12 +167,12 @@ constexpr int kRowPad = 8;     // LDS rows padded on both sides: the chain's blo
 // the row's first sample (in scan order) and at its error slot.  The 7/16
 // share starts as -0 (x + -0 == x for every x: the first sample keeps its
 // value and sign).
-template <int DIR>
+template <int DIR, bool WRAP>
 __device__ __forceinline__ void chain_row(double* pc, double* pe, int nx, double step, double inv) {
   double carry = -0.0;
   auto sample = [&](int c, double r) {
     double q;
-    const double e = quantise(r + carry, step, inv, &q);
+    const double e = quantise<WRAP>(r + carry, step, inv, &q);
     pc[DIR * c] = q;
     pe[DIR * c] = e;
     carry = e * (7.0 / 16);
@@ -206,10 +212,18 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
     const int dir = (j & 1) ? -1 : 1;
     if (tid == 0) {
       double* cur = row(j % 3);
-      if (dir == 1)
-        chain_row<1>(cur, err, nx, step, inv);
-      else
-        chain_row<-1>(cur + nx - 1, err + nx - 1, nx, step, inv);
+      const bool wrap_e = fabs(step) > 3.0;  // |levels| < 2
+      if (dir == 1) {
+        if (wrap_e)
+          chain_row<1, true>(cur, err, nx, step, inv);
+        else
+          chain_row<1, false>(cur, err, nx, step, inv);
+      } else {
+        if (wrap_e)
+          chain_row<-1, true>(cur + nx - 1, err + nx - 1, nx, step, inv);
+        else
+          chain_row<-1, false>(cur + nx - 1, err + nx - 1, nx, step, inv);
+      }
     } else if (tid >= 64) {
       // the buffer of row j-1 (levels) becomes row j+2's
       double* old = row((j + 2) % 3);

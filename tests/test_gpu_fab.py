@@ -35,7 +35,7 @@ def device_fs(ctx, phases, n, levels, snake, wrap_first=True):
 
 @pytest.mark.parametrize("n", [16, 64, 256])
 @pytest.mark.parametrize("snake", [True, False])
-@pytest.mark.parametrize("levels", [2, 4, 3])
+@pytest.mark.parametrize("levels", [2, 4, 3, 1, -3])
 def test_error_diffusion_bit_exact(ctx, n, snake, levels):
     rng = np.random.default_rng(n * 10 + levels)
     phi = rng.uniform(-6, 6, n * n)
