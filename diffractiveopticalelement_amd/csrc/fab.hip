@@ -85,6 +85,7 @@ __device__ __forceinline__ double quantise(double v, double step, double inv, do
 // holds the working values: row j+1's sums are written as its samples
 // complete, and the levels replace the values as the scan passes; wrapped at
 // the end.
+template <bool WRAP>
 __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ in, double* __restrict__ out,
                                                      int nx, int ny, int batch, double step, double inv,
                                                      int snake, int wrap_first) {
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(64) void k_diffuse_lane(const double* __restrict__ 
         // the first sample of a row has no 7/16 share (and keeps a -0)
         const double v = c ? rv[u] + carry : rv[u];
         double q;
-        const double e = quantise(v, step, inv, &q);
+        const double e = quantise<WRAP>(v, step, inv, &q);
         row[i] = q;
         carry = e * (7.0 / 16);
         if (below) {
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
     const int dir = (j & 1) ? -1 : 1;
     if (tid == 0) {
       double* cur = row(j % 3);
-      const bool wrap_e = fabs(step) > 3.0;  // |levels| < 2
+      const bool wrap_e = fabs(step) > 3.0;  // |levels| < 2: the error can reach -pi
       if (dir == 1) {
         if (wrap_e)
           chain_row<1, true>(cur, err, nx, step, inv);
@@ -256,6 +257,7 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
 // Plain (left-to-right) rows on a diagonal wavefront: step t processes sample
 // i = t - 2j of every row j in flight.  One workgroup per image; rows are
 // dealt to threads cyclically; out holds the working values in place.
+template <bool WRAP>
 __global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict__ in, double* __restrict__ out,
                                                        int nx, int ny, double step, double inv,
                                                        int wrap_first) {
@@ -281,7 +283,7 @@ __global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict_
         double* row = o + (long)j * nx;
         const double v = i ? row[i] + carry[q] : row[i];
         double qv;
-        const double e = quantise(v, step, inv, &qv);
+        const double e = quantise<WRAP>(v, step, inv, &qv);
         row[i] = qv;
         carry[q] = e * (7.0 / 16);
         if (j + 1 < ny) {
@@ -326,6 +328,7 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
   const int nx = phase->nx, ny = phase->ny, B = phase->batch;
   const double step = (2 * M_PI) / levels;
   const double inv = 1 / step;
+  const bool wrap_e = std::fabs(step) > 3.0;  // |levels| < 2: the error can reach -pi
   DOE_HIP(hipSetDevice(ctx->device));
   const double* in = static_cast<const double*>(phase->re);
   double* o = static_cast<double*>(out->re);
@@ -333,12 +336,13 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
     hipLaunchKernelGGL(k_diffuse_rows, dim3(B), dim3(kRowThreads), 4 * (nx + 2 * kRowPad) * sizeof(double),
                        ctx->stream, in, o, nx, ny, step, inv, wrap_first ? 1 : 0);
   } else if (snake) {
-    hipLaunchKernelGGL(k_diffuse_lane, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step,
-                       inv, 1, wrap_first ? 1 : 0);
+    auto k = wrap_e ? k_diffuse_lane<true> : k_diffuse_lane<false>;
+    hipLaunchKernelGGL(k, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step, inv, 1,
+                       wrap_first ? 1 : 0);
   } else {
     DOE_REQUIRE(ny <= 8 * 1024, DOE_ERR_UNSUPPORTED, "errorDiffusionQuantize: at most 8192 rows");
-    hipLaunchKernelGGL(k_diffuse_wave, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step, inv,
-                       wrap_first ? 1 : 0);
+    auto k = wrap_e ? k_diffuse_wave<true> : k_diffuse_wave<false>;
+    hipLaunchKernelGGL(k, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step, inv, wrap_first ? 1 : 0);
   }
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));

@@ -134,17 +134,18 @@ def test_error_diffusion_widest_lds_rows(ctx):
     o.free()
 
 
-def test_error_diffusion_long_rows_one_lane_per_image(ctx):
+@pytest.mark.parametrize("levels", [4, 1])
+def test_error_diffusion_long_rows_one_lane_per_image(ctx, levels):
     """Rows longer than 4096 samples take the one-lane-per-image schedule."""
     nx, ny, B = 4100, 3, 2
     rng = np.random.default_rng(12)
     x = rng.uniform(-5, 5, (B, ny, nx))
     f = ctx.upload(x, None, "f64")
     o = ctx.field(nx, ny, B, "f64")
-    ctx.error_diffusion(f, o, 4, True, True)
+    ctx.error_diffusion(f, o, levels, True, True)
     re, _ = o.download()
     for b in range(B):
-        np.testing.assert_array_equal(re[b], _gather_fs(x[b], 4))
+        np.testing.assert_array_equal(re[b], _gather_fs(x[b], levels))
     f.free()
     o.free()
 
