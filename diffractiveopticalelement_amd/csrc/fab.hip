@@ -255,50 +255,71 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
 }
 
 // Plain (left-to-right) rows on a diagonal wavefront: step t processes sample
-// i = t - 2j of every row j in flight.  One workgroup per image; rows are
-// dealt to threads cyclically; out holds the working values in place.
-template <bool WRAP>
-__global__ __launch_bounds__(1024) void k_diffuse_wave(const double* __restrict__ in, double* __restrict__ out,
-                                                       int nx, int ny, double step, double inv,
-                                                       int wrap_first) {
-  const int b = blockIdx.x;
+// i = t - 2j of every row j in flight.  One workgroup per image; row j is
+// thread j mod 1024's (R rows per thread).  Sample (i, j) gathers, in the
+// order the serial scan delivers them, 1/16 of e(i-1, j-1), 5/16 of
+// e(i, j-1), 3/16 of e(i+1, j-1) and 7/16 of e(i-1, j): the thread keeps the
+// row above's two older errors in registers and reads the newest, written
+// one step earlier, from a per-row LDS slot (two slots by step parity).  out
+// holds the wrapped inputs, then the levels, wrapped at the end.
+constexpr int kWaveThreads = 1024;
+constexpr int kWaveRowsMax = 8192;
+template <bool WRAP, int R>
+__global__ __launch_bounds__(kWaveThreads) void k_diffuse_wave(const double* __restrict__ in,
+                                                               double* __restrict__ out, int nx, int ny,
+                                                               double step, double inv, int wrap_first) {
+  __shared__ double eb[2][kWaveRowsMax];
   const long n = (long)nx * ny;
-  const double* src = in + (long)b * n;
-  double* o = out + (long)b * n;
-  for (long k = threadIdx.x; k < n; k += blockDim.x) o[k] = wrap_first ? wrap1(src[k]) : src[k];
+  const double* src = in + (long)blockIdx.x * n;
+  double* o = out + (long)blockIdx.x * n;
+  const int tid = threadIdx.x;
+  for (long k = tid; k < n; k += kWaveThreads) o[k] = wrap_first ? wrap1(src[k]) : src[k];
   __syncthreads();
-  // per owned row: the pending 7/16 share (rows beyond kMaxRows per thread
-  // are not launched: ny <= 8 * blockDim.x)
-  constexpr int kMaxRows = 8;
-  double carry[kMaxRows];
+  // x[r][d]: input of column i + d of the thread's row r (four loads in flight)
+  double carry[R], h1[R], h2[R], x[R][4];
 #pragma unroll
-  for (int q = 0; q < kMaxRows; ++q) carry[q] = 0;
+  for (int r = 0; r < R; ++r) {
+    const int j = tid + r * kWaveThreads;
+    carry[r] = -0.0;  // x + -0 == x: the first sample of a row keeps its value and sign
+    h1[r] = h2[r] = 0;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) x[r][d] = j < ny && d < nx ? o[(long)j * nx + d] : 0;
+  }
   const int steps = nx + 2 * (ny - 1);
   for (int t = 0; t < steps; ++t) {
 #pragma unroll
-    for (int q = 0; q < kMaxRows; ++q) {
-      const int j = threadIdx.x + q * blockDim.x;
+    for (int r = 0; r < R; ++r) {
+      const int j = tid + r * kWaveThreads;
       const int i = t - 2 * j;
-      if (j < ny && i >= 0 && i < nx) {
-        double* row = o + (long)j * nx;
-        const double v = i ? row[i] + carry[q] : row[i];
-        double qv;
-        const double e = quantise<WRAP>(v, step, inv, &qv);
-        row[i] = qv;
-        carry[q] = e * (7.0 / 16);
-        if (j + 1 < ny) {
-          double* nrow = row + nx;
-          // the order of the shares into a sample below matches the serial
-          // scan: (i+1, j+1) gets 1/16 now, (i, j+1) its 5/16, (i-1, j+1) its 3/16
-          if (i + 1 < nx) nrow[i + 1] += e * (1.0 / 16);
-          nrow[i] += e * (5.0 / 16);
-          if (i >= 1) nrow[i - 1] += e * (3.0 / 16);
+      if (j < ny && i >= -1 && i < nx) {
+        // e(i+1, j-1), written by row j-1 at step t-1 (t >= 1 whenever j >= 1)
+        const bool up = j > 0 && i + 1 < nx;
+        const double c = up ? eb[(t - 1) & 1][j - 1] : 0.0;
+        if (i >= 0) {
+          double v = x[r][0];
+          if (j > 0) {
+            if (i >= 1) v = v + h1[r] * (1.0 / 16);
+            v = v + h2[r] * (5.0 / 16);
+            if (up) v = v + c * (3.0 / 16);
+          }
+          v = v + carry[r];
+          double qv;
+          const double e = quantise<WRAP>(v, step, inv, &qv);
+          eb[t & 1][j] = e;
+          o[(long)j * nx + i] = qv;
+          carry[r] = e * (7.0 / 16);
+          x[r][0] = x[r][1];
+          x[r][1] = x[r][2];
+          x[r][2] = x[r][3];
+          if (i + 4 < nx) x[r][3] = o[(long)j * nx + i + 4];
         }
+        h1[r] = h2[r];
+        h2[r] = c;
       }
     }
     __syncthreads();
   }
-  for (long k = threadIdx.x; k < n; k += blockDim.x) o[k] = wrap1(o[k]);
+  for (long k = tid; k < n; k += kWaveThreads) o[k] = wrap1(o[k]);
 }
 
 // phaseToHeight: ((phase * s) % step + step) % step, s = lambda / (2 pi (n - 1)),
@@ -340,9 +361,14 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
     hipLaunchKernelGGL(k, dim3((B + 63) / 64), dim3(64), 0, ctx->stream, in, o, nx, ny, B, step, inv, 1,
                        wrap_first ? 1 : 0);
   } else {
-    DOE_REQUIRE(ny <= 8 * 1024, DOE_ERR_UNSUPPORTED, "errorDiffusionQuantize: at most 8192 rows");
-    auto k = wrap_e ? k_diffuse_wave<true> : k_diffuse_wave<false>;
-    hipLaunchKernelGGL(k, dim3(B), dim3(1024), 0, ctx->stream, in, o, nx, ny, step, inv, wrap_first ? 1 : 0);
+    DOE_REQUIRE(ny <= kWaveRowsMax, DOE_ERR_UNSUPPORTED, "errorDiffusionQuantize: at most 8192 rows");
+    const int rows = (ny + kWaveThreads - 1) / kWaveThreads;  // rows per thread
+    auto k = rows <= 1   ? (wrap_e ? k_diffuse_wave<true, 1> : k_diffuse_wave<false, 1>)
+             : rows <= 2 ? (wrap_e ? k_diffuse_wave<true, 2> : k_diffuse_wave<false, 2>)
+             : rows <= 4 ? (wrap_e ? k_diffuse_wave<true, 4> : k_diffuse_wave<false, 4>)
+                         : (wrap_e ? k_diffuse_wave<true, 8> : k_diffuse_wave<false, 8>);
+    hipLaunchKernelGGL(k, dim3(B), dim3(kWaveThreads), 0, ctx->stream, in, o, nx, ny, step, inv,
+                       wrap_first ? 1 : 0);
   }
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));

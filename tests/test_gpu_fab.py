@@ -85,18 +85,18 @@ def _wrap(p):
     return p
 
 
-def _gather_fs(x, levels, wrap_first=True):
-    """Serpentine Floyd–Steinberg of an ny x nx grid in gather form (each
-    sample pulls its 1/16, 5/16, 3/16 shares from the row above in that row's
-    scan order, then 7/16 of its predecessor); equal to the oracle's push
-    form on square grids (tests/test_fab_cpu.py)."""
+def _gather_fs(x, levels, wrap_first=True, snake=True):
+    """Floyd–Steinberg of an ny x nx grid in gather form (each sample pulls
+    its 1/16, 5/16, 3/16 shares from the row above in that row's scan order,
+    then 7/16 of its predecessor); equal to the oracle's push form on square
+    grids (tests/test_fab_cpu.py)."""
     ny, nx = x.shape
     step = 2 * math.pi / levels
     err = np.zeros((ny, nx))
     out = np.zeros((ny, nx))
     for j in range(ny):
-        d = -1 if j % 2 == 1 else 1
-        du = -1 if (j - 1) % 2 == 1 else 1
+        d = -1 if (snake and j % 2 == 1) else 1
+        du = -1 if (snake and (j - 1) % 2 == 1) else 1
         prev = None
         for c in range(nx):
             i = c if d == 1 else nx - 1 - c
@@ -116,6 +116,20 @@ def _gather_fs(x, levels, wrap_first=True):
                 e += 2 * math.pi
             out[j, i], err[j, i], prev = q, e, e
     return np.vectorize(_wrap)(out)
+
+
+@pytest.mark.parametrize("nx,ny", [(40, 1500), (8, 2100)])
+def test_error_diffusion_plain_rows_tall_images(ctx, nx, ny):
+    """More rows than the wavefront's 1024 threads: 2 and 4 rows per thread."""
+    rng = np.random.default_rng(ny)
+    x = rng.uniform(-5, 5, (1, ny, nx))
+    f = ctx.upload(x, None, "f64")
+    o = ctx.field(nx, ny, 1, "f64")
+    ctx.error_diffusion(f, o, 4, False, True)
+    re, _ = o.download()
+    np.testing.assert_array_equal(re[0], _gather_fs(x[0], 4, snake=False))
+    f.free()
+    o.free()
 
 
 def test_error_diffusion_widest_lds_rows(ctx):
