@@ -800,6 +800,12 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
   const int nx = out->nx, ny = out->ny, B = out->batch;
   std::vector<TgtRec> recs(B);
   std::vector<double> rasters;
+  struct RasterRef {
+    const double* p;
+    int w, h;
+    long off;
+  };
+  std::vector<RasterRef> raster_refs;
   std::vector<unsigned char> dark;
   const double mn = std::min((double)nx, (double)ny);
   for (int b = 0; b < B; ++b) {
@@ -866,8 +872,15 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
         R.gamma = S.gamma;
         R.background = S.background;
         R.gain = S.gain;
-        R.roff = (long)rasters.size();
-        rasters.insert(rasters.end(), S.raster, S.raster + (size_t)S.raster_w * S.raster_h);
+        // specs that share a host raster (same array and size) share one copy
+        R.roff = -1;
+        for (const RasterRef& u : raster_refs)
+          if (u.p == S.raster && u.w == S.raster_w && u.h == S.raster_h) R.roff = u.off;
+        if (R.roff < 0) {
+          R.roff = (long)rasters.size();
+          raster_refs.push_back({S.raster, S.raster_w, S.raster_h, R.roff});
+          rasters.insert(rasters.end(), S.raster, S.raster + (size_t)S.raster_w * S.raster_h);
+        }
         break;
       }
       default: set_error("makeTarget: unknown kind"); return DOE_ERR_INVALID;

@@ -130,6 +130,25 @@ def test_mixed_batch_equals_single_calls(ctx):
         np.testing.assert_array_equal(im_b[q], im1[0])
 
 
+def test_specs_sharing_a_raster_equal_single_calls(ctx):
+    """Specs of one call that pass the same host raster share one device copy;
+    each still equals its own single call (different gain, fit, gamma; a
+    second raster in between)."""
+    nx, ny = 120, 88
+    rng = np.random.default_rng(21)
+    img = np.ascontiguousarray(rng.random((23, 31)))
+    txt = text_raster("Q")
+    specs = [ctx.target_spec("raster", raster=img), ctx.target_spec("raster", raster=txt),
+             ctx.target_spec("raster", raster=img, gain=2.0, cover=1),
+             ctx.target_spec("raster", raster=img, gamma=2.2, background=0.1),
+             ctx.target_spec("raster", raster=txt, cover=1)]
+    re_b, im_b = device_target(ctx, specs, nx, ny)
+    for q, sp in enumerate(specs):
+        re1, im1 = device_target(ctx, [sp], nx, ny)
+        np.testing.assert_array_equal(re_b[q], re1[0])
+        np.testing.assert_array_equal(im_b[q], im1[0])
+
+
 # (8, 8): the largest halo of the tiled ROI kernel; (9, 8) and (40, 7): the separable passes
 @pytest.mark.parametrize("dilate,soft,threshold", [(2, 2, 0.05), (0.3, 0, 0.05), (4.5, 3, 0.2), (8, 8, 0.05),
                                                    (9, 8, 0.05), (16, 0, 0.05), (40, 7, 0.01)])
