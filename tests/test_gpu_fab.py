@@ -118,9 +118,9 @@ def _gather_fs(x, levels, wrap_first=True, snake=True):
     return np.vectorize(_wrap)(out)
 
 
-@pytest.mark.parametrize("nx,ny", [(40, 1500), (8, 2100)])
+@pytest.mark.parametrize("nx,ny", [(40, 1500), (8, 2100), (4, 5000)])
 def test_error_diffusion_plain_rows_tall_images(ctx, nx, ny):
-    """More rows than the wavefront's 1024 threads: 2 and 4 rows per thread."""
+    """More rows than the wavefront's 1024 threads: 2, 4 and 8 rows per thread."""
     rng = np.random.default_rng(ny)
     x = rng.uniform(-5, 5, (1, ny, nx))
     f = ctx.upload(x, None, "f64")
