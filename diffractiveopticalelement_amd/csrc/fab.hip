@@ -344,7 +344,14 @@ int doe_error_diffusion(doe_ctx* ctx, const doe_field* phase, doe_field* out, in
               "errorDiffusionQuantize: FP64 fields");
   DOE_REQUIRE(phase->nx == out->nx && phase->ny == out->ny && phase->batch == out->batch, DOE_ERR_SHAPE,
               "errorDiffusionQuantize: phase and output grids differ");
-  DOE_REQUIRE(phase->re != out->re, DOE_ERR_INVALID, "errorDiffusionQuantize: output overlaps the phase");
+  {
+    // the schedules read input rows ahead of the rows they write
+    const char* a = static_cast<const char*>(phase->re);
+    const char* b = static_cast<const char*>(out->re);
+    const size_t bytes = phase->count() * sizeof(double);
+    DOE_REQUIRE(a + bytes <= b || b + bytes <= a, DOE_ERR_INVALID,
+                "errorDiffusionQuantize: output overlaps the phase");
+  }
   DOE_REQUIRE(levels != 0, DOE_ERR_INVALID, "errorDiffusionQuantize: levels must be non-zero");
   const int nx = phase->nx, ny = phase->ny, B = phase->batch;
   const double step = (2 * M_PI) / levels;

@@ -211,6 +211,13 @@ def test_argument_errors(ctx):
         ctx.error_diffusion(f, g)
     with pytest.raises(nat.DoeError, match="overlaps"):
         ctx.error_diffusion(f, f)
+    # an output that starts one row into the phase overlaps it too
+    shifted = ctx.wrap(f.ptrs()[0] + 8 * 8, f.ptrs()[1] + 8 * 8, 8, 7, 1, "f64")
+    head = ctx.wrap(f.ptrs()[0], f.ptrs()[1], 8, 7, 1, "f64")
+    with pytest.raises(nat.DoeError, match="overlaps"):
+        ctx.error_diffusion(head, shifted)
+    shifted.free()
+    head.free()
     with pytest.raises(nat.DoeError, match="non-zero"):
         ctx.error_diffusion(f, ctx.field(8, 8, 1, "f64"), levels=0)
     f.free()
