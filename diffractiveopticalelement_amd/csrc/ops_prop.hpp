@@ -123,6 +123,13 @@ struct OpAsmH {
     const double fy = double(sy) * dfy;
     double w = 1.0;
     if (p.band) w = soft_window(fabs(fx), p.lim_x, p.inner_x) * soft_window(fabs(fy), p.lim_y, p.inner_y);
+    // Outside the band limit H is 0 whatever the phase: leave before the
+    // square root and the trig (at long distances most of the spectrum).
+    if (w == 0.0) {
+      hr = T(0);
+      hi = T(0);
+      return;
+    }
     const double arg = inv_l2 - fx * fx - fy * fy;
     double phase;
     if (arg >= 0.0) {
@@ -133,11 +140,6 @@ struct OpAsmH {
       return;
     } else {
       phase = p.two_pi_z * sqrt(-arg);
-    }
-    if (w == 0.0) {
-      hr = T(0);
-      hi = T(0);
-      return;
     }
     T c, s;
     cis<T>(phase, c, s);
