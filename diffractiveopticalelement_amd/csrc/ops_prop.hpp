@@ -197,9 +197,15 @@ struct OpChirp {
       const double py = (mode & CH_OUTGRID) ? p.dy_out : p.dy_in;
       const double x = double(i - (nx >> 1)) * px;  // (i - nx/2) * dx
       const double y = double(j - (ny >> 1)) * py;
+      // FP64 fields keep propagate.js:189's division; FP32 fields multiply by
+      // the reciprocal (one rounding apart: below 1e-11 rad at the largest
+      // phase of a 2048^2 grid, far under FP32's resolution) and skip the
+      // division's reciprocal-and-refine sequence at every point.
       double q;
-      if constexpr (true) {
+      if constexpr (sizeof(T) == 8) {
         q = (kPi * (x * x + y * y)) / p.laz;
+      } else {
+        q = (kPi * (x * x + y * y)) * p.inv_laz;
       }
       if (mode & CH_MINUS_HALF_PI) q = q - kPi / 2;
       cis<T>(q, cr, ci);
