@@ -193,7 +193,12 @@ struct OpFFB {
     }
     const T tp = m > T(0) ? t : a;
     if (a > T(0)) {
-      const T s = tp / a;
+      // FP32: hardware reciprocal (about 1 ulp) instead of the division sequence
+      T s;
+      if constexpr (sizeof(T) == 4)
+        s = tp * __builtin_amdgcn_rcpf(a);
+      else
+        s = tp / a;
       re *= s;
       im *= s;
     } else {
@@ -261,10 +266,21 @@ struct OpFFA {
     const T cb = ((x + y) & 1) ? T(-1) : T(1);
     const bool wp = b.weighted ? (b.state[f].improved != 0) : (write_phase != 0);
     if (wp) b.phase[0][k] = atan2(cb * im, cb * re);
-    const T r = sqrt(re * re + im * im);
     const T il = ax.il;
-    if (r > T(0)) {
-      const T s = il / r;
+    T s = T(0);
+    bool nz;
+    if constexpr (sizeof(T) == 4) {
+      // FP32: one reciprocal square root (about 1 ulp) instead of a
+      // correctly rounded square root and division per point
+      const T q = re * re + im * im;
+      nz = q > T(0);
+      if (nz) s = il * rsqrtf(q);
+    } else {
+      const T r = sqrt(re * re + im * im);
+      nz = r > T(0);
+      if (nz) s = il / r;
+    }
+    if (nz) {
       re *= s;
       im *= s;
     } else {
