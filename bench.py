@@ -67,34 +67,74 @@ B, N = 16, 2048
 DX, LAM = 4e-6, 633e-9
 
 
-def cpu_baseline(n: int = 2048, reps: int = 8) -> dict | None:
-    """Oracle fft2d (FP64, radix-2, one node thread) on `reps` 2048^2 transforms."""
+C1_CFG = {"n": 256, "pitch": 8e-6, "lambda": 532e-9, "distance": 0.03, "mode": "image-plane",
+          "algorithm": "wgs", "iterations": 100, "seed": 0, "target": {"kind": "text", "text": "DOE"}}
+
+
+def host_info() -> dict:
+    """CPU model, core count and node version of the host running the baseline."""
+    model = platform.processor() or platform.machine()
+    try:
+        for ln in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    node = None
+    if shutil.which("node"):
+        node = subprocess.run(["node", "--version"], capture_output=True, text=True).stdout.strip()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "node": node,
+            "pinned": "taskset -c 0" if shutil.which("taskset") else None}
+
+
+def cpu_baseline(n: int = 2048, reps: int = 4) -> dict | None:
+    """The reference's node path (the FP64 JavaScript oracle: radix-2 FFT,
+    per-call transfer function, shifts) on ONE host core (taskset -c 0), on a
+    bounded sample of each workload: fft2d 2048^2 (the metric), band-limited
+    ASM and Fresnel at 2048^2, one far-field WGS iteration at 1024^2, the
+    small reference configuration (256^2, 100 WGS iterations, simulate +
+    screenMetrics), target + ROI and planeToRGB at 1024^2."""
     if shutil.which("node") is None:
         return None
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_js as oj  # the oracle is the CPU baseline's code, nothing else
 
-    _, res = oj.run("time", {"what": "fft2d", "n": n, "reps": reps}, timeout=900)
+    def t(args, timeout=900):
+        return oj.run("time", args, timeout=timeout, pin_cpu=0)[1]
+
+    res = t({"what": "fft2d", "n": n, "reps": reps})
     ms = res["ms"]
     gbs = 32 * n * n / (ms * 1e-3) / 1e9
-    # the target / figure leg's host counterparts (one node thread)
-    _, rt = oj.run("time", {"what": "target", "kind": "text", "n": 1024, "reps": 2}, timeout=600)
-    _, ri = oj.run("time", {"what": "image", "n": 1024, "reps": 2,
-                            "opts": {"map": "viridis", "percentile": 0.99}}, timeout=600)
+    ra = t({"what": "asm", "n": n, "reps": 1, "dx": DX, "lambda": LAM, "z": 0.055})
+    rf = t({"what": "fresnel", "n": n, "reps": 1, "dx": DX, "lambda": LAM, "z": 0.055})
+    rw = t({"what": "wgs_ff", "n": 1024, "reps": 1, "iterations": 1})
+    _, rc1 = oj.run("projector", {"cfg": C1_CFG}, timeout=900, pin_cpu=0)
+    rt = t({"what": "target", "kind": "text", "n": 1024, "reps": 2})
+    ri = t({"what": "image", "n": 1024, "reps": 2, "opts": {"map": "viridis", "percentile": 0.99}})
     return {
         "value": round(gbs, 4),
         "unit": "GB/s",
         "cores": 1,
         "kind": "port",
         "sample": (f"{reps} forward fft2d of {n}x{n} complex FP64 by the JS oracle (radix-2 DIT, "
-                   f"strided columns) on one node {res.get('node')} thread, 1 warm-up; "
+                   f"strided columns) on one node {res.get('node')} thread pinned to core 0, 1 warm-up; "
                    f"{ms:.1f} ms per transform; GB/s counts the same 32 B/pt as the GPU value"),
-        "host": platform.processor() or platform.machine(),
-        "targets": {"make_target_text_1024sq_ms": round(rt["ms"], 2),
-                    "plane_to_rgb_p99_1024sq_ms": round(ri["ms"], 2),
-                    "sample": "makeTarget('text') with its autoRoi, and planeToRGB (viridis, "
-                              "percentile 0.99), each on a 1024^2 grid by the JS oracle on one "
-                              "node thread, 2 reps after 1 warm-up"},
+        "host": host_info(),
+        "per_unit_ms": {
+            "fft2d_2048sq": round(ms, 1),
+            "asm_bandlimited_2048sq": round(ra["ms"], 1),
+            "fresnel_2048sq": round(rf["ms"], 1),
+            "wgs_far_field_iteration_1024sq": round(rw["ms"], 1),
+            "c1_design_100_iterations_256sq": round(rc1["design_ms"], 1),
+            "c1_simulate_screen_metrics_256sq": round(rc1["simulate_metrics_ms"], 1),
+            "make_target_text_1024sq": round(rt["ms"], 2),
+            "plane_to_rgb_p99_1024sq": round(ri["ms"], 2),
+        },
+        "per_unit_sample": ("one unit each (1 warm-up + 1 timed; fft2d 4, targets/images 2), JS oracle on "
+                            "one pinned node thread; the batched configurations extrapolate linearly: "
+                            "16 x 2048^2 ASM fwd + inv + Fresnel = 16 x (2 asm + 1 fresnel); "
+                            "C1 design = 100 WGS iterations at 256^2 (ProjectorSystem, seed 0)"),
+        "c1_metrics": {k: rc1["metrics"][k] for k in ("rmse", "efficiency", "correlation", "ssim")},
     }
 
 
@@ -139,6 +179,144 @@ def storage_variants(ctx, field, steps: int) -> dict:
                    "bytes_per_pt": 8 * size, "tuning": ctx.tune_get(N, N, B, dt)}
         g.free()
     return out
+
+
+def _rel(a, b) -> float:
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def dft_crop(x: np.ndarray, k0: int, k1: int, size: int) -> np.ndarray:
+    """Naive DFT (the reference's dft2dNaive convention, exp(-2 pi i n k / N),
+    unscaled) of an ny x nx grid at the output rows k0..k0+size and columns
+    k1..k1+size, in FP64: separable partial sums with exactly reduced
+    twiddle angles (n k mod N)."""
+    ny, nx = x.shape
+    ky = np.arange(k0, k0 + size)
+    kx = np.arange(k1, k1 + size)
+    wy = np.exp(-2j * np.pi * (np.outer(ky, np.arange(ny)) % ny) / ny)
+    wx = np.exp(-2j * np.pi * (np.outer(np.arange(nx), kx) % nx) / nx)
+    return wy @ (x @ wx)
+
+
+def errors_leg(ctx, field, zs, picks=(0, 15)) -> dict:
+    """Error of the timed FP32 results against the FP64 path, on fields
+    `picks` of the batch (their FP32 values promoted to FP64 on both sides):
+    fft2d vs the device's FP64 check mode, numpy's FP64 FFT and a naive DFT
+    on a 256^2 crop; band-limited ASM forward (+ inverse) and Fresnel vs the
+    FP64 check mode; FP64 check mode vs numpy; Parseval of the Fresnel
+    transform (FP32)."""
+    xs = []
+    for b in picks:
+        v = ctx.wrap(field.ptrs()[0] + b * N * N * 4, field.ptrs()[1] + b * N * N * 4, N, N, 1, "f32")
+        re, im = v.download()
+        v.free()
+        xs.append(re[0] + 1j * im[0])
+    x = np.stack(xs)
+    z = np.asarray([zs[b] for b in picks])
+    out = {"fields": list(picks), "z_m": [round(float(v), 6) for v in z]}
+
+    def run(dtype, op):
+        f = ctx.upload(x.real, x.imag, dtype)
+        g = ctx.field(N, N, len(picks), dtype)
+        op(f, g)
+        ctx.sync()
+        re, im = g.download()
+        f.free()
+        g.free()
+        return re + 1j * im
+
+    def fft_op(f, g):
+        ctx.lib.doe_field_copy(ctx.h, f.h, g.h)
+        ctx.fft2d(g)
+
+    y32 = run("f32", fft_op)
+    y64 = run("f64", fft_op)
+    ynp = np.fft.fft2(x)
+    crop = dft_crop(x[0], 896, 896, 256)  # centre of the spectrum
+    out["fft2d"] = {"f32_vs_f64_device": _rel(y32, y64), "f32_vs_numpy_f64": _rel(y32, ynp),
+                    "f64_device_vs_numpy": _rel(y64, ynp),
+                    "f32_vs_naive_dft_crop256": _rel(y32[0, 896:1152, 896:1152], crop),
+                    "f64_device_vs_naive_dft_crop256": _rel(y64[0, 896:1152, 896:1152], crop)}
+    del y32, y64, ynp
+
+    def asm_rt(f, g):
+        ctx.asm(f, g, DX, DX, LAM, z)
+    a32 = run("f32", asm_rt)
+    a64 = run("f64", asm_rt)
+
+    def asm_back(f, g):
+        ctx.asm(f, g, DX, DX, LAM, z)
+        ctx.asm(g, g, DX, DX, LAM, -z)
+    b32 = run("f32", asm_back)
+    b64 = run("f64", asm_back)
+    out["asm_band_limited"] = {"forward_f32_vs_f64_device": _rel(a32, a64),
+                               "round_trip_f32_vs_f64_device": _rel(b32, b64),
+                               "round_trip_f64_vs_input": _rel(b64, x)}
+    del a32, a64, b32, b64
+
+    def fres(f, g):
+        ctx.fresnel(f, g, DX, DX, LAM, z)
+    r32 = run("f32", fres)
+    r64 = run("f64", fres)
+    odx = LAM * np.abs(z) / (N * DX)
+    p_in = np.sum(np.abs(x) ** 2, axis=(1, 2)) * DX * DX
+    p_out = np.sum(np.abs(r32) ** 2, axis=(1, 2)) * odx * odx
+    out["fresnel"] = {"f32_vs_f64_device": _rel(r32, r64),
+                      "parseval_f32_max_abs_dev": float(np.max(np.abs(p_out / p_in - 1)))}
+    for k, v in out.items():
+        if isinstance(v, dict):
+            out[k] = {kk: (float(f"{vv:.3e}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
+    out["note"] = ("relative L2 over the picked fields; the device FP64 check mode is pinned to the FP64 "
+                   "oracle (<= 1e-12) by the tests; gates: FP32 <= 1e-5, Parseval <= 1e-6")
+    return out
+
+
+C1_NODE = r"""
+import * as d from './diffractiveopticalelement_amd/js/index.js';
+d.setPrecision('f32');
+const cfg = __CFG__;
+const now = () => Number(process.hrtime.bigint()) / 1e6;
+const sys = new d.ProjectorSystem(cfg);
+let r = sys.design();
+let m = sys.screenMetrics(sys.simulate());
+const td = [], ts = [];
+for (let i = 0; i < __REPS__; i++) {
+  const a = now();
+  r = sys.design();
+  const b = now();
+  m = sys.screenMetrics(sys.simulate());
+  ts.push(now() - b);
+  td.push(b - a);
+}
+console.log(JSON.stringify({ design_ms: td, simulate_metrics_ms: ts, rmse: m.rmse, efficiency: m.efficiency,
+  correlation: m.correlation, ssim: m.ssim, final_error: r.error[r.error.length - 1] }));
+"""
+
+
+def c1_leg(reps: int = 5) -> dict | None:
+    """The small reference configuration through the JavaScript drop-in
+    (ProjectorSystem -> N-API addon -> this library): design (100 image-plane
+    WGS iterations at 256^2, seed 0, text 'DOE'), then simulate +
+    screenMetrics; wall-clock per call, host arrays in and out (the drop-in's
+    contract), 1 warm-up call."""
+    if shutil.which("node") is None:
+        return None
+    src = C1_NODE.replace("__CFG__", json.dumps(C1_CFG)).replace("__REPS__", str(reps))
+    proc = subprocess.run(["node", "--input-type=module", "-e", src], cwd=str(ROOT), capture_output=True,
+                          text=True, timeout=600)
+    if proc.returncode != 0:
+        return {"error": proc.stderr[-1500:]}
+    r = json.loads(proc.stdout.strip().splitlines()[-1])
+    return {
+        "config": {"workload": "ProjectorSystem n 256, pitch 8 um, 532 nm, z 30 mm, image-plane WGS, "
+                               "100 iterations, seed 0, target text 'DOE'; FP32 on the device",
+                   "path": "JS drop-in -> N-API -> libdoehip"},
+        "design_ms_median": round(float(np.median(r["design_ms"])), 2),
+        "simulate_metrics_ms_median": round(float(np.median(r["simulate_metrics_ms"])), 2),
+        "design_ms": [round(v, 2) for v in r["design_ms"]],
+        "metrics": {k: r[k] for k in ("rmse", "efficiency", "correlation", "ssim")},
+        "reference_published": "4.5 s for 60 iterations at 256^2 on 2 vCPU (README.md:204)",
+    }
 
 
 def c3_targets(n: int, rng, polygon: bool) -> np.ndarray:
@@ -364,6 +542,8 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the fft2d autotuner (built-in configuration)")
     ap.add_argument("--no-wgs", action="store_true", help="skip the WGS warm-start leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the small reference configuration leg")
+    ap.add_argument("--no-errors", action="store_true", help="skip the error statements")
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven GS leg")
     ap.add_argument("--no-targets", action="store_true",
                     help="skip the target / figure and fabrication legs")
@@ -409,20 +589,34 @@ def main() -> int:
         rep_t = ctx.autotune_fft2d(N, N, B, "f32", reps=2)
         tuning = ctx.tune_get(N, N, B, "f32")
         tuning["candidates"] = len(rep_t.strip().splitlines()) - 1
+    # steps alternate forward and inverse transforms (the same traffic), so
+    # the batch stays finite and an even number of steps returns it to the
+    # input: the round trip after the timed loop is the run's checksum
+    v0 = ctx.wrap(field.ptrs()[0], field.ptrs()[1], N, N, 1, "f32")
+    x0 = np.concatenate(v0.download())
     ctx.set_flags(asynchronous=True, profile=False)
-    for _ in range(args.warmup):
-        ctx.fft2d(field)
+    for i in range(args.warmup):
+        ctx.fft2d(field, inverse=bool(i & 1))
     ctx.sync()
     ctx.set_flags(asynchronous=True, profile=True)
     t0 = time.perf_counter()
     ctx.timer_start()
-    for _ in range(args.steps):
-        ctx.fft2d(field)
+    for i in range(args.steps):
+        ctx.fft2d(field, inverse=bool((args.warmup + i) & 1))
     ms_events = ctx.timer_stop()
     t1 = time.perf_counter()
     rep = ctx.profile_report()
     ms_step = ms_events / args.steps
     value = 32 * pts / (ms_step * 1e-3) / 1e9
+    ctx.set_flags(asynchronous=True, profile=False)
+    if (args.warmup + args.steps) & 1:
+        ctx.fft2d(field, inverse=True)  # close the last round trip (untimed)
+    ctx.sync()
+    x1 = np.concatenate(v0.download())
+    v0.free()
+    checksum = {"round_trips": (args.warmup + args.steps + 1) // 2,
+                "field0_rel_l2_vs_input": float(f"{_rel(x1, x0):.3e}"),
+                "all_finite": bool(np.all(np.isfinite(x1)))}
 
     kernels = {}
     for tag, (cnt, tot, kpts) in rep.items():
@@ -459,11 +653,13 @@ def main() -> int:
     ctx.set_flags(asynchronous=True, profile=False)
     ctx.sync()
     scratch.free()
+    errors = None if args.no_errors else errors_leg(ctx, field, zs)
     variants = storage_variants(ctx, field, args.steps)
     field.free()
     c3 = None if args.no_c3 else c3_leg(ctx)
     targets = None if args.no_targets else targets_leg(ctx)
     fab = None if args.no_targets else fab_leg(ctx)
+    c1 = None if args.no_c1 else c1_leg()
     wgs = None if args.no_wgs else wgs_leg(ctx, args.wgs_train, args.wgs_steps, args.wgs_eval,
                                            args.wgs_iters)
 
@@ -484,8 +680,11 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (Gaussian amplitude x seeded uniform phase, seed 0)",
         "config": {"workload": "batched propagation: 16 x 2048^2 complex FP32, pitch 4 um, "
-                               "lambda 633 nm, z ~ U[10,100] mm; step = fft2d forward",
+                               "lambda 633 nm, z ~ U[10,100] mm; step = one fft2d of the batch "
+                               "(forward and inverse alternate)",
                    "batch": B, "n": N},
+        "checksum": checksum,
+        "errors": errors,
         "roofline": {
             "bound": "hbm",
             "kernel": dom_tag,
@@ -504,6 +703,7 @@ def main() -> int:
         "gs_batch_uneven": c3,
         "targets": targets,
         "fabrication": fab,
+        "c1_projector": c1,
         "wgs": wgs,
         "cpu_baseline": cpu,
     }

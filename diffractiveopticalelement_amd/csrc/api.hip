@@ -127,7 +127,7 @@ int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev) {
   return DOE_OK;
 }
 
-int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf) {
+int ensure_reduce(doe_ctx* ctx, size_t n) {
   if (n > ctx->partials_cap) {
     DOE_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->partials) DOE_HIP(hipFree(ctx->partials));
@@ -135,15 +135,6 @@ int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf) {
     while (cap < n) cap <<= 1;
     DOE_HIP(hipMalloc((void**)&ctx->partials, cap * sizeof(double)));
     ctx->partials_cap = cap;
-  }
-  if (nf > ctx->counters_cap) {
-    DOE_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->counters) DOE_HIP(hipFree(ctx->counters));
-    size_t cap = 64;
-    while (cap < nf) cap <<= 1;
-    DOE_HIP(hipMalloc((void**)&ctx->counters, cap * sizeof(unsigned)));
-    DOE_HIP(hipMemset(ctx->counters, 0, cap * sizeof(unsigned)));
-    ctx->counters_cap = cap;
   }
   return DOE_OK;
 }
@@ -370,7 +361,6 @@ int doe_ctx_destroy(doe_ctx* ctx) {
   if (ctx->dparams) hipFree(ctx->dparams);
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->partials) hipFree(ctx->partials);
-  if (ctx->counters) hipFree(ctx->counters);
   if (ctx->work_re) hipFree(ctx->work_re);
   if (ctx->work_im) hipFree(ctx->work_im);
   for (auto& r : ctx->prof) {

@@ -82,11 +82,9 @@ struct doe_ctx {
   // staging for FP64 host <-> device conversion
   void* stage = nullptr;
   size_t stage_cap = 0;
-  // reduction scratch: partials + per-field arrival counters (zeroed once)
+  // reduction scratch: per-workgroup partial sums of the reducing passes
   double* partials = nullptr;
   size_t partials_cap = 0;  // doubles
-  unsigned* counters = nullptr;
-  size_t counters_cap = 0;
   // work planes for passes that permute rows (split column passes)
   void* work_re = nullptr;
   void* work_im = nullptr;
@@ -116,8 +114,8 @@ int get_twiddle(doe_ctx* ctx, int n, int tsize, Twiddle* out);
 // Ensure a device parameter block of `bytes` and upload `host` into it on the
 // context stream (synchronous w.r.t. the host buffer).
 int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev);
-// Ensure `n` doubles of partials and `nf` zeroed counters.
-int ensure_reduce(doe_ctx* ctx, size_t n, size_t nf);
+// Ensure `n` doubles of reduction partials (ctx->partials).
+int ensure_reduce(doe_ctx* ctx, size_t n);
 int ensure_stage(doe_ctx* ctx, size_t bytes);
 // Ensure two work planes of `bytes` each (ctx->work_re / work_im).
 int ensure_work(doe_ctx* ctx, size_t bytes);

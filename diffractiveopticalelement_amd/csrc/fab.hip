@@ -44,11 +44,13 @@ __device__ __forceinline__ double wrap1(double p) {
   return p;
 }
 
-// Math.round: nearest integer, ties toward +Infinity.
+// Math.round: nearest integer, ties toward +Infinity; a zero result keeps
+// the sign of x (Math.round returns -0 for x in [-0.5, -0]).
 __device__ __forceinline__ double js_round(double x) {
   if (x != x || x == INFINITY || x == -INFINITY) return x;
   const double f = floor(x);
-  return (x - f >= 0.5) ? f + 1 : f;
+  const double r = (x - f >= 0.5) ? f + 1 : f;
+  return r == 0 ? copysign(0.0, x) : r;
 }
 
 // Math.round(v / step) with the quotient taken as v * inv (inv = 1 / step):
@@ -60,7 +62,10 @@ __device__ __forceinline__ double round_quot(double v, double step, double inv) 
   const double y = v * inv;
   const double f = floor(y);
   const double d = (y - f) - 0.5;
-  if (fabs(d) > 0x1p-40 * fmax(1.0, fabs(y))) return d >= 0 ? f + 1 : f;
+  if (fabs(d) > 0x1p-40 * fmax(1.0, fabs(y))) {
+    const double r = d >= 0 ? f + 1 : f;
+    return r == 0 ? copysign(0.0, y) : r;
+  }
   return js_round(v / step);
 }
 

@@ -107,7 +107,6 @@ struct OpIPMP {
   static constexpr bool HAS_PRE = false, HAS_POST = false;
   GsBuf<T> b;
   double* partials;
-  unsigned* counters;
   int it;
   template <typename U>
   __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
@@ -127,7 +126,6 @@ struct OpIPProj {
   static constexpr bool HAS_PRE = true, HAS_POST = false;
   GsBuf<T> b;
   double* partials;
-  unsigned* counters;
   int it;
   __device__ __forceinline__ void pre(int, int x, int y, T& re, T& im, double* acc) const {
     ip_measure_project<T>(b, it, (long)y * b.nx + x, re, im, acc);
@@ -148,7 +146,6 @@ struct OpIPMeas {
   static constexpr bool HAS_PRE = false, HAS_POST = true;
   GsBuf<T> b;
   double* partials;
-  unsigned* counters;
   int prev_it;
   template <typename U>
   __device__ __forceinline__ void pre(int, int, int, U&, U&, double*) const {}
@@ -327,7 +324,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   if (rc) return rc;
   // reduction scratch: partials per workgroup (row or column pass), one
   // arrival counter per field
-  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF, 1);
+  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF);
   if (rc) return rc;
   T* re = W.fre;
   T* im = W.fim;
@@ -339,7 +336,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ff_init_row", op0, re, im, re, im, 1,
                                                             pl, sz)));
     for (int it = 0; it < K; ++it) {
-      OpFFB<T> opb{b, ctx->partials, ctx->counters, it, it == K - 1 ? 1 : 0};
+      OpFFB<T> opb{b, ctx->partials, it, it == K - 1 ? 1 : 0};
       DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, 1, pl, sz)));
       OpFFA<T> opa{b, it == K - 1 ? 1 : 0};
       DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ff_row_phase", opa, re, im, re, im, 1,
@@ -372,14 +369,14 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
     for (int it = 0; it < K; ++it) {
       if (!weighted || it == 0) {
-        OpIPMP<T> op{b, ctx->partials, ctx->counters, it};
+        OpIPMP<T> op{b, ctx->partials, it};
         DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_project", op, re, im, re, im,
                                                              1, pl, sz)));
       } else {
-        OpIPMeas<T> om{b, ctx->partials, ctx->counters, it - 1};
+        OpIPMeas<T> om{b, ctx->partials, it - 1};
         DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re,
                                                               im, 1, pl, sz)));
-        OpIPProj<T> op{b, ctx->partials, ctx->counters, it};
+        OpIPProj<T> op{b, ctx->partials, it};
         DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_row_reproject", op, re, im, re,
                                                                im, 1, pl, sz)));
       }
@@ -391,7 +388,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     }
     // closing measurement: GS final forward amplitude (doe.js:418-426); WGS
     // best check + weight update of the last iteration.
-    OpIPMeas<T> om{b, ctx->partials, ctx->counters, K - 1};
+    OpIPMeas<T> om{b, ctx->partials, K - 1};
     DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re, im, 1,
                                                           pl, sz)));
   }
@@ -520,7 +517,7 @@ int run_gs_multiplane(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum
   Plan2 pl;
   rc = make_plan2(ctx, nx, ny, sizeof(T), col_split<T>(nx, ny), &pl);
   if (rc) return rc;
-  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF, 1);
+  rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF);
   if (rc) return rc;
   T* re = W.fre;
   T* im = W.fim;
@@ -549,7 +546,7 @@ int run_gs_multiplane(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum
       bq.tgt = tgt_d + size_t(q) * n;
       bq.roi = roi_d + size_t(q) * n;
       bq.err_hist = W.hist + hk * size_t(q);
-      OpIPMP<T> op{bq, ctx->partials, ctx->counters, it};
+      OpIPMP<T> op{bq, ctx->partials, it};
       DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_project", op, re, im, re, im, 1,
                                                            pl, sz)));
       DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_hconj", hfor(q, 1), re, im, re, im, 1, pl,

@@ -142,7 +142,7 @@ hipError_t gsb_iterations(doe_gs_batch* B, int K) {
     for (auto& c : B->cls) {
       float* re = B->re + c.off;
       float* im = B->im + c.off;
-      OpFFB<float> opb{c.b, c.partials, c.counters, it, last};
+      OpFFB<float> opb{c.b, c.partials, it, last};
       const hipError_t e = col_pass<float, float, 1, false, true>(
           ctx, gsb_tag(c.n, 0), opb, re, im, re, im, int(c.users.size()), c.pl, GsSizes{});
       if (e != hipSuccess) return e;
@@ -222,12 +222,6 @@ int doe_gs_batch_create(doe_ctx* ctx, int count, const int* sides, int algorithm
     if (!rc) c.state = (GsState*)p;
     if (!rc) rc = B->alloc(&p, sizeof(double) * R_NFF * slots);
     if (!rc) c.partials = (double*)p;
-    if (!rc) rc = B->alloc(&p, sizeof(unsigned) * nd);
-    if (!rc) {
-      c.counters = (unsigned*)p;
-      // arrival counters start at zero once; the finalising workgroup resets them
-      if (hipMemset(c.counters, 0, sizeof(unsigned) * nd) != hipSuccess) rc = DOE_ERR_HIP;
-    }
   }
   if (!rc && (hipMemset(B->phi0, 0, tb) != hipSuccess || hipMemset(B->ph[0], 0, tb) != hipSuccess ||
               hipMemset(B->ph[1], 0, tb) != hipSuccess))

@@ -13,7 +13,6 @@ struct doe_gs_batch {
     size_t off = 0;          // element offset of the class's first design
     doe::GsState* state = nullptr;
     double* partials = nullptr;
-    unsigned* counters = nullptr;
     size_t hist_off = 0;     // design d's histories at hist + hist_off + d*3K
     doe::Plan2 pl;
     doe::GsBuf<float> b{};

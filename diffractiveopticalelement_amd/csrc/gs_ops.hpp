@@ -134,7 +134,6 @@ struct OpFFB {
   static constexpr bool PREFETCH = sizeof(TT) == 4 && N <= 512;
   GsBuf<T> b;
   double* partials;
-  unsigned* counters;
   int it;
   int write_amp;  // GS: last iteration writes |Y| to amp[0]
   template <typename U>
@@ -193,10 +192,12 @@ struct OpFFB {
     }
     const T tp = m > T(0) ? t : a;
     if (a > T(0)) {
-      // FP32: hardware reciprocal (about 1 ulp) instead of the division sequence
+      // FP32: hardware reciprocal (about 1 ulp) instead of the division
+      // sequence, for normal amplitudes only (the raw reciprocal is not
+      // specified for denormal inputs)
       T s;
       if constexpr (sizeof(T) == 4)
-        s = tp * __builtin_amdgcn_rcpf(a);
+        s = a >= __FLT_MIN__ ? tp * __builtin_amdgcn_rcpf(a) : tp / a;
       else
         s = tp / a;
       re *= s;
@@ -274,7 +275,7 @@ struct OpFFA {
       // correctly rounded square root and division per point
       const T q = re * re + im * im;
       nz = q > T(0);
-      if (nz) s = il * rsqrtf(q);
+      if (nz) s = q >= __FLT_MIN__ ? il * rsqrtf(q) : il / sqrtf(q);
     } else {
       const T r = sqrt(re * re + im * im);
       nz = r > T(0);

@@ -310,7 +310,7 @@ int run_metrics(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const 
   // partials: A | B | E, then folded A | B | E and the cells
   const long oA = 0, oB = oA + (long)nA * A_N, oE = oB + nBb * B_N, oFA = oE + (long)nE * E_N;
   const long oFB = oFA + A_N, oFE = oFB + B_N, oC = oFE + E_N, total = oC + ncell;
-  int rc = ensure_reduce(ctx, size_t(total), 0);
+  int rc = ensure_reduce(ctx, size_t(total));
   if (rc) return rc;
   double* P = ctx->partials;
   const S* sre = static_cast<const S*>(scr->re);

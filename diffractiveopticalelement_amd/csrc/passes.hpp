@@ -18,11 +18,12 @@
 // and either stores (MODE 0) or runs a `mid` hook and a second FFT in the
 // opposite stage order (MODE 1), then a `post` hook, and stores.  Hooks see
 // (field, x, y) in natural order.  Ops with NACC > 0 accumulate FP64 partial
-// sums per thread; each workgroup reduces them into `partials`, and the last
-// workgroup of each field (per-field arrival counter) folds that field's
-// partials in a fixed order and calls op.finalize(field, sums) —
-// deterministic, and the counter is reset by the finalizing workgroup, so no
-// memset is needed between launches.
+// sums per thread; each workgroup reduces them in a fixed order and stores
+// them into its own slot of `partials` (plain stores), and k_finalize,
+// launched right after the pass with one workgroup per field, folds that
+// field's slots in slot order and calls op.finalize(field, sums):
+// deterministic, no memset between launches and no cross-workgroup
+// synchronisation inside a pass.
 #pragma once
 #include "fft_engine.hpp"
 

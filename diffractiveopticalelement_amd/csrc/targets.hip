@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "ctx.hpp"
@@ -800,12 +802,8 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
   const int nx = out->nx, ny = out->ny, B = out->batch;
   std::vector<TgtRec> recs(B);
   std::vector<double> rasters;
-  struct RasterRef {
-    const double* p;
-    int w, h;
-    long off;
-  };
-  std::vector<RasterRef> raster_refs;
+  // host raster (pointer, width, height) -> its offset in `rasters`
+  std::map<std::tuple<const double*, int, int>, long> raster_refs;
   std::vector<unsigned char> dark;
   const double mn = std::min((double)nx, (double)ny);
   for (int b = 0; b < B; ++b) {
@@ -873,12 +871,13 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
         R.background = S.background;
         R.gain = S.gain;
         // specs that share a host raster (same array and size) share one copy
-        R.roff = -1;
-        for (const RasterRef& u : raster_refs)
-          if (u.p == S.raster && u.w == S.raster_w && u.h == S.raster_h) R.roff = u.off;
-        if (R.roff < 0) {
+        const auto key = std::make_tuple(S.raster, S.raster_w, S.raster_h);
+        const auto hit = raster_refs.find(key);
+        if (hit != raster_refs.end()) {
+          R.roff = hit->second;
+        } else {
           R.roff = (long)rasters.size();
-          raster_refs.push_back({S.raster, S.raster_w, S.raster_h, R.roff});
+          raster_refs.emplace(key, R.roff);
           rasters.insert(rasters.end(), S.raster, S.raster + (size_t)S.raster_w * S.raster_h);
         }
         break;
@@ -903,7 +902,7 @@ int make_impl(doe_ctx* ctx, const doe_target_spec* specs, doe_field* out, unsign
   double* part = nullptr;
   if (maxima) {
     // room for the maxima and, after them, the per-field thresholds of the ROI
-    rc = ensure_reduce(ctx, size_t(B) * (per + 1), 0);
+    rc = ensure_reduce(ctx, size_t(B) * (per + 1));
     if (rc) return rc;
     part = ctx->partials;
     *maxima = per;
@@ -927,7 +926,7 @@ int roi_impl(doe_ctx* ctx, doe_field* t, double dilate, double soft, double thre
   const bool has_soft = !(soft <= 0);
   const int s = has_soft ? (int)std::min((double)big, js_max(1, js_round(soft))) : 0;
   const long nblk = known ? long(known) : std::max(1L, std::min(64L, (n + kTB * 16 - 1) / (kTB * 16)));
-  int rc = ensure_reduce(ctx, size_t(B) * (nblk + 1), 0);  // no reallocation when `known`
+  int rc = ensure_reduce(ctx, size_t(B) * (nblk + 1));  // no reallocation when `known`
   if (rc) return rc;
   double* part = ctx->partials;
   double* thr = ctx->partials + size_t(B) * nblk;
@@ -1108,7 +1107,7 @@ int doe_field_image(doe_ctx* ctx, const doe_field* f, int index, const doe_image
   // reduction scratch: block min / max keys (2 per block), st[3], range[2], hist[256] (u32)
   const unsigned nblk = std::max(1u, std::min(1024u, (unsigned)((n + kTB * 8 - 1) / (kTB * 8))));
   const size_t o_st = 2 * size_t(nblk), o_rg = o_st + 3, o_h = o_rg + 3;
-  rc = ensure_reduce(ctx, o_h + 128, 0);
+  rc = ensure_reduce(ctx, o_h + 128);
   if (rc) return rc;
   unsigned long long* part = reinterpret_cast<unsigned long long*>(ctx->partials);
   unsigned long long* st = part + o_st;

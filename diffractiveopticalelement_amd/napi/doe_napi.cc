@@ -8,6 +8,7 @@
 // One process-wide library context on device 0 is created on first use.
 #include <node_api.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -90,6 +91,28 @@ double* f64(napi_env env, napi_value v, size_t* len) {
   if (t != napi_float64_array) return nullptr;
   if (len) *len = n;
   return (double*)data;
+}
+
+// A Float64Array of at least `need` elements.  Returns nullptr with a pending
+// RangeError when the array is shorter (the library would read or write
+// `need` elements through the pointer), and nullptr without an exception
+// when the value is not a Float64Array at all.
+double* f64_need(napi_env env, napi_value v, size_t need, const char* what) {
+  size_t n = 0;
+  double* p = f64(env, v, &n);
+  if (p && n < need) {
+    const std::string msg = std::string(what) + ": Float64Array of length " + std::to_string(n) +
+                            " is shorter than the " + std::to_string(need) + " elements required";
+    napi_throw_range_error(env, nullptr, msg.c_str());
+    return nullptr;
+  }
+  return p;
+}
+
+bool pending(napi_env env) {
+  bool p = false;
+  napi_is_exception_pending(env, &p);
+  return p;
 }
 
 napi_value make_f64(napi_env env, size_t n, double** data) {
@@ -197,40 +220,55 @@ napi_value FieldCreate(napi_env env, napi_callback_info info) {
   return ext;
 }
 
-napi_value FieldUpload(napi_env env, napi_callback_info info) {
-  Args a;
-  get_args(env, info, a);
-  doe_field* f = field_of(env, a.v[0]);
-  double* re = f64(env, a.v[1], nullptr);
-  double* im = f64(env, a.v[2], nullptr);
-  if (!f || !re || !im) {
-    napi_throw_type_error(env, nullptr, "fieldUpload(handle, re, im)");
-    return nullptr;
-  }
-  throw_status(env, doe_field_upload(ctx(env), f, re, im));
-  return nullptr;
+// Elements of every plane of a field (nx * ny * batch).
+size_t count_of(doe_field* f) {
+  int nx = 0, ny = 0, b = 0;
+  doe_field_info(f, &nx, &ny, &b, nullptr, nullptr, nullptr);
+  return size_t(nx) * size_t(ny) * size_t(b);
 }
 
-napi_value FieldDownload(napi_env env, napi_callback_info info) {
+// fieldUpload / fieldDownload(handle, re, im): both planes must hold at
+// least nx * ny * batch elements (RangeError otherwise).
+napi_value FieldTransfer(napi_env env, napi_callback_info info, bool up) {
   Args a;
   get_args(env, info, a);
-  doe_field* f = field_of(env, a.v[0]);
-  double* re = f64(env, a.v[1], nullptr);
-  double* im = f64(env, a.v[2], nullptr);
-  if (!f || !re || !im) {
-    napi_throw_type_error(env, nullptr, "fieldDownload(handle, re, im)");
+  const char* what = up ? "fieldUpload" : "fieldDownload";
+  doe_field* f = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
+  if (!f || a.n < 3) {
+    napi_throw_type_error(env, nullptr, up ? "fieldUpload(handle, re, im)" : "fieldDownload(handle, re, im)");
     return nullptr;
   }
-  throw_status(env, doe_field_download(ctx(env), f, re, im));
+  const size_t need = count_of(f);
+  double* re = f64_need(env, a.v[1], need, what);
+  double* im = re ? f64_need(env, a.v[2], need, what) : nullptr;
+  if (!re || !im) {
+    if (!pending(env))
+      napi_throw_type_error(env, nullptr, up ? "fieldUpload(handle, re, im)" : "fieldDownload(handle, re, im)");
+    return nullptr;
+  }
+  throw_status(env, up ? doe_field_upload(ctx(env), f, re, im) : doe_field_download(ctx(env), f, re, im));
   return nullptr;
 }
+napi_value FieldUpload(napi_env env, napi_callback_info info) { return FieldTransfer(env, info, true); }
+napi_value FieldDownload(napi_env env, napi_callback_info info) { return FieldTransfer(env, info, false); }
 
-// per-batch distances: a number or a Float64Array
-std::vector<double> zlist(napi_env env, napi_value v, int batch) {
+// Per-batch distances: a number (every field) or a Float64Array of at least
+// `batch` entries.  Returns false with a pending RangeError when shorter.
+bool zlist(napi_env env, napi_value v, int batch, const char* what, std::vector<double>* out) {
   size_t n = 0;
   double* p = f64(env, v, &n);
-  if (p) return std::vector<double>(p, p + n);
-  return std::vector<double>(batch, num(env, v));
+  if (p) {
+    if (n < size_t(batch)) {
+      const std::string msg = std::string(what) + ": " + std::to_string(n) +
+                              " distances for a batch of " + std::to_string(batch) + " fields";
+      napi_throw_range_error(env, nullptr, msg.c_str());
+      return false;
+    }
+    out->assign(p, p + batch);
+  } else {
+    out->assign(size_t(batch), num(env, v));
+  }
+  return true;
 }
 
 int batch_of(doe_field* f) {
@@ -249,7 +287,8 @@ napi_value Asm(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "asm(hin, hout, dx, dy, lambda, z, bandLimit, evanescent, softEdge)");
     return nullptr;
   }
-  std::vector<double> z = zlist(env, a.v[5], batch_of(fi));
+  std::vector<double> z;
+  if (!zlist(env, a.v[5], batch_of(fi), "asm", &z)) return nullptr;
   throw_status(env, doe_asm(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
                             z.data(), boolean(env, a.v[6], true), boolean(env, a.v[7], true),
                             num(env, a.v[8], 0.02)));
@@ -296,7 +335,8 @@ napi_value Fresnel(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   const int b = batch_of(fi);
-  std::vector<double> z = zlist(env, a.v[5], b), odx(b), ody(b);
+  std::vector<double> z, odx(b), ody(b);
+  if (!zlist(env, a.v[5], b, "fresnel", &z)) return nullptr;
   const int rc = doe_fresnel(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
                              z.data(), boolean(env, a.v[6], false), a.n > 7 ? num(env, a.v[7]) : 0,
                              a.n > 8 ? num(env, a.v[8]) : 0, odx.data(), ody.data());
@@ -319,7 +359,8 @@ napi_value Lens(napi_env env, napi_callback_info info) {
     return nullptr;
   }
   const int b = batch_of(fi);
-  std::vector<double> f = zlist(env, a.v[5], b), odx(b), ody(b);
+  std::vector<double> f, odx(b), ody(b);
+  if (!zlist(env, a.v[5], b, "lens", &f)) return nullptr;
   const int rc = doe_lens(ctx(env), fi, fo, num(env, a.v[2]), num(env, a.v[3]), num(env, a.v[4]),
                           f.data(), odx.data(), ody.data());
   if (throw_status(env, rc)) return nullptr;
@@ -471,7 +512,9 @@ napi_value GsMultiPlane(napi_env env, napi_callback_info info) {
                           "gsMultiPlane(illum, planes, nx, ny, dx, dy, lambda, iterations, dtype, initPhase)");
     return nullptr;
   }
-  double* illum = f64(env, a.v[0], nullptr);
+  const size_t n = size_t(std::max(0, (int)num(env, a.v[2]))) * size_t(std::max(0, (int)num(env, a.v[3])));
+  double* illum = f64_need(env, a.v[0], n, "gsMultiPlane: illumination");
+  if (pending(env)) return nullptr;
   bool is_arr = false;
   napi_is_array(env, a.v[1], &is_arr);
   if (!illum || !is_arr) {
@@ -487,9 +530,11 @@ napi_value GsMultiPlane(napi_env env, napi_callback_info info) {
     napi_get_named_property(env, el, "z", &v);
     planes[q].z = num(env, v);
     napi_get_named_property(env, el, "targetAmp", &v);
-    planes[q].target_amp = f64(env, v, nullptr);
+    planes[q].target_amp = f64_need(env, v, n, "gsMultiPlane: targetAmp");
+    if (pending(env)) return nullptr;
     napi_get_named_property(env, el, "roi", &v);
-    planes[q].roi = f64(env, v, nullptr);
+    planes[q].roi = f64_need(env, v, n, "gsMultiPlane: roi");
+    if (pending(env)) return nullptr;
     napi_get_named_property(env, el, "weight", &v);
     planes[q].weight = num(env, v, 1.0);
     if (!planes[q].target_amp) {
@@ -505,10 +550,10 @@ napi_value GsMultiPlane(napi_env env, napi_callback_info info) {
   plan.lambda = num(env, a.v[6]);
   plan.iterations = (int)num(env, a.v[7]);
   plan.dtype = (int)num(env, a.v[8], DOE_F32);
-  const double* init = a.n > 9 ? f64(env, a.v[9], nullptr) : nullptr;
+  const double* init = a.n > 9 ? f64_need(env, a.v[9], n, "gsMultiPlane: initPhase") : nullptr;
+  if (pending(env)) return nullptr;
   doe_ctx* c = ctx(env);
   if (!c) return nullptr;
-  const size_t n = size_t(plan.nx) * plan.ny;
   const size_t K = plan.iterations > 0 ? size_t(plan.iterations) : 0;
   double *phase, *err, *eff;
   napi_value jphase = make_f64(env, n, &phase);
@@ -737,8 +782,12 @@ napi_value RasterBlur(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "rasterBlur(src, dst, sigma, taps)");
     return nullptr;
   }
-  const double* taps = a.n > 3 ? f64(env, a.v[3], nullptr) : nullptr;
-  throw_status(env, doe_raster_blur(ctx(env), s, d, num(env, a.v[2], NAN), taps));
+  const double sigma = num(env, a.v[2], NAN);
+  // the library reads 2r + 1 taps, r = max(1, ceil(3 sigma)) (doe.js:765)
+  const size_t ntaps = sigma > 0 && std::isfinite(sigma) ? size_t(2 * std::max(1.0, std::ceil(3 * sigma)) + 1) : 0;
+  const double* taps = a.n > 3 ? f64_need(env, a.v[3], ntaps, "rasterBlur: taps") : nullptr;
+  if (pending(env)) return nullptr;
+  throw_status(env, doe_raster_blur(ctx(env), s, d, sigma, taps));
   return nullptr;
 }
 

@@ -686,6 +686,9 @@ class Field:
     def upload(self, re, im) -> None:
         re = _darr(re).reshape(-1)
         im = _darr(im).reshape(-1)
+        n = self.nx * self.ny * self.batch
+        if re.size != n or im.size != n:
+            raise ValueError(f"Field.upload: planes of {re.size} and {im.size} values for a field of {n}")
         check(self.ctx.lib.doe_field_upload(self.ctx.h, self.h, _dptr(re), _dptr(im)))
 
     def ptrs(self):

@@ -20,8 +20,10 @@ def node_available() -> bool:
     return shutil.which("node") is not None
 
 
-def run(op: str, args: dict | None = None, inputs: dict | None = None, timeout: float = 600):
-    """Run one oracle job; returns (arrays: dict[str, ndarray], result: dict)."""
+def run(op: str, args: dict | None = None, inputs: dict | None = None, timeout: float = 600,
+        pin_cpu: int | None = None):
+    """Run one oracle job; returns (arrays: dict[str, ndarray], result: dict).
+    pin_cpu: run node on that one host core (taskset), for timings."""
     with tempfile.TemporaryDirectory(prefix="doe_oracle_") as d:
         dp = Path(d)
         (dp / "job.json").write_text(json.dumps({"op": op, "args": args or {}}))
@@ -29,8 +31,10 @@ def run(op: str, args: dict | None = None, inputs: dict | None = None, timeout: 
             if arr is None:
                 continue
             np.ascontiguousarray(np.asarray(arr, dtype="<f8")).ravel().tofile(dp / f"in_{name}.f64")
-        proc = subprocess.run(["node", str(RUNNER), str(dp)], capture_output=True, text=True,
-                              timeout=timeout)
+        cmd = ["node", str(RUNNER), str(dp)]
+        if pin_cpu is not None and shutil.which("taskset"):
+            cmd = ["taskset", "-c", str(pin_cpu)] + cmd
+        proc = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if proc.returncode != 0:
             raise RuntimeError(f"oracle {op} failed:\n{proc.stderr[-4000:]}")
         res = json.loads((dp / "result.json").read_text())

@@ -17,13 +17,16 @@ import pytest
 from scipy import ndimage
 
 import oracle_js as oj
+from bitcmp import assert_same_bits
 
 pytestmark = pytest.mark.skipif(not oj.node_available(), reason="node is required for the oracle")
 
 
 def js_round(x):
+    """ECMAScript Math.round: ties toward +Infinity, and -0 for x in [-0.5, -0]."""
     f = math.floor(x)
-    return f + 1 if x - f >= 0.5 else f
+    r = float(f + 1 if x - f >= 0.5 else f)
+    return math.copysign(0.0, x) if r == 0 else r
 
 
 def wrap(p):
@@ -74,7 +77,19 @@ def test_error_diffusion_matches_gather_form(n, levels, snake, wrap_first):
     phi = rng.uniform(-7, 7, n * n)
     out, _ = oj.run("error_diffusion", {"levels": levels, "opts": {"snake": snake, "wrapFirst": wrap_first}},
                     {"phase": phi})
-    np.testing.assert_array_equal(out["out"], gather_fs(phi, n, levels, snake, wrap_first))
+    assert_same_bits(out["out"], gather_fs(phi, n, levels, snake, wrap_first))
+
+
+def test_error_diffusion_keeps_negative_zero_levels():
+    """Math.round gives -0 for quotients in [-0.5, 0), so such levels are -0
+    in the reference's output (and the wrap keeps the sign)."""
+    rng = np.random.default_rng(5)
+    n = 16
+    phi = rng.uniform(-0.6, 0.6, n * n)
+    out, _ = oj.run("error_diffusion", {"levels": 4, "opts": {"snake": True}}, {"phase": phi})
+    neg0 = (out["out"] == 0) & np.signbit(out["out"])
+    assert neg0.sum() > 10
+    assert_same_bits(out["out"], gather_fs(phi, n, 4))
 
 
 def test_error_diffusion_reference_behaviour():

@@ -11,6 +11,7 @@ import pytest
 
 from diffractiveopticalelement_amd import native as nat
 import oracle_js as oj
+from bitcmp import assert_same_bits
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not oj.node_available(), reason="node (the oracle's runtime) missing")]
@@ -42,7 +43,7 @@ def test_error_diffusion_bit_exact(ctx, n, snake, levels):
     print("error diffusion", n, snake, levels, flush=True)
     out, _ = oj.run("error_diffusion", {"levels": levels, "opts": {"snake": snake}}, {"phase": phi})
     got = device_fs(ctx, [phi], n, levels, snake)[0]
-    np.testing.assert_array_equal(got, out["out"])
+    assert_same_bits(got, out["out"])
 
 
 @pytest.mark.parametrize("snake", [True, False])
@@ -54,7 +55,7 @@ def test_error_diffusion_batch_and_no_wrap(ctx, snake):
     for q, phi in enumerate(phis):
         out, _ = oj.run("error_diffusion", {"levels": 8, "opts": {"snake": snake, "wrapFirst": False}},
                         {"phase": phi})
-        np.testing.assert_array_equal(got[q], out["out"])
+        assert_same_bits(got[q], out["out"])
 
 
 @pytest.mark.parametrize("snake", [True, False])
@@ -73,7 +74,7 @@ def test_error_diffusion_half_level_inputs(ctx, snake, levels):
             out, _ = oj.run("error_diffusion", {"levels": levels, "opts": {"snake": snake, "wrapFirst": wrap_first}},
                             {"phase": phi})
             got = device_fs(ctx, [phi], n, levels, snake, wrap_first)[0]
-            np.testing.assert_array_equal(got, out["out"])
+            assert_same_bits(got, out["out"])
 
 
 def _wrap(p):
@@ -108,7 +109,8 @@ def _gather_fs(x, levels, wrap_first=True, snake=True):
             if prev is not None:
                 v += prev * 7 / 16
             f = math.floor(v / step)
-            q = (f + 1 if v / step - f >= 0.5 else f) * step
+            r = float(f + 1 if v / step - f >= 0.5 else f)
+            q = (math.copysign(0.0, v / step) if r == 0 else r) * step  # Math.round's -0
             e = v - q
             if e > math.pi:
                 e -= 2 * math.pi
@@ -127,7 +129,7 @@ def test_error_diffusion_plain_rows_tall_images(ctx, nx, ny):
     o = ctx.field(nx, ny, 1, "f64")
     ctx.error_diffusion(f, o, 4, False, True)
     re, _ = o.download()
-    np.testing.assert_array_equal(re[0], _gather_fs(x[0], 4, snake=False))
+    assert_same_bits(re[0], _gather_fs(x[0], 4, snake=False))
     f.free()
     o.free()
 
@@ -143,7 +145,7 @@ def test_error_diffusion_widest_lds_rows(ctx):
     ctx.error_diffusion(f, o, 3, True, True)
     re, _ = o.download()
     for b in range(B):
-        np.testing.assert_array_equal(re[b], _gather_fs(x[b], 3))
+        assert_same_bits(re[b], _gather_fs(x[b], 3))
     f.free()
     o.free()
 
@@ -159,7 +161,7 @@ def test_error_diffusion_long_rows_one_lane_per_image(ctx, levels):
     ctx.error_diffusion(f, o, levels, True, True)
     re, _ = o.download()
     for b in range(B):
-        np.testing.assert_array_equal(re[b], _gather_fs(x[b], levels))
+        assert_same_bits(re[b], _gather_fs(x[b], levels))
     f.free()
     o.free()
 
@@ -187,7 +189,7 @@ def test_error_diffusion_smooth_phase_and_timing(ctx):
     o = ctx.field(n, n, 1, "f64")
     ctx.error_diffusion(f, o, 4, False)
     re, _ = o.download()
-    np.testing.assert_array_equal(re.ravel(), out["out"])
+    assert_same_bits(re.ravel(), out["out"])
 
 
 def test_phase_to_height_bit_exact(ctx):
@@ -199,7 +201,7 @@ def test_phase_to_height_bit_exact(ctx):
         o = ctx.field(24, 40, 3, "f64")
         ctx.phase_to_height(f, o, lam, nidx)
         re, _ = o.download()
-        np.testing.assert_array_equal(re.ravel(), out["out"])
+        assert_same_bits(re.ravel(), out["out"])
         f.free()
         o.free()
 
