@@ -2,14 +2,18 @@
 // phase-retrieval loops (ref: src/doe.js:49-86, 327-546, 639-730, 761-905,
 // 979-1000).  ES2019.
 //
-// Pinning: the FFT/propagator parts are pinned closed-form (see propagate.js);
-// the loops themselves have no closed form.  tests/test_oracle_cpu.py checks
-// them against the reference's own behavioural tests (ref test/doe.test.js,
-// test/projector.test.js thresholds) and the published design results
-// (ref README.md:29, 35: image-plane 'DOE' rmse 0.0064 / efficiency 0.78,
-// far-field rmse 8.2 %, rho 0.991, efficiency 81 %), plus invariants
-// (determinism, unit-modulus DOE plane, power conservation, first-iteration
-// identities).  Iterate-level values beyond those are "parity unpinned".
+// Pinning: the FFT/propagator parts are pinned closed-form (see propagate.js).
+// tests/test_oracle_cpu.py checks the loops against the reference's own
+// behavioural tests (ref test/doe.test.js, test/projector.test.js
+// thresholds), the published design results (ref README.md:29, 35:
+// image-plane 'DOE' rmse 0.0064 / efficiency 0.78, far-field rmse 8.2 %,
+// rho 0.991, efficiency 81 %) and first-iteration identities;
+// tests/test_oracle_pins_cpu.py checks each step against numpy written from
+// its definition: the HIO relaxation and escape period, the far-field WGS
+// weight update (clamp, power, clamp, renormalisation) over two iterations,
+// the best-iterate bookkeeping over nine, the a = 0 projection branch and
+// the illumination profiles; tests/test_oracle_mutants_cpu.py plants slips
+// in a copy of this file and checks that those pins fail.
 
 import { ComplexField, discMask } from './field.js';
 import { propagateAngularSpectrumInPlace, lensPhase, wrapPhase } from './propagate.js';

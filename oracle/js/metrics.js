@@ -1,9 +1,14 @@
 // ORACLE (test infrastructure only) — FP64 restatement of the screen-metric
 // reductions (ref: src/metrics.js:10-321).  ES2019.
 //
-// Pinned by tests/test_oracle_cpu.py against closed forms (identical planes:
-// rmse 0, correlation 1, ssim 1; uniform planes; numpy reimplementations of
-// the published definitions for rmse/correlation/regionStats).
+// Pinned by tests/test_oracle_cpu.py (identical planes: rmse 0, correlation
+// 1, ssim 1; numpy forms of rmse, correlation, regionStats) and
+// tests/test_oracle_pins_cpu.py: SSIM on piecewise-constant and affine
+// windows (C1, C2, the data range L, target weighting, partial windows
+// skipped), zeroOrderFraction of a plane wave (1), a checkerboard (0) and a
+// random field, rmsRadius of Gaussian beams (w / 2 per axis, about a given
+// centre), spotUniformity of equal and 1:3 spots above a background below
+// the cut; tests/test_oracle_mutants_cpu.py checks those pins catch slips.
 
 export function maskedSum(plane, mask) {
   let s = 0;

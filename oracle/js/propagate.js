@@ -4,7 +4,9 @@
 // Pinned by tests/test_oracle_cpu.py: closed-form H against numpy, ASM
 // unitarity and round trip, Gaussian w(z), band-limit zeroing, brute-force
 // Fresnel integral, Fresnel Parseval/inverse, ASM == Fresnel at z_c, lens
-// transform unitarity and distance invariance.
+// transform unitarity and distance invariance; tests/test_oracle_pins_cpu.py:
+// the raised-cosine band edge (1 up to (1 - softEdge) lim, 0 from lim, the
+// cosine between, separable in fx and fy).
 
 import { ComplexField } from './field.js';
 import { fft2d, fftshift2d, ifftshift2d } from './fft.js';

@@ -15,10 +15,10 @@ import {
   focalPlaneSpectrum,
   focusSweep,
 } from './js/propagate.js';
-import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane, blurRaster, errorDiffusionQuantize, quantizePhase, phaseToHeight } from './js/doe.js';
+import { gerchbergSaxton, weightedGerchbergSaxton, fourierCgh, designDOE, randomPhase, gerchbergSaxtonMultiPlane, blurRaster, errorDiffusionQuantize, quantizePhase, phaseToHeight, illuminationProfile } from './js/doe.js';
 import { ProjectorSystem, makeTarget, autoRoi, resampleRaster, renderText } from './js/projector.js';
 import { planeToRGB, planeToGray } from './js/color.js';
-import { ssim, rmsError, correlation, regionStats } from './js/metrics.js';
+import { ssim, rmsError, correlation, regionStats, zeroOrderFraction, rmsRadius, spotUniformity } from './js/metrics.js';
 
 const dir = process.argv[2];
 const job = JSON.parse(fs.readFileSync(path.join(dir, 'job.json'), 'utf8'));
@@ -213,6 +213,20 @@ if (op === 'fft2d' || op === 'dft2d') {
   res.correlation = correlation(p, t, m);
   res.ssim = ssim(p, t, a.nx, a.ny, { window: 8, weight: a.weight || null });
   res.stats = regionStats(p, m);
+} else if (op === 'metric_fns') {
+  // zeroOrderFraction of the field in_re/in_im; rmsRadius and spotUniformity
+  // of in_plane (either part may be absent)
+  if (inp('re')) res.zeroOrder = zeroOrderFraction(fieldOf(a.nx, a.ny, a.dx || 1, a.dy || 1));
+  const p = inp('plane');
+  if (p) {
+    res.rmsRadius = rmsRadius(p, a.nx, a.dx, a.cx, a.cy, a.dy);
+    const su = spotUniformity(p, a.nx, a.ny, a.spots || {});
+    res.spotUniformity = su.uniformity;
+    res.spotCells = su.cells;
+    res.spotCv = su.cv;
+  }
+} else if (op === 'illumination') {
+  out('profile', illuminationProfile(a.type, a.nx, a.ny, a.dx, a.dy, a.opts || {}));
 } else if (op === 'time') {
   // CPU baseline timing of one unit of work on a single JS thread,
   // 1 warm-up + reps (the reference's bench.mjs methodology).

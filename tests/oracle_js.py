@@ -5,6 +5,7 @@ The oracle is the checker only: tests compare the HIP path against it.
 from __future__ import annotations
 
 import json
+import os
 import shutil
 import subprocess
 import tempfile
@@ -13,7 +14,9 @@ from pathlib import Path
 import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
-RUNNER = ROOT / "oracle" / "run.mjs"
+# DOE_ORACLE_DIR: run another copy of the oracle (the mutation tests plant a
+# mistake in a temporary copy and check that the pins catch it)
+RUNNER = Path(os.environ.get("DOE_ORACLE_DIR", ROOT / "oracle")) / "run.mjs"
 
 
 def node_available() -> bool:
