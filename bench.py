@@ -263,11 +263,46 @@ def errors_leg(ctx, field, zs, picks=(0, 15)) -> dict:
     p_out = np.sum(np.abs(r32) ** 2, axis=(1, 2)) * odx * odx
     out["fresnel"] = {"f32_vs_f64_device": _rel(r32, r64),
                       "parseval_f32_max_abs_dev": float(np.max(np.abs(p_out / p_in - 1)))}
+    out["c4_4096sq"] = c4_errors(ctx)
     for k, v in out.items():
         if isinstance(v, dict):
             out[k] = {kk: (float(f"{vv:.3e}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
     out["note"] = ("relative L2 over the picked fields; the device FP64 check mode is pinned to the FP64 "
                    "oracle (<= 1e-12) by the tests; gates: FP32 <= 1e-5, Parseval <= 1e-6")
+    return out
+
+
+def c4_errors(ctx, n: int = 4096) -> dict:
+    """The precision sweep's figures on its complex N(0, 1) field at 4096^2:
+    fft2d and band-limited ASM (z = 50 mm, 532 nm, 2 um) stored in FP32,
+    FP16, BF16 against the FP64 check mode, and fft2d against a naive DFT on
+    a 256^2 crop of the spectrum."""
+    rng = np.random.default_rng(44)
+    x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2)
+    dx, lam, z = 2e-6, 532e-9, 0.05
+
+    def run(dt, asm):
+        f = ctx.upload(x.real, x.imag, dt)
+        if asm:
+            ctx.asm(f, f, dx, dx, lam, z)
+        else:
+            ctx.fft2d(f)
+        re, im = f.download()
+        f.free()
+        return re[0] + 1j * im[0]
+
+    out = {"field": "complex N(0,1), seed 44"}
+    crop = dft_crop(x, 1920, 1920, 256)
+    for asm in (False, True):
+        ref = run("f64", asm)
+        name = "asm" if asm else "fft2d"
+        if not asm:
+            out["fft2d_f64_vs_naive_dft_crop256"] = _rel(ref[1920:2176, 1920:2176], crop)
+        for dt in ("f32", "f16", "bf16"):
+            y = run(dt, asm)
+            out[f"{name}_{dt}_vs_f64"] = _rel(y, ref)
+            if not asm:
+                out[f"fft2d_{dt}_vs_naive_dft_crop256"] = _rel(y[1920:2176, 1920:2176], crop)
     return out
 
 

@@ -54,6 +54,31 @@ def test_naive_dft_and_round_trip():
     assert np.abs(b["re"] - re).max() < 1e-12 and np.abs(b["im"] - im).max() < 1e-12
 
 
+def test_dft_crop_oracle_pins():
+    """oracle/dft_crop.py: whole small grids equal the JS oracle's
+    dft2dNaive (both directions); crops of a 512^2 spectrum equal numpy's."""
+    import sys
+    sys.path.insert(0, str(oj.ROOT / "oracle"))
+    import dft_crop
+
+    nx, ny = 16, 8
+    re, im = lcg(nx * ny, 31), lcg(nx * ny, 37)
+    x = (re + 1j * im).reshape(ny, nx)
+    for inverse in (False, True):
+        out, _ = oj.run("dft2d", {"nx": nx, "ny": ny, "inverse": inverse}, {"re": re, "im": im})
+        ref = (out["re"] + 1j * out["im"]).reshape(ny, nx)
+        assert rel(dft_crop.dft2_crop(x, 0, 0, ny, nx, inverse), ref) < 1e-13
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((512, 512)) + 1j * rng.standard_normal((512, 512))
+    full = np.fft.fft2(u)
+    for ky0, kx0 in ((0, 0), (200, 450), (480, 17)):  # the last wraps around
+        got = dft_crop.dft2_crop(u, ky0, kx0, 64)
+        want = full[np.ix_((ky0 + np.arange(64)) % 512, (kx0 + np.arange(64)) % 512)]
+        assert rel(got, want) < 1e-13
+    back = dft_crop.dft2_crop(full, 100, 30, 32, inverse=True)
+    assert rel(back, u[100:132, 30:62]) < 1e-13
+
+
 def test_delta_flat_and_parseval():
     n = 32
     re = np.zeros(n * 1)
@@ -145,6 +170,29 @@ def test_fresnel_brute_force_integral():
             ref[v, uu] = np.sum(g * np.exp(1j * q)) * dx * dx / (LAM * z)
     assert rel((out["re"] + 1j * out["im"]).reshape(n, n), ref) < 1e-9
     assert abs(res["dx"] - du) < 1e-18
+
+
+def test_fresnel_brute_force_non_square():
+    """8 x 2 grid: (nx + ny) / 2 is odd, so the reference's global phase
+    k z - pi (nx + ny) / 2 differs from k z by pi here (on square grids of
+    even side the term is a multiple of 2 pi)."""
+    nx, ny, dx, z = 8, 2, 20e-6, 40e-3
+    rng = np.random.default_rng(2)
+    g = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    out, res = oj.run("fresnel", {"nx": nx, "ny": ny, "dx": dx, "dy": dx, "lambda": LAM, "z": z},
+                      {"re": g.real.ravel(), "im": g.imag.ravel()})
+    dux, duy = LAM * z / (nx * dx), LAM * z / (ny * dx)
+    x = (np.arange(nx) - nx / 2) * dx
+    y = (np.arange(ny) - ny / 2) * dx
+    X, Y = np.meshgrid(x, y)
+    ref = np.zeros((ny, nx), complex)
+    for j in range(ny):
+        for i in range(nx):
+            u, v = (i - nx / 2) * dux, (j - ny / 2) * duy
+            q = np.pi * ((u - X) ** 2 + (v - Y) ** 2) / (LAM * z) + 2 * np.pi * z / LAM - np.pi / 2
+            ref[j, i] = np.sum(g * np.exp(1j * q)) * dx * dx / (LAM * z)
+    assert rel((out["re"] + 1j * out["im"]).reshape(ny, nx), ref) < 1e-9
+    assert abs(res["dx"] - dux) < 1e-18 and abs(res["dy"] - duy) < 1e-18
 
 
 def test_fresnel_parseval_and_inverse():

@@ -72,6 +72,8 @@ MUTANTS = [
     ("js/propagate.js", "const amp = (dx * dy) / (lambda * az);", "const amp = (dx * dy) / (lambda * z * z);",
      f"{OLD}::test_fresnel_brute_force_integral"),
     ("js/doe.js", "    s ^= s >>> 17;", "    s ^= s >>> 15;", f"{OLD}::test_random_phase_known_answers"),
+    ("js/propagate.js", "const kz = k * z - (Math.PI * (nx + ny)) / 2;", "const kz = k * z;",
+     f"{OLD}::test_fresnel_brute_force_non_square"),
 ]
 
 
