@@ -318,6 +318,10 @@ int doe_ctx_create(int device, void* stream, doe_ctx** out) {
   DOE_HIP(hipGetDeviceCount(&c));
   DOE_REQUIRE(device >= 0 && device < c, DOE_ERR_INVALID, "no such HIP device");
   DOE_HIP(hipSetDevice(device));
+  {
+    const int rc = metrics_scratch_init(device);  // the no-allocation entry point's scratch
+    if (rc) return rc;
+  }
   doe_ctx* ctx = new doe_ctx();
   ctx->device = device;
   if (stream) {

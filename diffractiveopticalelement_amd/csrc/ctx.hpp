@@ -119,6 +119,8 @@ int ensure_reduce(doe_ctx* ctx, size_t n);
 int ensure_stage(doe_ctx* ctx, size_t bytes);
 // Ensure two work planes of `bytes` each (ctx->work_re / work_im).
 int ensure_work(doe_ctx* ctx, size_t bytes);
+// Resolve the fixed scratch of doe_screen_metrics_noalloc on `device` (current).
+int metrics_scratch_init(int device);
 
 // Compile-time dispatch over the supported power-of-two line lengths.
 template <class F>

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <vector>
 
 #include "ctx.hpp"
@@ -43,6 +44,19 @@ constexpr unsigned long long kOpsB =
     (unsigned long long)OP_MIN << (2 * B_RMIN) | (unsigned long long)OP_MAX << (2 * B_RMAX);
 // element components
 enum { E_SR, E_SI, E_P, E_AMP, E_N };
+
+// Fixed scratch of doe_screen_metrics_noalloc: the reduction partials (FP64)
+// followed by the spot-cell bounds (int).  One copy per GPU, held until the
+// process ends; its device address is resolved once per device by
+// metrics_scratch_init and kept in g_scratch (host code never passes the
+// array's name to a kernel).  Every partial slot is written by its kernel
+// before it is read, so nothing needs clearing between calls.
+constexpr size_t kScratchBytes = size_t(DOE_NOALLOC_SCRATCH_BYTES);
+constexpr long kNoallocMaxCells = 4096;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
 
 __device__ __forceinline__ double op_apply(int op, double a, double b) {
   return op == OP_SUM ? a + b : (op == OP_MAX ? fmax(a, b) : fmin(a, b));
@@ -282,21 +296,22 @@ __global__ __launch_bounds__(kMB) void k_resample_centred(const S* __restrict__ 
 // First pixel index of each of `cells` cells along a side of n pixels, with
 // the reference's mapping c(i) = min(cells - 1, floor(i / n * cells)), which
 // never decreases with i: cell c is [b[c], b[c+1]), b[cells] = n.
-std::vector<int> cell_bounds(int n, int cells) {
-  std::vector<int> b(cells + 1, n);
+void cell_bounds(int n, int cells, int* b) {
+  for (int c = 0; c <= cells; ++c) b[c] = n;
   int next = 0;
   for (int i = 0; i < n; ++i) {
     const int c = std::min(cells - 1, int(std::floor((double(i) / double(n)) * double(cells))));
     while (next <= c) b[next++] = i;
   }
-  return b;
 }
 
 int blocks_of(long n) { return int(std::min<long>((n + kMB - 1) / kMB, kMaxBlocksA)); }
 
+// fixed: the no-allocation variant's scratch (device address), or nullptr
+// to use the context's reduction buffers (allocated on demand).
 template <class S, class SE>
 int run_metrics(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const doe_field* el,
-                const doe_screen_opts* o, doe_metrics* out) {
+                const doe_screen_opts* o, doe_metrics* out, unsigned char* fixed) {
   hipStream_t st = ctx->stream;
   const int nx = scr->nx, ny = scr->ny;
   const long n = (long)nx * ny;
@@ -310,15 +325,30 @@ int run_metrics(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const 
   // partials: A | B | E, then folded A | B | E and the cells
   const long oA = 0, oB = oA + (long)nA * A_N, oE = oB + nBb * B_N, oFA = oE + (long)nE * E_N;
   const long oFB = oFA + A_N, oFE = oFB + B_N, oC = oFE + E_N, total = oC + ncell;
-  int rc = ensure_reduce(ctx, size_t(total));
-  if (rc) return rc;
-  double* P = ctx->partials;
+  // the cell bounds follow the partials (fixed scratch only)
+  const size_t bounds_off = (size_t(total) * sizeof(double) + 255) & ~size_t(255);
+  const size_t nbounds = ncell > 0 ? size_t(cols + 1 + rows + 1) : 0;
+  double* P;
+  int rc;
+  if (fixed) {
+    DOE_REQUIRE(ncell <= kNoallocMaxCells, DOE_ERR_UNSUPPORTED,
+                "doe_screen_metrics_noalloc: at most 4096 spot cells");
+    DOE_REQUIRE(bounds_off + nbounds * sizeof(int) <= kScratchBytes, DOE_ERR_UNSUPPORTED,
+                "doe_screen_metrics_noalloc: the screen needs " +
+                    std::to_string(bounds_off + nbounds * sizeof(int)) + " bytes of scratch, more than the " +
+                    std::to_string(kScratchBytes) + " the entry point holds");
+    P = reinterpret_cast<double*>(fixed);
+  } else {
+    rc = ensure_reduce(ctx, size_t(total));
+    if (rc) return rc;
+    P = ctx->partials;
+  }
   const S* sre = static_cast<const S*>(scr->re);
   const S* sim = static_cast<const S*>(scr->im);
   const double* tg = static_cast<const double*>(tgt->re);
   const double* mk = static_cast<const double*>(tgt->im);
   const double cell = o->sdx * o->sdy;
-  prof_begin(ctx, "screen_metrics", n);
+  if (!fixed) prof_begin(ctx, "screen_metrics", n);
   hipLaunchKernelGGL(k_met_a<S>, dim3(nA), dim3(kMB), 0, st, sre, sim, tg, mk, nx, ny, o->sdx, o->sdy,
                      P + oA);
   hipLaunchKernelGGL(k_met_fold, dim3(1), dim3(kMB), 0, st, P + oA, nA, int(A_N), kOpsA, P + oFA);
@@ -328,23 +358,45 @@ int run_metrics(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const 
   hipLaunchKernelGGL(k_met_elem<SE>, dim3(nE), dim3(kMB), 0, st, static_cast<const SE*>(el->re),
                      static_cast<const SE*>(el->im), ne, P + oE);
   hipLaunchKernelGGL(k_met_fold, dim3(1), dim3(kMB), 0, st, P + oE, nE, int(E_N), 0ull, P + oFE);
+  // host staging: stack arrays for the fixed-scratch variant (bounded cell
+  // count), vectors otherwise
+  int bounds_fix[2 * (kNoallocMaxCells + 1)];
+  double h_fix[A_N + B_N + E_N + kNoallocMaxCells];
+  std::vector<int> bounds_vec;
+  std::vector<double> h_vec;
   if (ncell > 0) {
-    std::vector<int> b = cell_bounds(nx, cols), c = cell_bounds(ny, rows);
-    std::vector<int> both(b);
-    both.insert(both.end(), c.begin(), c.end());
-    void* dp = nullptr;
-    rc = upload_params(ctx, both.data(), both.size() * sizeof(int), &dp);
-    if (rc) return rc;
-    const int* dbx = static_cast<const int*>(dp);
+    int* both = bounds_fix;
+    if (!fixed) {
+      bounds_vec.resize(nbounds);
+      both = bounds_vec.data();
+    }
+    cell_bounds(nx, cols, both);
+    cell_bounds(ny, rows, both + cols + 1);
+    const int* dbx;
+    if (fixed) {
+      int* db = reinterpret_cast<int*>(fixed + bounds_off);
+      DOE_HIP(hipMemcpyAsync(db, both, nbounds * sizeof(int), hipMemcpyHostToDevice, st));
+      dbx = db;
+    } else {
+      void* dp = nullptr;
+      rc = upload_params(ctx, both, nbounds * sizeof(int), &dp);
+      if (rc) return rc;
+      dbx = static_cast<const int*>(dp);
+    }
     hipLaunchKernelGGL(k_met_cells<S>, dim3(unsigned(ncell)), dim3(kMB), 0, st, sre, sim, nx, cols, dbx,
                        dbx + cols + 1, P + oFA, o->grid_min_peak, P + oC);
   }
   DOE_HIP(hipGetLastError());
-  prof_end(ctx);
-  std::vector<double> h(size_t(total - oFA));
-  DOE_HIP(hipMemcpyAsync(h.data(), P + oFA, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (!fixed) prof_end(ctx);
+  const size_t nh = size_t(total - oFA);
+  double* h = h_fix;
+  if (!fixed) {
+    h_vec.resize(nh);
+    h = h_vec.data();
+  }
+  DOE_HIP(hipMemcpyAsync(h, P + oFA, nh * sizeof(double), hipMemcpyDeviceToHost, st));
   DOE_HIP(hipStreamSynchronize(st));
-  const double* A = h.data();
+  const double* A = h;
   const double* B = A + A_N;
   const double* E = B + B_N;
   const double* C = E + E_N;
@@ -404,25 +456,18 @@ int run_metrics(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const 
 
 template <class S>
 int run_metrics_el(doe_ctx* ctx, const doe_field* scr, const doe_field* tgt, const doe_field* el,
-                   const doe_screen_opts* o, doe_metrics* out) {
+                   const doe_screen_opts* o, doe_metrics* out, unsigned char* fixed) {
   switch (el->dtype) {
-    case DOE_F32: return run_metrics<S, float>(ctx, scr, tgt, el, o, out);
-    case DOE_F64: return run_metrics<S, double>(ctx, scr, tgt, el, o, out);
-    case DOE_F16: return run_metrics<S, __half>(ctx, scr, tgt, el, o, out);
-    case DOE_BF16: return run_metrics<S, __hip_bfloat16>(ctx, scr, tgt, el, o, out);
+    case DOE_F32: return run_metrics<S, float>(ctx, scr, tgt, el, o, out, fixed);
+    case DOE_F64: return run_metrics<S, double>(ctx, scr, tgt, el, o, out, fixed);
+    case DOE_F16: return run_metrics<S, __half>(ctx, scr, tgt, el, o, out, fixed);
+    case DOE_BF16: return run_metrics<S, __hip_bfloat16>(ctx, scr, tgt, el, o, out, fixed);
   }
   set_error("unknown dtype");
   return DOE_ERR_INVALID;
 }
 
-}  // namespace
-}  // namespace doe
-
-using namespace doe;
-
-extern "C" {
-
-int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+int check_metrics_args(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
                        const doe_field* element, const doe_screen_opts* opts, doe_metrics* out) {
   DOE_REQUIRE(ctx && screen && target && element && opts && out, DOE_ERR_INVALID, "null argument");
   DOE_REQUIRE(screen->batch == 1 && target->batch == 1 && element->batch == 1, DOE_ERR_SHAPE,
@@ -437,15 +482,66 @@ int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* t
               DOE_ERR_INVALID, "screenMetrics: grid needs both cols and rows");
   DOE_REQUIRE((long)opts->grid_cols * opts->grid_rows <= 65536, DOE_ERR_INVALID,
               "screenMetrics: at most 65536 spot cells");
-  DOE_HIP(hipSetDevice(ctx->device));
+  return DOE_OK;
+}
+
+int metrics_dispatch(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+                     const doe_field* element, const doe_screen_opts* opts, doe_metrics* out,
+                     unsigned char* fixed) {
   switch (screen->dtype) {
-    case DOE_F32: return run_metrics_el<float>(ctx, screen, target, element, opts, out);
-    case DOE_F64: return run_metrics_el<double>(ctx, screen, target, element, opts, out);
-    case DOE_F16: return run_metrics_el<__half>(ctx, screen, target, element, opts, out);
-    case DOE_BF16: return run_metrics_el<__hip_bfloat16>(ctx, screen, target, element, opts, out);
+    case DOE_F32: return run_metrics_el<float>(ctx, screen, target, element, opts, out, fixed);
+    case DOE_F64: return run_metrics_el<double>(ctx, screen, target, element, opts, out, fixed);
+    case DOE_F16: return run_metrics_el<__half>(ctx, screen, target, element, opts, out, fixed);
+    case DOE_BF16: return run_metrics_el<__hip_bfloat16>(ctx, screen, target, element, opts, out, fixed);
   }
   set_error("unknown dtype");
   return DOE_ERR_INVALID;
+}
+
+}  // namespace
+
+// Resolve the device address of `scratch` on `device` (the caller made it
+// current), once per device.  Called by doe_ctx_create.
+int metrics_scratch_init(int device) {
+  DOE_REQUIRE(device >= 0 && device < kMaxDevices, DOE_ERR_UNSUPPORTED,
+              "doe_screen_metrics_noalloc: device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[device]) return DOE_OK;
+  void* ptr = nullptr;
+  DOE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  DOE_REQUIRE(ptr, DOE_ERR_HIP, "doe_screen_metrics_noalloc: no address for the scratch array");
+  g_scratch[device] = ptr;
+  return DOE_OK;
+}
+
+}  // namespace doe
+
+using namespace doe;
+
+extern "C" {
+
+int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+                       const doe_field* element, const doe_screen_opts* opts, doe_metrics* out) {
+  int rc = check_metrics_args(ctx, screen, target, element, opts, out);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(ctx->device));
+  return metrics_dispatch(ctx, screen, target, element, opts, out, nullptr);
+}
+
+int doe_screen_metrics_noalloc(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+                               const doe_field* element, const doe_screen_opts* opts, doe_metrics* out) {
+  int rc = check_metrics_args(ctx, screen, target, element, opts, out);
+  if (rc) return rc;
+  DOE_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices, DOE_ERR_UNSUPPORTED,
+              "doe_screen_metrics_noalloc: device index beyond 63");
+  unsigned char* fixed;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    fixed = static_cast<unsigned char*>(g_scratch[ctx->device]);
+  }
+  DOE_REQUIRE(fixed, DOE_ERR_STATE, "doe_screen_metrics_noalloc: scratch not set up for this device");
+  DOE_HIP(hipSetDevice(ctx->device));
+  return metrics_dispatch(ctx, screen, target, element, opts, out, fixed);
 }
 
 int doe_screen_metrics_batch(doe_ctx* ctx, const doe_field* screens, const doe_field* target,

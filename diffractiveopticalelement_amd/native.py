@@ -167,6 +167,8 @@ SIGNATURES = [
     ("doe_screen_metrics", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts), C.POINTER(Metrics)]),
     ("doe_screen_metrics_batch", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
                                            C.POINTER(Metrics)]),
+    ("doe_screen_metrics_noalloc", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
+                                             C.POINTER(Metrics)]),
     ("doe_field_resample_centred", C.c_int, [_VP, _VP, _VP]),
     ("doe_field_copy_slices", C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, C.c_int]),
     ("doe_target_spec_init", C.c_int, [C.POINTER(TargetSpec), C.c_int]),
@@ -367,15 +369,16 @@ class Context:
 
     def screen_metrics(self, screen: "Field", target: "Field", element: "Field", sdx: float,
                        sdy: float, edx: float, edy: float, grid=None, min_peak: float = 0.05,
-                       weight_target: bool = True) -> dict:
+                       weight_target: bool = True, noalloc: bool = False) -> dict:
         """screenMetrics of a device-resident screen field (target: FP64 field with
         re = target irradiance, im = ROI; element: illumination * e^{i phase});
-        grid = (cols, rows) of a spot array or None"""
+        grid = (cols, rows) of a spot array or None.  noalloc: the entry point
+        that allocates no device memory (doe_screen_metrics_noalloc)."""
         o = ScreenOpts(sdx, sdy, edx, edy, grid[0] if grid else 0, grid[1] if grid else 0, min_peak,
                        1 if weight_target else 0)
         m = Metrics()
-        check(self.lib.doe_screen_metrics(self.h, screen.h, target.h, element.h, C.byref(o),
-                                          C.byref(m)))
+        fn = self.lib.doe_screen_metrics_noalloc if noalloc else self.lib.doe_screen_metrics
+        check(fn(self.h, screen.h, target.h, element.h, C.byref(o), C.byref(m)))
         out = {n: getattr(m, n) for n in METRIC_NAMES}
         if not m.has_spots:
             out["spot_uniformity"] = None

@@ -330,6 +330,20 @@ int doe_screen_metrics(doe_ctx* ctx, const doe_field* screen, const doe_field* t
 int doe_screen_metrics_batch(doe_ctx* ctx, const doe_field* screens, const doe_field* target,
                              const doe_field* element, const doe_screen_opts* opts,
                              doe_metrics* out);
+/* doe_screen_metrics without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as doe_screen_metrics; its
+ * reduction partials and spot-cell bounds live in a fixed scratch array of
+ * DOE_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends, set up by
+ * doe_ctx_create), which bounds the screen to about 8192 x 8192 points and
+ * the spot grid to 4096 cells (DOE_ERR_UNSUPPORTED beyond).  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (several
+ * contexts or threads at once) race; calls in sequence do not. */
+#define DOE_NOALLOC_SCRATCH_BYTES (6 << 20)
+int doe_screen_metrics_noalloc(doe_ctx* ctx, const doe_field* screen, const doe_field* target,
+                               const doe_field* element, const doe_screen_opts* opts,
+                               doe_metrics* out);
 
 /* Same-pitch resample of ComplexField#resample (ref: src/field.js:443-465):
  * centred zero-padding or crop, dst[j][i] = src[j - oy][i - ox] with
