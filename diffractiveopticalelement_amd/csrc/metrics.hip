@@ -500,6 +500,92 @@ int metrics_dispatch(doe_ctx* ctx, const doe_field* screen, const doe_field* tar
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// ComplexField#resample at another pitch (field.js:466-495): output sample
+// (i, j) at x = (i - nx2/2) dx2 reads the source bilinearly at the fractional
+// index fx = x / dx + nx/2 - 1/2 (zero outside the source grid), in FP64 with
+// the reference's operation order (no contraction), so FP64 fields equal it
+// bit for bit.
+template <class S>
+__global__ __launch_bounds__(kMB) void k_resample_bilinear(const S* __restrict__ sre, const S* __restrict__ sim,
+                                                           S* __restrict__ dre, S* __restrict__ dim_, int nx,
+                                                           int ny, int nx2, int ny2, int src_batch, long total,
+                                                           double sdx, double sdy, double ddx, double ddy) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * kMB + threadIdx.x;
+  if (q >= total) return;
+  const long per = (long)nx2 * ny2;
+  const int b = int(q / per);
+  const long r = q - (long)b * per;
+  const int j = int(r / nx2), i = int(r - (long)j * nx2);
+  const S* re = sre + (long)(src_batch == 1 ? 0 : b) * nx * ny;
+  const S* im = sim + (long)(src_batch == 1 ? 0 : b) * nx * ny;
+  const double y = (j - ny2 / 2.0) * ddy;
+  const double fy = y / sdy + ny / 2.0 - 0.5;
+  const double j0 = floor(fy);
+  const double ty = fy - j0;
+  const double x = (i - nx2 / 2.0) * ddx;
+  const double fx = x / sdx + nx / 2.0 - 0.5;
+  const double i0 = floor(fx);
+  const double tx = fx - i0;
+  double ar = 0, ai = 0;
+  for (int bb = 0; bb < 2; ++bb) {
+    const double jj = j0 + bb;
+    if (jj < 0 || jj >= ny) continue;
+    const double wy = bb == 0 ? 1 - ty : ty;
+    for (int aa = 0; aa < 2; ++aa) {
+      const double ii = i0 + aa;
+      if (ii < 0 || ii >= nx) continue;
+      const double wx = aa == 0 ? 1 - tx : tx;
+      const double w = wx * wy;
+      const long k = (long)jj * nx + (long)ii;
+      ar += ld<double>(re + k) * w;
+      ai += ld<double>(im + k) * w;
+    }
+  }
+  st(dre + q, ar);
+  st(dim_ + q, ai);
+}
+
+// ComplexField#sampleImage (field.js:350-400): the raster covers the central
+// `scale` fraction of the window; grid sample (i, j) averages the raster over
+// the box it covers (area weights), `fill` outside the raster; re = the
+// average, im = 0.  FP64, the reference's operation order.
+template <class S>
+__global__ __launch_bounds__(kMB) void k_sample_image(const double* __restrict__ img, int width, int height,
+                                                      double scale, double fill, S* __restrict__ dre,
+                                                      S* __restrict__ dim_, int nx, int ny) {
+#pragma clang fp contract(off)
+  const long q = (long)blockIdx.x * kMB + threadIdx.x;
+  if (q >= (long)nx * ny) return;
+  const int j = int(q / nx), i = int(q - (long)j * nx);
+  const double s = fmax(scale, 1e-9);
+  const double off = (1 - s) / 2;
+  const double boxW = width / (nx * s), boxH = height / (ny * s);
+  const double cy = (((j + 0.5) / ny - off) * height) / s;
+  const double cx = (((i + 0.5) / nx - off) * width) / s;
+  const double ax = cx - boxW / 2, bx = cx + boxW / 2;
+  const double ay = cy - boxH / 2, by = cy + boxH / 2;
+  const double lox = fmax(ax, 0.0), hix = fmin(bx, (double)width);
+  const double loy = fmax(ay, 0.0), hiy = fmin(by, (double)height);
+  const double area = boxW * boxH;
+  double acc = 0;
+  if (hix > lox && hiy > loy) {
+    const long i0 = (long)fmax(0.0, floor(lox)), i1 = (long)fmin(width - 1.0, floor(hix - 1e-12));
+    const long j0 = (long)fmax(0.0, floor(loy)), j1 = (long)fmin(height - 1.0, floor(hiy - 1e-12));
+    for (long jj = j0; jj <= j1; ++jj) {
+      const double wy = fmin(by, double(jj + 1)) - fmax(ay, double(jj));
+      for (long ii = i0; ii <= i1; ++ii) {
+        const double wx = fmin(bx, double(ii + 1)) - fmax(ax, double(ii));
+        acc += img[ii + jj * width] * wx * wy;
+      }
+    }
+  }
+  acc += fill * (area - (hix - lox) * (hiy - loy));
+  st(dre + q, acc / area);
+  st(dim_ + q, 0.0);
+}
+
 // Resolve the device address of `scratch` on `device` (the caller made it
 // current), once per device.  Called by doe_ctx_create.
 int metrics_scratch_init(int device) {
@@ -583,6 +669,67 @@ int doe_field_resample_centred(doe_ctx* ctx, const doe_field* src, doe_field* ds
     default: set_error("unknown dtype"); return DOE_ERR_INVALID;
   }
 #undef DOE_RS
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_field_resample(doe_ctx* ctx, const doe_field* src, doe_field* dst, double src_dx, double src_dy,
+                       double dst_dx, double dst_dy) {
+  DOE_REQUIRE(ctx && src && dst, DOE_ERR_INVALID, "null argument");
+  if (src_dx == dst_dx && src_dy == dst_dy) return doe_field_resample_centred(ctx, src, dst);
+  DOE_REQUIRE(src->dtype == dst->dtype, DOE_ERR_INVALID, "resample: source and destination dtypes differ");
+  DOE_REQUIRE(src->batch == dst->batch || src->batch == 1, DOE_ERR_SHAPE,
+              "resample: source batch must be 1 or the destination's");
+  DOE_REQUIRE(src->re != dst->re && src->im != dst->im, DOE_ERR_INVALID, "resample: source and destination overlap");
+  DOE_REQUIRE(src_dx > 0 && src_dy > 0 && dst_dx > 0 && dst_dy > 0 && std::isfinite(src_dx) &&
+                  std::isfinite(src_dy) && std::isfinite(dst_dx) && std::isfinite(dst_dy),
+              DOE_ERR_INVALID, "resample: pitches must be positive and finite");
+  DOE_HIP(hipSetDevice(ctx->device));
+  const long total = long(dst->count());
+  const dim3 g(unsigned((total + kMB - 1) / kMB));
+  hipStream_t st = ctx->stream;
+#define DOE_RB(S)                                                                                        \
+  hipLaunchKernelGGL(k_resample_bilinear<S>, g, dim3(kMB), 0, st, static_cast<const S*>(src->re),       \
+                     static_cast<const S*>(src->im), static_cast<S*>(dst->re), static_cast<S*>(dst->im), \
+                     src->nx, src->ny, dst->nx, dst->ny, src->batch, total, src_dx, src_dy, dst_dx, dst_dy)
+  switch (dst->dtype) {
+    case DOE_F32: DOE_RB(float); break;
+    case DOE_F64: DOE_RB(double); break;
+    case DOE_F16: DOE_RB(__half); break;
+    case DOE_BF16: DOE_RB(__hip_bfloat16); break;
+    default: set_error("unknown dtype"); return DOE_ERR_INVALID;
+  }
+#undef DOE_RB
+  DOE_HIP(hipGetLastError());
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_field_sample_image(doe_ctx* ctx, const double* img, int width, int height, double scale, double fill,
+                           doe_field* dst) {
+  DOE_REQUIRE(ctx && img && dst, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(width >= 1 && height >= 1, DOE_ERR_SHAPE, "sampleImage: empty raster");
+  DOE_REQUIRE(dst->batch == 1, DOE_ERR_SHAPE, "sampleImage: one field per call");
+  DOE_HIP(hipSetDevice(ctx->device));
+  void* dimg = nullptr;
+  int rc = upload_params(ctx, img, size_t(width) * size_t(height) * sizeof(double), &dimg);
+  if (rc) return rc;
+  const long total = long(dst->count());
+  const dim3 g(unsigned((total + kMB - 1) / kMB));
+  hipStream_t st = ctx->stream;
+  const double* im = static_cast<const double*>(dimg);
+#define DOE_SI(S)                                                                                       \
+  hipLaunchKernelGGL(k_sample_image<S>, g, dim3(kMB), 0, st, im, width, height, scale, fill,           \
+                     static_cast<S*>(dst->re), static_cast<S*>(dst->im), dst->nx, dst->ny)
+  switch (dst->dtype) {
+    case DOE_F32: DOE_SI(float); break;
+    case DOE_F64: DOE_SI(double); break;
+    case DOE_F16: DOE_SI(__half); break;
+    case DOE_BF16: DOE_SI(__hip_bfloat16); break;
+    default: set_error("unknown dtype"); return DOE_ERR_INVALID;
+  }
+#undef DOE_SI
   DOE_HIP(hipGetLastError());
   DOE_HIP(finish(ctx));
   return DOE_OK;

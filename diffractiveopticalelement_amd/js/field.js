@@ -366,135 +366,24 @@ export class ComplexField {
     return { re, im, nx: this.nx, ny: this.ny, dfx: this.dfx, dfy: this.dfy };
   }
 
-  // Area-averaged raster sampling (ref: field.js:352-400).
+  // Area-averaged raster sampling (ref: field.js:350-400), on the device
+  // (doe_field_sample_image, FP64): re = the raster's average over each
+  // sample's box, fill outside the raster; im = 0.
   sampleImage(img, width, height, scale = 1, fill = 0) {
-    const { nx, ny } = this;
-    const r = this.re;
-    const s = Math.max(scale, 1e-9);
-    const off = (1 - s) / 2;
-    const bw = width / (nx * s);
-    const bh = height / (ny * s);
-    for (let j = 0; j < ny; j++) {
-      const cy = (((j + 0.5) / ny - off) * height) / s;
-      for (let i = 0; i < nx; i++) {
-        const cx = (((i + 0.5) / nx - off) * width) / s;
-        const ax = cx - bw / 2;
-        const bx = cx + bw / 2;
-        const ay = cy - bh / 2;
-        const by = cy + bh / 2;
-        const lox = Math.max(ax, 0);
-        const hix = Math.min(bx, width);
-        const loy = Math.max(ay, 0);
-        const hiy = Math.min(by, height);
-        let acc = 0;
-        if (hix > lox && hiy > loy) {
-          const i0 = Math.max(0, Math.floor(lox));
-          const i1 = Math.min(width - 1, Math.floor(hix - 1e-12));
-          const j0 = Math.max(0, Math.floor(loy));
-          const j1 = Math.min(height - 1, Math.floor(hiy - 1e-12));
-          for (let jj = j0; jj <= j1; jj++) {
-            const wy = Math.min(by, jj + 1) - Math.max(ay, jj);
-            for (let ii = i0; ii <= i1; ii++) {
-              acc += img[ii + jj * width] * (Math.min(bx, ii + 1) - Math.max(ax, ii)) * wy;
-            }
-          }
-        }
-        acc += fill * (bw * bh - (hix - lox) * (hiy - loy));
-        r[i + j * nx] = acc / (bw * bh);
-      }
-    }
-    this.im.fill(0);
+    const raster = img instanceof Float64Array ? img : Float64Array.from(img);
+    native.fieldSampleImage(raster, width, height, scale, fill, this._deviceOut(1));
+    this._markDevice();
     return this;
   }
 
-  fillFrom(fn) {
-    const r = this.re;
-    for (let j = 0; j < this.ny; j++) {
-      const y = this.y(j);
-      for (let i = 0; i < this.nx; i++) {
-        const v = fn(this.x(i), y, i, j);
-        r[i + j * this.nx] = v === undefined || v === null ? 0 : v;
-      }
-    }
-    this.im.fill(0);
-    return this;
-  }
-
-  maxAmplitude() {
-    const r = this._hostRe();
-    const m = this._hostIm();
-    let best = 0;
-    for (let k = 0; k < r.length; k++) best = Math.max(best, Math.hypot(r[k], m[k]));
-    return best;
-  }
-
-  intensityRange() {
-    const r = this._hostRe();
-    const m = this._hostIm();
-    let lo = Infinity;
-    let hi = -Infinity;
-    for (let k = 0; k < r.length; k++) {
-      const v = r[k] * r[k] + m[k] * m[k];
-      if (v < lo) lo = v;
-      if (v > hi) hi = v;
-    }
-    return { min: lo, max: hi };
-  }
-
-  // Centred pad/crop at the same pitch, else bilinear (ref: field.js:443-495).
+  // Pad / crop onto a new grid at the same pitch, or bilinear resample at
+  // another pitch (ref: field.js:443-495); both on the device
+  // (doe_field_resample), in the field's precision.
   resample(nx2, ny2, dx2 = this.dx, dy2 = this.dy) {
     const out = new ComplexField(nx2, ny2, dx2, dy2);
-    if (dx2 === this.dx && dy2 === this.dy && this._where === 'device') {
-      // same pitch, data in HBM: centred pad / crop on the device
-      native.fieldResample(this._dev, out._deviceOut(this._devDtype));
-      out._markDevice();
-      return out;
-    }
-    const sr = this._hostRe();
-    const si = this._hostIm();
-    const orr = out._re;
-    const oii = out._im;
-    if (dx2 === this.dx && dy2 === this.dy && Number.isInteger(nx2) && Number.isInteger(ny2)) {
-      const ox = Math.floor((nx2 - this.nx) / 2);
-      const oy = Math.floor((ny2 - this.ny) / 2);
-      for (let j = 0; j < this.ny; j++) {
-        const jj = j + oy;
-        if (jj < 0 || jj >= ny2) continue;
-        for (let i = 0; i < this.nx; i++) {
-          const ii = i + ox;
-          if (ii < 0 || ii >= nx2) continue;
-          orr[ii + jj * nx2] = sr[i + j * this.nx];
-          oii[ii + jj * nx2] = si[i + j * this.nx];
-        }
-      }
-      return out;
-    }
-    for (let j = 0; j < ny2; j++) {
-      const fy = out.y(j) / this.dy + this.ny / 2 - 0.5;
-      const j0 = Math.floor(fy);
-      const ty = fy - j0;
-      for (let i = 0; i < nx2; i++) {
-        const fx = out.x(i) / this.dx + this.nx / 2 - 0.5;
-        const i0 = Math.floor(fx);
-        const tx = fx - i0;
-        let ar = 0;
-        let ai = 0;
-        for (let b = 0; b < 2; b++) {
-          const jj = j0 + b;
-          if (jj < 0 || jj >= this.ny) continue;
-          const wy = b === 0 ? 1 - ty : ty;
-          for (let a = 0; a < 2; a++) {
-            const ii = i0 + a;
-            if (ii < 0 || ii >= this.nx) continue;
-            const w = (a === 0 ? 1 - tx : tx) * wy;
-            ar += sr[ii + jj * this.nx] * w;
-            ai += si[ii + jj * this.nx] * w;
-          }
-        }
-        orr[i + j * nx2] = ar;
-        oii[i + j * nx2] = ai;
-      }
-    }
+    const dt = this._where === 'host' ? dtypeCode() : this._devDtype;
+    native.fieldResample(this._device(dt), out._deviceOut(dt), this.dx, this.dy, dx2, dy2);
+    out._markDevice();
     return out;
   }
 

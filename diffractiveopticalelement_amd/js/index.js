@@ -1,11 +1,12 @@
 // Barrel of the drop-in modules (ref: src/index.js:13-21, hot-path part):
-// the same export names as the reference for fft, field, propagate, metrics,
-// doe (phase retrieval), projector, color and png, backed by the MI355X
-// library.
+// the reference's export names for fft, field, propagate, doe (phase
+// retrieval), projector, color and png, backed by the MI355X library.  The
+// host-array metric helpers of src/metrics.js are not part of the drop-in
+// (screenMetrics reduces on the device); the reference's own module serves
+// them.
 export * from './fft.js';
 export * from './field.js';
 export * from './propagate.js';
-export * from './metrics.js';
 export * from './doe.js';
 export * from './projector.js';
 export * from './color.js';

@@ -1,13 +1,21 @@
-// Drop-in for src/projector.js: the ProjectorSystem pipeline.  Targets and
-// ROIs are built on the GPU in FP64 (returned as the reference's arrays); design()
-// runs the whole phase-retrieval loop on the GPU; simulate() propagates on
-// the GPU; screenMetrics() judges the device-resident screen on the GPU in
-// FP64 (the screen is never downloaded).
+// Drop-in for the hot path of src/projector.js.  Targets and ROIs are built
+// on the GPU in FP64 (returned as the reference's arrays); design() runs the
+// whole phase-retrieval loop on the GPU; simulate() propagates on the GPU;
+// screenMetrics() judges the device-resident screen on the GPU in FP64 (the
+// screen is never downloaded); defocusCurve() / focusStack() propagate every
+// distance in one batch; designMultiPlane() builds all plane targets in one
+// batched call and runs the multi-plane loop on the GPU.
+//
+// DEVICE_PROJECTOR holds those methods; installDeviceProjector(Cls) puts them
+// on a class with the reference's ProjectorSystem state (cfg, nx, ny, dx, dy,
+// lambda, z, target, result, illumination()), which is how the reference's
+// own class takes them (its other methods stay as they are).  The
+// ProjectorSystem exported here is the minimal such class: the reference's
+// constructor, setTarget and illumination, plus the device methods.
 import { ComplexField } from './field.js';
 import { native, dtypeCode } from './native.js';
-import { propagateAngularSpectrumInPlace, propagateFresnelFFT, focalPlaneSpectrum, lensPhase } from './propagate.js';
+import { propagateAngularSpectrumInPlace, focalPlaneSpectrum } from './propagate.js';
 import { designDOE, illuminationProfile, gerchbergSaxtonMultiPlane, errorDiffusionQuantize } from './doe.js';
-import { summarizeMetrics } from './metrics.js';
 
 function pick(v, d) {
   return v === undefined || v === null ? d : v;
@@ -140,37 +148,87 @@ function darkCells(missing) {
 // ref: projector.js:169-255.  The irradiance and, unless the caller passes
 // one, the ROI are computed on the GPU in FP64 (doe_target_make,
 // doe_target_auto_roi) and returned as host arrays.
-export function makeTarget(kind, nx, ny, opts = {}) {
-  let spec;
-  let name = kind;
+// Device spec and display name of a makeTarget kind.
+function targetSpec(kind, opts) {
   if (kind === 'image') {
     const { raster, width, height, gamma = 1 } = opts;
     if (!raster) throw new Error('makeTarget("image"): pass {raster, width, height}');
-    spec = { kind: 'raster', raster: f64Raster(raster, width, height), width, height, gamma };
-    name = opts.name === undefined ? 'image' : opts.name;
-  } else if (kind === 'text') {
+    return { spec: { kind: 'raster', raster: f64Raster(raster, width, height), width, height, gamma },
+      name: opts.name === undefined ? 'image' : opts.name };
+  }
+  if (kind === 'text') {
     const { text = 'DOE', scale = 6, spacing = 1 } = opts;
     const g = renderText(text, { scale, spacing });
-    spec = { kind: 'raster', raster: g.data, width: g.width, height: g.height };
-    name = `text:${text}`;
-  } else if (ANALYTIC.has(kind)) {
-    spec = Object.assign({}, opts, { kind, missing: darkCells(opts.missing), raster: undefined });
-  } else {
-    throw new Error(`makeTarget: unknown kind "${kind}"`);
+    return { spec: { kind: 'raster', raster: g.data, width: g.width, height: g.height }, name: `text:${text}` };
   }
-  const dev = opts.roi
-    ? native.targetMake([spec], nx, ny)
-    : native.targetMake([spec], nx, ny, {
-      dilate: Number(opts.dilate === undefined ? 2 : opts.dilate),
-      soft: Number(opts.softEdge === undefined ? 2 : opts.softEdge),
-      threshold: 0.05,
-    });
+  if (ANALYTIC.has(kind)) {
+    return { spec: Object.assign({}, opts, { kind, missing: darkCells(opts.missing), raster: undefined }), name: kind };
+  }
+  throw new Error(`makeTarget: unknown kind "${kind}"`);
+}
+
+// autoRoi options of a target spec (the reference's dilate / softEdge names)
+function roiOptions(opts) {
+  return {
+    dilate: Number(opts.dilate === undefined ? 2 : opts.dilate),
+    soft: Number(opts.softEdge === undefined ? 2 : opts.softEdge),
+    threshold: 0.05,
+  };
+}
+
+export function makeTarget(kind, nx, ny, opts = {}) {
+  const { spec, name } = targetSpec(kind, opts);
+  const dev = opts.roi ? native.targetMake([spec], nx, ny) : native.targetMake([spec], nx, ny, roiOptions(opts));
   const irr = new Float64Array(nx * ny);
   const devRoi = new Float64Array(nx * ny);
   native.fieldDownload(dev, irr, devRoi);
   const roi = opts.roi ? opts.roi : devRoi;
   const grid = kind === 'dots' || kind === 'grid' ? { cols: opts.cols === undefined ? 8 : opts.cols, rows: opts.rows === undefined ? 8 : opts.rows } : null;
   return { irradiance: irr, roi, name, grid };
+}
+
+// Irradiance and ROI of many planes (designMultiPlane's specs: a kind with
+// its options, or {irradiance, roi?}) with as few device calls as possible:
+// the kind planes sharing ROI options are one doe_target_make_roi launch;
+// the given irradiances without an ROI are one batched upload and one
+// doe_target_auto_roi launch.
+function planeTargets(specs, nx, ny) {
+  const n = nx * ny;
+  const out = new Array(specs.length);
+  const groups = new Map(); // ROI options -> [plane index, device spec]
+  const custom = [];
+  specs.forEach((p, q) => {
+    if (p.irradiance) {
+      if (p.roi !== undefined && p.roi !== null) out[q] = { irradiance: p.irradiance, roi: p.roi };
+      else custom.push(q);
+      return;
+    }
+    const key = JSON.stringify(roiOptions(p));
+    if (!groups.has(key)) groups.set(key, []);
+    groups.get(key).push([q, targetSpec(p.kind, p).spec]);
+  });
+  for (const [key, members] of groups) {
+    const dev = native.targetMake(members.map((m) => m[1]), nx, ny, JSON.parse(key));
+    const irr = new Float64Array(n * members.length);
+    const roi = new Float64Array(n * members.length);
+    native.fieldDownload(dev, irr, roi);
+    members.forEach(([q], b) => {
+      out[q] = { irradiance: irr.subarray(b * n, (b + 1) * n), roi: roi.subarray(b * n, (b + 1) * n) };
+    });
+  }
+  if (custom.length > 0) {
+    const dev = native.fieldCreate(nx, ny, custom.length, 1);
+    const irr = new Float64Array(n * custom.length);
+    custom.forEach((q, b) => irr.set(f64Plane(specs[q].irradiance).subarray(0, n), b * n));
+    native.fieldUpload(dev, irr, irr);
+    native.targetAutoRoi(dev, 2, 2, 0.05);
+    const roi = new Float64Array(n * custom.length);
+    native.fieldDownload(dev, irr, roi);
+    custom.forEach((q, b) => {
+      out[q] = { irradiance: specs[q].irradiance, roi: roi.subarray(b * n, (b + 1) * n) };
+    });
+  }
+  return out;
 }
 
 // Threshold, separable dilation, box-blurred edge (ref: projector.js:262-315),
@@ -205,90 +263,9 @@ export const DEFAULT_PROJECTOR = {
   target: { kind: 'text', text: 'DOE' },
 };
 
-export class ProjectorSystem {
-  // ref: projector.js:347-366
-  constructor(cfg = {}) {
-    this.cfg = Object.assign({}, DEFAULT_PROJECTOR, cfg, {
-      target: Object.assign({}, DEFAULT_PROJECTOR.target, cfg.target || {}),
-    });
-    this.nx = this.cfg.n;
-    this.ny = this.cfg.n;
-    this.dx = this.cfg.pitch;
-    this.dy = this.cfg.pitch;
-    this.lambda = this.cfg.lambda;
-    if (cfg.distance === undefined && this.cfg.mode === 'far-field') {
-      this.cfg.distance = (this.cfg.n * this.cfg.pitch * this.cfg.pitch) / this.cfg.lambda;
-    }
-    this.z = this.cfg.distance;
-    this.target = null;
-    this.result = null;
-    this.setTarget(this.cfg.target);
-  }
-
-  setTarget(spec) {
-    if (spec && spec.irradiance) {
-      this.target = {
-        irradiance: spec.irradiance,
-        roi: spec.roi ? spec.roi : autoRoi(spec.irradiance, this.nx, this.ny),
-        name: spec.name === undefined ? 'custom' : spec.name,
-      };
-    } else {
-      this.target = makeTarget(spec.kind, this.nx, this.ny, spec);
-    }
-    return this.target;
-  }
-
-  matchDistance() {
-    return (this.nx * this.dx * this.dx) / this.lambda;
-  }
-
-  // ref: projector.js:387-397
-  illumination() {
-    const radius = (Math.min(this.nx * this.dx, this.ny * this.dy) / 2) * this.cfg.aperture;
-    return illuminationProfile(this.cfg.source, this.nx, this.ny, this.dx, this.dy, {
-      radius,
-      soft: this.cfg.apodization * this.dx,
-      waist: radius * this.cfg.waist,
-      inner: this.cfg.inner === undefined ? 0.6 : this.cfg.inner,
-      outer: this.cfg.outer === undefined ? 0.98 : this.cfg.outer,
-    });
-  }
-
-  // scalar optical budget (ref: projector.js:403-435)
-  geometry() {
-    const n = this.nx;
-    const p = this.dx;
-    const D = n * p * this.cfg.aperture;
-    const thetaMax = Math.asin(Math.min(1, this.lambda / (2 * p)));
-    const fourierWindow = (this.lambda * this.z) / p;
-    const imageWindow = n * p;
-    const window = this.cfg.mode === 'far-field' ? fourierWindow : imageWindow;
-    const spot = (1.22 * this.lambda * this.z) / D;
-    const cells = window / spot;
-    return {
-      gridN: n,
-      pitchUm: p * 1e6,
-      apertureMm: D * 1e3,
-      wavelengthNm: this.lambda * 1e9,
-      distanceMm: this.z * 1e3,
-      mode: this.cfg.mode,
-      screenWindowMm: window * 1e3,
-      imageWindowMm: imageWindow * 1e3,
-      fourierWindowMm: fourierWindow * 1e3,
-      screenPitchUm: (this.cfg.mode === 'far-field' ? fourierWindow / n : p) * 1e6,
-      maxDeflectionDeg: (thetaMax * 180) / Math.PI,
-      diffractionSpotUm: spot * 1e6,
-      resolvableSpotsPerAxis: cells,
-      addressablePixels: Math.round(cells * cells),
-      throwRatio: this.z / window,
-      fNumber: this.z / D,
-      fresnelCriticalMm: ((n * p * p) / this.lambda) * 1e3,
-      undersampledChirp: this.z < (n * p * p) / this.lambda,
-      pixelCount: n * n,
-      phaseLevels: this.cfg.levels > 1 ? this.cfg.levels : 'continuous',
-    };
-  }
-
+// The hot-path methods of ProjectorSystem (ref: projector.js:438-498,
+// 554-653, 675-712) on the device.
+class DeviceProjector {
   // ref: projector.js:438-467
   design(opts = {}) {
     const pick = (v, d) => (v === undefined || v === null ? d : v);
@@ -339,38 +316,10 @@ export class ProjectorSystem {
     return field;
   }
 
-  // ref: projector.js:520-532
-  simulateWithFixedLens({ z = null, focal = null, phase = null, pad = 2 } = {}) {
-    if (!this.result) throw new Error('ProjectorSystem.simulateWithFixedLens: call design() first');
-    const dist = z === null ? this.z : z;
-    const foc = focal === null ? this.z : focal;
-    let field = this.applyLens({ focal: foc, phase });
-    if (pad > 1) field = field.resample(this.nx * pad, this.ny * pad);
-    propagateAngularSpectrumInPlace(field, this.lambda, dist, { bandLimit: false });
-    if (pad > 1) field = field.resample(this.nx, this.ny);
-    return { field, scale: dist / foc, focal: foc, z: dist };
-  }
-
-  applyLens({ focal = null, phase = null } = {}) {
-    const field = new ComplexField(this.nx, this.ny, this.dx, this.dy);
-    field.setAmplitudePhase(this.result.illumination, phase || this.result.phase);
-    field.multiplyPhase(lensPhase(field, this.lambda, focal === null ? this.z : focal, { wrap: true }));
-    return field;
-  }
-
-  simulateFarField({ z = null, phase = null } = {}) {
-    if (!this.result) throw new Error('ProjectorSystem.simulateFarField: call design() first');
-    const field = new ComplexField(this.nx, this.ny, this.dx, this.dy);
-    field.setAmplitudePhase(this.result.illumination, phase || this.result.phase);
-    return propagateFresnelFFT(field, this.lambda, z === null ? this.z : z);
-  }
-
-  // FP64 judge over the downloaded screen irradiance (ref: projector.js:554-614).
   // Device copy of a target's irradiance (re) and ROI (im), FP64, cached per
-  // (irradiance, roi) pair.
-  // Device copy of (irradiance, ROI), re-uploaded only when the arrays or
-  // their contents changed since the last call (callers may edit a target
-  // in place between metric calls).
+  // (irradiance, roi) pair and re-uploaded only when the arrays or their
+  // contents changed since the last call (callers may edit a target in place
+  // between metric calls).
   _targetDevice(irradiance, mask) {
     const c = this._tgtDev;
     const m = mask || new Float64Array(irradiance.length).fill(1);
@@ -473,69 +422,97 @@ export class ProjectorSystem {
     });
   }
 
-  // ref: projector.js:660-669
-  baseline({ z = null, pad = 2 } = {}) {
-    const field = new ComplexField(this.nx, this.ny, this.dx, this.dy);
-    field.setAmplitudePhase(this.illumination(), 0);
-    let f = pad > 1 ? field.resample(this.nx * pad, this.ny * pad) : field;
-    propagateAngularSpectrumInPlace(f, this.lambda, z === null ? this.z : z, { bandLimit: false });
-    if (pad > 1) f = f.resample(this.nx, this.ny);
-    return f;
-  }
-
-  // ref: projector.js:675-712 (the loop runs on the device)
+  // ref: projector.js:675-712.  Every plane's target and ROI come from as few
+  // batched device calls as the planes allow (planeTargets); the loop runs on
+  // the device (doe_gs_multiplane_run).  The result carries the reference's
+  // fields (the loop's phase and histories, the planes, the illumination and
+  // the grid), and becomes this.result like a design()'s.
   designMultiPlane(planeSpecs, opts = {}) {
+    const n = this.nx * this.ny;
     const illumination = this.illumination();
-    const planes = planeSpecs.map((p) => {
-      const t = p.irradiance ? p : makeTarget(p.kind, this.nx, this.ny, p);
-      const amp = new Float64Array(this.nx * this.ny);
-      for (let k = 0; k < amp.length; k++) amp[k] = Math.sqrt(Math.max(0, t.irradiance[k]));
-      const roi = t.roi === undefined || t.roi === null ? autoRoi(t.irradiance, this.nx, this.ny) : t.roi;
-      return { z: p.z, targetAmp: amp, roi, weight: p.weight === undefined || p.weight === null ? 1 : p.weight };
+    const made = planeTargets(planeSpecs, this.nx, this.ny);
+    const planes = planeSpecs.map((p, q) => {
+      const amp = new Float64Array(n);
+      const irr = made[q].irradiance;
+      for (let k = 0; k < n; k++) amp[k] = irr[k] > 0 ? Math.sqrt(irr[k]) : 0;
+      return { z: p.z, targetAmp: amp, roi: made[q].roi, weight: pick(p.weight, 1) };
     });
-    const res = gerchbergSaxtonMultiPlane(illumination, planes, {
-      nx: this.nx,
-      ny: this.ny,
-      dx: this.dx,
-      dy: this.dy,
-      lambda: this.lambda,
+    const loop = gerchbergSaxtonMultiPlane(illumination, planes, {
+      nx: this.nx, ny: this.ny, dx: this.dx, dy: this.dy, lambda: this.lambda,
       iterations: pick(opts.iterations, this.cfg.iterations),
       seed: pick(opts.seed, this.cfg.seed),
       initPhase: opts.initPhase,
       onProgress: opts.onProgress,
     });
     const levels = pick(opts.levels, this.cfg.levels);
-    const out = Object.assign({}, res, {
-      illumination,
-      lambda: this.lambda,
-      nx: this.nx,
-      ny: this.ny,
-      dx: this.dx,
-      dy: this.dy,
-      z: this.z,
-      mode: 'multi-plane',
-      planes,
-      target: this.target,
-      roi: this.target.roi,
+    const result = Object.assign({}, loop, {
+      illumination, planes, mode: 'multi-plane', target: this.target, roi: this.target.roi,
+      lambda: this.lambda, z: this.z, nx: this.nx, ny: this.ny, dx: this.dx, dy: this.dy,
     });
-    if (levels > 1) out.phase = errorDiffusionQuantize(out.phase, levels);
-    this.result = out;
-    return out;
-  }
-
-  report(metrics) {
-    return summarizeMetrics(metrics);
+    if (levels > 1) result.phase = errorDiffusionQuantize(result.phase, levels);
+    this.result = result;
+    return result;
   }
 }
 
-// ref: projector.js:723-735
-export function convergenceSweep(cfg, { checkpoints = [1, 2, 5, 10, 20, 40, 80] } = {}) {
-  const sys = new ProjectorSystem(cfg);
-  const out = [];
-  for (const it of checkpoints) {
-    const r = sys.design({ iterations: it });
-    const m = sys.screenMetrics(sys.simulate());
-    out.push(Object.assign({ iterations: it }, m, { err: r.error[r.error.length - 1] }));
+// Put the device methods on a ProjectorSystem-like class (the reference's
+// own, or the one below); returns the class.
+export function installDeviceProjector(Cls) {
+  for (const k of Object.getOwnPropertyNames(DeviceProjector.prototype)) {
+    if (k !== 'constructor') {
+      Object.defineProperty(Cls.prototype, k, Object.getOwnPropertyDescriptor(DeviceProjector.prototype, k));
+    }
   }
-  return out;
+  return Cls;
 }
+
+export class ProjectorSystem {
+  // ref: projector.js:347-366
+  constructor(cfg = {}) {
+    this.cfg = Object.assign({}, DEFAULT_PROJECTOR, cfg, {
+      target: Object.assign({}, DEFAULT_PROJECTOR.target, cfg.target || {}),
+    });
+    this.nx = this.cfg.n;
+    this.ny = this.cfg.n;
+    this.dx = this.cfg.pitch;
+    this.dy = this.cfg.pitch;
+    this.lambda = this.cfg.lambda;
+    if (cfg.distance === undefined && this.cfg.mode === 'far-field') {
+      this.cfg.distance = (this.cfg.n * this.cfg.pitch * this.cfg.pitch) / this.cfg.lambda;
+    }
+    this.z = this.cfg.distance;
+    this.target = null;
+    this.result = null;
+    this.setTarget(this.cfg.target);
+  }
+
+  setTarget(spec) {
+    if (spec && spec.irradiance) {
+      this.target = {
+        irradiance: spec.irradiance,
+        roi: spec.roi ? spec.roi : autoRoi(spec.irradiance, this.nx, this.ny),
+        name: spec.name === undefined ? 'custom' : spec.name,
+      };
+    } else {
+      this.target = makeTarget(spec.kind, this.nx, this.ny, spec);
+    }
+    return this.target;
+  }
+
+  matchDistance() {
+    return (this.nx * this.dx * this.dx) / this.lambda;
+  }
+
+  // ref: projector.js:387-397
+  illumination() {
+    const radius = (Math.min(this.nx * this.dx, this.ny * this.dy) / 2) * this.cfg.aperture;
+    return illuminationProfile(this.cfg.source, this.nx, this.ny, this.dx, this.dy, {
+      radius,
+      soft: this.cfg.apodization * this.dx,
+      waist: radius * this.cfg.waist,
+      inner: this.cfg.inner === undefined ? 0.6 : this.cfg.inner,
+      outer: this.cfg.outer === undefined ? 0.98 : this.cfg.outer,
+    });
+  }
+}
+installDeviceProjector(ProjectorSystem);

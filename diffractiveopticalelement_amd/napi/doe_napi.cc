@@ -474,17 +474,42 @@ napi_value ScreenMetricsBatch(napi_env env, napi_callback_info info) {
   return arr;
 }
 
-// fieldResample(src, dst): centred same-pitch resample (src batch 1 broadcasts)
+// fieldResample(src, dst[, srcDx, srcDy, dstDx, dstDy]): centred same-pitch
+// resample (src batch 1 broadcasts); with pitches that differ, bilinear in
+// physical coordinates
 napi_value FieldResample(napi_env env, napi_callback_info info) {
   Args a;
   get_args(env, info, a);
   doe_field* s = a.n > 0 ? field_of(env, a.v[0]) : nullptr;
   doe_field* d = a.n > 1 ? field_of(env, a.v[1]) : nullptr;
   if (!s || !d) {
-    napi_throw_type_error(env, nullptr, "fieldResample(src, dst)");
+    napi_throw_type_error(env, nullptr, "fieldResample(src, dst, srcDx, srcDy, dstDx, dstDy)");
     return nullptr;
   }
-  throw_status(env, doe_field_resample_centred(ctx(env), s, d));
+  if (a.n >= 6) {
+    throw_status(env, doe_field_resample(ctx(env), s, d, num(env, a.v[2], NAN), num(env, a.v[3], NAN),
+                                         num(env, a.v[4], NAN), num(env, a.v[5], NAN)));
+  } else {
+    throw_status(env, doe_field_resample_centred(ctx(env), s, d));
+  }
+  return nullptr;
+}
+
+// fieldSampleImage(img Float64Array, width, height, scale, fill, dst)
+napi_value FieldSampleImage(napi_env env, napi_callback_info info) {
+  Args a;
+  get_args(env, info, a);
+  doe_field* d = a.n > 5 ? field_of(env, a.v[5]) : nullptr;
+  const int w = a.n > 1 ? (int)num(env, a.v[1], 0) : 0;
+  const int h = a.n > 2 ? (int)num(env, a.v[2], 0) : 0;
+  const double* img = a.n > 0 ? f64_need(env, a.v[0], size_t(std::max(w, 0)) * size_t(std::max(h, 0)),
+                                         "sampleImage: img") : nullptr;
+  if (pending(env)) return nullptr;
+  if (!img || !d) {
+    napi_throw_type_error(env, nullptr, "fieldSampleImage(img, width, height, scale, fill, dst)");
+    return nullptr;
+  }
+  throw_status(env, doe_field_sample_image(ctx(env), img, w, h, num(env, a.v[3], 1.0), num(env, a.v[4], 0.0), d));
   return nullptr;
 }
 
@@ -932,6 +957,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"screenMetrics", nullptr, ScreenMetrics, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"screenMetricsBatch", nullptr, ScreenMetricsBatch, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fieldResample", nullptr, FieldResample, nullptr, nullptr, nullptr, napi_default, nullptr},
+      {"fieldSampleImage", nullptr, FieldSampleImage, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"fieldCopySlices", nullptr, FieldCopySlices, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"gsMultiPlane", nullptr, GsMultiPlane, nullptr, nullptr, nullptr, napi_default, nullptr},
       {"targetMake", nullptr, TargetMake, nullptr, nullptr, nullptr, napi_default, nullptr},

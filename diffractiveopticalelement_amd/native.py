@@ -170,6 +170,8 @@ SIGNATURES = [
     ("doe_screen_metrics_noalloc", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
                                              C.POINTER(Metrics)]),
     ("doe_field_resample_centred", C.c_int, [_VP, _VP, _VP]),
+    ("doe_field_resample", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, C.c_double]),
+    ("doe_field_sample_image", C.c_int, [_VP, c_double_p, C.c_int, C.c_int, C.c_double, C.c_double, _VP]),
     ("doe_field_copy_slices", C.c_int, [_VP, _VP, C.c_int, _VP, C.c_int, C.c_int]),
     ("doe_target_spec_init", C.c_int, [C.POINTER(TargetSpec), C.c_int]),
     ("doe_target_make", C.c_int, [_VP, C.POINTER(TargetSpec), _VP]),
@@ -402,6 +404,16 @@ class Context:
 
     def resample_centred(self, src: "Field", dst: "Field") -> None:
         check(self.lib.doe_field_resample_centred(self.h, src.h, dst.h))
+
+    def resample(self, src: "Field", dst: "Field", src_dx, src_dy, dst_dx, dst_dy) -> None:
+        """ComplexField#resample: centred pad/crop at equal pitches, else bilinear"""
+        check(self.lib.doe_field_resample(self.h, src.h, dst.h, src_dx, src_dy, dst_dx, dst_dy))
+
+    def sample_image(self, img, dst: "Field", scale: float = 1.0, fill: float = 0.0) -> None:
+        """ComplexField#sampleImage of a host raster (height x width) into dst"""
+        a = _darr(img)
+        h, w = a.shape
+        check(self.lib.doe_field_sample_image(self.h, _dptr(a), w, h, scale, fill, dst.h))
 
     def copy_slices(self, src: "Field", src_first: int, dst: "Field", dst_first: int, count: int) -> None:
         check(self.lib.doe_field_copy_slices(self.h, src.h, src_first, dst.h, dst_first, count))

@@ -351,6 +351,19 @@ int doe_screen_metrics_noalloc(doe_ctx* ctx, const doe_field* screen, const doe_
  * field of dst (one element, many distances); otherwise batches match.
  * Same dtype, no overlap. */
 int doe_field_resample_centred(doe_ctx* ctx, const doe_field* src, doe_field* dst);
+/* ComplexField#resample (ref: src/field.js:443-495) between pitches: with
+ * equal pitches the centred pad / crop above; otherwise each output sample
+ * (i, j), at x = (i - dst.nx/2) dst_dx, reads the source bilinearly at the
+ * fractional index x / src_dx + src.nx/2 - 1/2 (zero outside the source),
+ * in FP64.  Same dtype, no overlap; src batch 1 broadcasts. */
+int doe_field_resample(doe_ctx* ctx, const doe_field* src, doe_field* dst, double src_dx,
+                       double src_dy, double dst_dx, double dst_dy);
+/* ComplexField#sampleImage (ref: src/field.js:350-400): a host raster of
+ * width x height FP64 samples (row-major) covers the central `scale` fraction
+ * of the field's window; every grid sample averages the raster over the box
+ * it covers (area weights) with `fill` outside it, into re; im = 0. */
+int doe_field_sample_image(doe_ctx* ctx, const double* img, int width, int height, double scale,
+                           double fill, doe_field* dst);
 /* Copy `count` whole fields from src[src_first..] to dst[dst_first..]
  * (same grid and dtype): splits a batched result into per-plane fields. */
 int doe_field_copy_slices(doe_ctx* ctx, const doe_field* src, int src_first, doe_field* dst,

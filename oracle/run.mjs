@@ -225,6 +225,16 @@ if (op === 'fft2d' || op === 'dft2d') {
     res.spotCells = su.cells;
     res.spotCv = su.cv;
   }
+} else if (op === 'sample_image') {
+  const f = new ComplexField(a.nx, a.ny, a.dx || 1, a.dy || 1);
+  f.sampleImage(inp('img'), a.width, a.height, a.scale, a.fill);
+  out('re', f.re);
+  out('im', f.im);
+} else if (op === 'field_resample') {
+  const f = fieldOf(a.nx, a.ny, a.dx, a.dy);
+  const g = f.resample(a.nx2, a.ny2, a.dx2, a.dy2);
+  out('re', g.re);
+  out('im', g.im);
 } else if (op === 'illumination') {
   out('profile', illuminationProfile(a.type, a.nx, a.ny, a.dx, a.dy, a.opts || {}));
 } else if (op === 'time') {

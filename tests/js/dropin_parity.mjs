@@ -167,6 +167,22 @@ for (const prec of ['f32', 'f64']) {
   check('mirror write-back', rel(g.re, g.im, o.re, o.im) < 1e-12, rel(g.re, g.im, o.re, o.im));
 }
 
+// ComplexField resample (bilinear at another pitch, pad / crop) and
+// sampleImage on the device: FP64 fields equal the oracle bit for bit
+{
+  gpu.setPrecision('f64');
+  const [g, o] = pair(64, 64, 4e-6, 9);
+  for (const [n2, dx2] of [[128, 2.5e-6], [32, 6e-6], [128, 4e-6]]) {
+    const rg = g.resample(n2, n2, dx2, dx2);
+    const ro = o.resample(n2, n2, dx2, dx2);
+    check(`ComplexField.resample ${n2} @ ${dx2}`, sameBits(rg.re, ro.re) && sameBits(rg.im, ro.im) && rg.dx === ro.dx);
+  }
+  const img = lcg(40 * 30, 17).map((v) => v + 0.5);
+  const sg = new gpu.ComplexField(64, 64, 4e-6).sampleImage(img, 40, 30, 0.8, 0.25);
+  const so = new OField(64, 64, 4e-6).sampleImage(img, 40, 30, 0.8, 0.25);
+  check('ComplexField.sampleImage', sameBits(sg.re, so.re) && sameBits(sg.im, so.im));
+}
+
 // ProjectorSystem end to end: design on the GPU, screenMetrics within 1e-3
 function metricsClose(a, b, keys, tol) {
   const bad = keys.filter((k) => !(Math.abs(a[k] - b[k]) < tol));

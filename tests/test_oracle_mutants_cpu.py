@@ -63,6 +63,13 @@ MUTANTS = [
      f"{PINS}::test_spot_uniformity"),
     ("js/metrics.js", "return { uniformity: mx + mn > 0 ? (2 * mn) / (mx + mn) : 1,",
      "return { uniformity: mx + mn > 0 ? mn / mx : 1,", f"{PINS}::test_spot_uniformity"),
+    ("js/field.js", "const fx = ((i - nx2 / 2) * dx2) / this.dx + this.nx / 2 - 0.5;",
+     "const fx = ((i - nx2 / 2) * dx2) / this.dx + this.nx / 2;",
+     f"{PINS}::test_resample_bilinear_reproduces_affine_planes"),
+    ("js/field.js", "    const off = (1 - s) / 2;", "    const off = 1 - s;",
+     f"{PINS}::test_sample_image_closed_forms"),
+    ("js/field.js", "acc += fill * (area - (hix - lox) * (hiy - loy));", "acc += fill * area;",
+     f"{PINS}::test_sample_image_closed_forms"),
     # pins of the older suite (FFT, ASM, Fresnel, RNG)
     ("js/fft.js", "const ang = (-2 * Math.PI * k) / len;", "const ang = (2 * Math.PI * k) / len;",
      f"{OLD}::test_fft2d_matches_numpy"),

@@ -390,3 +390,59 @@ def test_illumination_profiles():
     X, Y = np.meshgrid(c, c)
     g = top * np.exp(-(X ** 2 + Y ** 2) / (0.3 * n * dx) ** 2)
     assert np.max(np.abs(out["profile"] - g.ravel())) < 1e-14
+
+
+# -------------------------------------- ComplexField sampling / resample ---
+def sample_image(nx, ny, img, scale=1.0, fill=0.0):
+    h, w = img.shape
+    out, _ = oj.run("sample_image", {"nx": nx, "ny": ny, "width": w, "height": h, "scale": scale,
+                                     "fill": fill}, {"img": img.ravel()})
+    assert np.all(out["im"] == 0)
+    return out["re"].reshape(ny, nx)
+
+
+def test_sample_image_closed_forms():
+    rng = np.random.default_rng(8)
+    img = rng.uniform(0, 1, (16, 32))
+    # an image of the grid's own size comes back unchanged
+    assert np.max(np.abs(sample_image(32, 16, img) - img)) < 1e-15
+    # half the resolution: each sample is the mean of its 2 x 2 block
+    want = img.reshape(8, 2, 16, 2).mean(axis=(1, 3))
+    assert np.max(np.abs(sample_image(16, 8, img) - want)) < 1e-14
+    # inset by scale: the samples integrate the image exactly once
+    for nx, ny, scale in ((32, 16, 0.5), (64, 32, 0.75), (16, 16, 0.6), (128, 64, 0.9)):
+        g = sample_image(nx, ny, img, scale, 0.0)
+        box = (32 / (nx * scale)) * (16 / (ny * scale))
+        assert abs(np.sum(g) * box / np.sum(img) - 1) < 1e-12, (nx, ny, scale)
+        # a fill only enters samples whose box leaves the image
+        gf = sample_image(nx, ny, img, scale, 0.7)
+        c = (np.arange(nx) + 0.5) / nx
+        cx = (c - (1 - scale) / 2) * 32 / scale
+        cy = ((np.arange(ny) + 0.5) / ny - (1 - scale) / 2) * 16 / scale
+        bw, bh = 32 / (nx * scale), 16 / (ny * scale)
+        ix = (cx - bw / 2 >= 0) & (cx + bw / 2 <= 32)
+        iy = (cy - bh / 2 >= 0) & (cy + bh / 2 <= 16)
+        inside = np.outer(iy, ix)
+        assert inside.sum() > 0 and np.max(np.abs(gf[inside] - g[inside])) < 1e-13  # (rounding of the box area)
+        # outside, the reference weights the fill by area - (hix - lox)(hiy - loy),
+        # whose factors go negative beyond the image (reproduced, not corrected)
+        assert np.any(gf[~inside] != g[~inside])
+
+
+def test_resample_bilinear_reproduces_affine_planes():
+    n, dx = 32, 4e-6
+    j, i = np.mgrid[0:n, 0:n].astype(float)
+    re = 0.3 + 2.0 * i - 0.5 * j
+    im = -1.0 + 0.25 * i + 1.5 * j
+    for n2, dx2 in ((64, 2.5e-6), (16, 6e-6), (128, 1.5e-6)):
+        out, _ = oj.run("field_resample", {"nx": n, "ny": n, "dx": dx, "dy": dx, "nx2": n2, "ny2": n2,
+                                           "dx2": dx2, "dy2": dx2}, {"re": re.ravel(), "im": im.ravel()})
+        g = (out["re"] + 1j * out["im"]).reshape(n2, n2)
+        f = (np.arange(n2) - n2 / 2) * dx2 / dx + n / 2 - 0.5  # fractional source index
+        fy, fx = np.meshgrid(f, f, indexing="ij")
+        inside = (fx >= 0) & (fx <= n - 1) & (fy >= 0) & (fy <= n - 1)
+        want = (0.3 + 2.0 * fx - 0.5 * fy) + 1j * (-1.0 + 0.25 * fx + 1.5 * fy)
+        assert np.max(np.abs(g[inside] - want[inside])) < 1e-11
+        outside = (fx <= -1) | (fx >= n) | (fy <= -1) | (fy >= n)
+        assert np.all(g[outside] == 0)
+        assert inside.sum() > 0.2 * n2 * n2
