@@ -157,6 +157,66 @@ __global__ void k_ws_phase(const float* __restrict__ re, const float* __restrict
   phi[i] = atan2f(cb * im[i], cb * re[i]);
 }
 
+// Per image d of the pooled inputs x (res x res): the centroid (cu, cv) and
+// the RMS radius of x about it, as weights; mom[d] = {cu, cv, r}.  One
+// workgroup per image, FP64 sums folded in a fixed order.
+__global__ void __launch_bounds__(256) k_ws_moments(const float* __restrict__ x, int res,
+                                                    double* __restrict__ mom) {
+  __shared__ double sh[4][256];
+  const long per = (long)res * res;
+  const float* xd = x + (long)blockIdx.x * per;
+  double s0 = 0, su = 0, sv = 0, s2 = 0;
+  for (long k = threadIdx.x; k < per; k += 256) {
+    const double w = xd[k];
+    const int v = int(k / res), u = int(k - (long)v * res);
+    s0 += w;
+    su += w * u;
+    sv += w * v;
+    s2 += w * (double(u) * u + double(v) * v);
+  }
+  sh[0][threadIdx.x] = s0;
+  sh[1][threadIdx.x] = su;
+  sh[2][threadIdx.x] = sv;
+  sh[3][threadIdx.x] = s2;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int c = 0; c < 4; ++c) sh[c][threadIdx.x] += sh[c][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double w0 = sh[0][0] > 0 ? sh[0][0] : 1e-30;
+    const double cu = sh[1][0] / w0, cv = sh[2][0] / w0;
+    const double r2 = fmax(sh[3][0] / w0 - cu * cu - cv * cv, 0.0);
+    mom[3 * blockIdx.x] = cu;
+    mom[3 * blockIdx.x + 1] = cv;
+    mom[3 * blockIdx.x + 2] = sqrt(r2);
+  }
+}
+
+// Quadratic target-plane phasors exp(i a f^2 |(u, v) - (cu, cv)|^2) per image,
+// f = n / res full-resolution pixels per pooled pixel, a = spread pi / r with
+// r the RMS radius in full-resolution pixels (at least 4): a chirp of that
+// curvature moves a target point at radius r to element radius ~spread n.
+__global__ void k_ws_quadratic(const double* __restrict__ mom, float* __restrict__ z, int res, int n,
+                               double spread, long total) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long per = (long)res * res;
+  const int d = int(i / per);
+  const long r = i - (long)d * per;
+  const int v = int(r / res), u = int(r - (long)v * res);
+  const double f = double(n) / res;
+  const double cu = mom[3 * d], cv = mom[3 * d + 1];
+  const double rr = fmax(mom[3 * d + 2] * f, 4.0);
+  const double a = spread * M_PI / rr;
+  const double ph = a * f * f * ((u - cu) * (u - cu) + (v - cv) * (v - cv));
+  double sn, cs;
+  sincos(ph, &sn, &cs);
+  z[2 * i] = float(cs);
+  z[2 * i + 1] = float(sn);
+}
+
 unsigned blk(long n) { return unsigned((n + 255) / 256); }
 
 int check_sides(const doe_gs_batch* B, int res) {
@@ -307,6 +367,41 @@ int doe_warmstart_apply_phasor(doe_gs_batch* B, int res, const float* z) {
   int rc = check_sides(B, res);
   if (rc) return rc;
   DOE_HIP(hipSetDevice(B->ctx->device));
+  rc = ws_apply_phasor(B, res, z);
+  if (rc) return rc;
+  DOE_HIP(finish(B->ctx));
+  return DOE_OK;
+}
+
+// Initial phases from the quadratic target-plane phase of every design's
+// pooled target (k_ws_quadratic), computed on the device.  `scratch` holds
+// count * res^2 * 3 floats; the moments take 3 doubles per design in the
+// context's reduction buffer.
+int doe_warmstart_apply_quadratic(doe_gs_batch* B, int res, double spread, float* scratch) {
+  DOE_REQUIRE(B && scratch, DOE_ERR_INVALID, "null argument");
+  DOE_REQUIRE(spread > 0 && std::isfinite(spread), DOE_ERR_INVALID, "warm start: spread must be positive");
+  int rc = check_sides(B, res);
+  if (rc) return rc;
+  DOE_HIP(hipSetDevice(B->ctx->device));
+  hipStream_t st = B->ctx->stream;
+  const long per = long(res) * res;
+  float* in = scratch;
+  float* z = scratch + per * B->count;
+  rc = ws_inputs(B, res, in);
+  if (rc) return rc;
+  rc = ensure_reduce(B->ctx, size_t(3) * B->count);
+  if (rc) return rc;
+  double* mom = B->ctx->partials;
+  hipLaunchKernelGGL(k_ws_moments, dim3(unsigned(B->count)), dim3(256), 0, st, in, res, mom);
+  long o = 0;
+  for (const auto& c : B->cls) {
+    const int nd = int(c.users.size());
+    const long total = per * nd;
+    hipLaunchKernelGGL(k_ws_quadratic, dim3(blk(total)), dim3(256), 0, st, mom + 3 * o, z + 2 * per * o, res,
+                       c.n, spread, total);
+    o += nd;
+  }
+  DOE_HIP(hipGetLastError());
   rc = ws_apply_phasor(B, res, z);
   if (rc) return rc;
   DOE_HIP(finish(B->ctx));

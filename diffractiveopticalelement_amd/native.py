@@ -163,6 +163,7 @@ SIGNATURES = [
     ("doe_warmstart_labels", C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
     ("doe_warmstart_apply", C.c_int, [_VP, _VP, _VP]),
     ("doe_warmstart_apply_phasor", C.c_int, [_VP, C.c_int, _VP]),
+    ("doe_warmstart_apply_quadratic", C.c_int, [_VP, C.c_int, C.c_double, _VP]),
     ("doe_gs_batch_restart_best", C.c_int, [_VP]),
     ("doe_screen_metrics", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts), C.POINTER(Metrics)]),
     ("doe_screen_metrics_batch", C.c_int, [_VP, _VP, _VP, _VP, C.POINTER(ScreenOpts),
@@ -610,6 +611,9 @@ class GsBatch:
 
     def warm_apply(self, net: "WsNet", scratch_ptr: int) -> None:
         check(self.ctx.lib.doe_warmstart_apply(self.h, net.h, scratch_ptr))
+
+    def warm_apply_quadratic(self, res: int, spread: float, scratch_ptr: int) -> None:
+        check(self.ctx.lib.doe_warmstart_apply_quadratic(self.h, res, spread, scratch_ptr))
 
     def warm_apply_phasor(self, res: int, phasor_ptr: int) -> None:
         check(self.ctx.lib.doe_warmstart_apply_phasor(self.h, res, phasor_ptr))

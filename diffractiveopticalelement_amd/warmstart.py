@@ -15,6 +15,8 @@ the support I > 0.05 max, dilated by 2 px.
 
 Usage (GPU):
     python -m diffractiveopticalelement_amd.warmstart --train 512 --steps 1500 --eval 32
+    python -m diffractiveopticalelement_amd.warmstart --stream --train 20000 --label-iters 500 \
+        --res 1024 --eval 32 --eval-iters 500 --save wsnet.npy --out result.json
 """
 from __future__ import annotations
 
@@ -153,15 +155,12 @@ def _phasor_field(ctx, z: np.ndarray):
     return zf
 
 
-def start_quadratic(ctx, b, count: int, n: int, res: int):
-    """replace every design's initial phase by the quadratic target-plane start"""
-    inp = ctx.field(res, res, count, "f32")
-    b.warm_inputs(res, inp.ptrs()[0])
-    x, _ = inp.download()
-    zf = _phasor_field(ctx, quadratic_phasors(x, n))
-    b.warm_apply_phasor(res, zf.ptrs()[0])
-    inp.free()
-    zf.free()
+def start_quadratic(ctx, b, count: int, n: int, res: int, spread: float = 0.25):
+    """replace every design's initial phase by the quadratic target-plane
+    start, computed on the device (quadratic_phasors is its host form)"""
+    scratch = ctx.field(res, res, 3 * count, "f32")
+    b.warm_apply_quadratic(res, spread, scratch.ptrs()[0])
+    scratch.free()
 
 
 def make_pairs(ctx, count: int, n: int, res: int, iters: int, seed: int, chunk: int = 64):
@@ -218,6 +217,78 @@ def train(ctx, data, res: int, base: int = 32, batch: int = 8, steps: int = 1000
     for f in (inp, lab, wts):
         f.free()
     return net, hist
+
+
+def _gen_chunk(job):
+    """host side of a training chunk: (target amplitude float32, ROI uint8)
+    of `count` designs of side n from `seed`, in fill_batch's order (runs in
+    a worker process)"""
+    seed, count, n = job
+    rng = np.random.default_rng(seed)
+    ill = illumination(n)
+    out = []
+    for i in range(count):
+        t, roi = design_inputs(make_target(FAMILIES[i % len(FAMILIES)], n, rng), ill)
+        out.append((t.astype(np.float32), roi.astype(np.uint8)))
+    return out
+
+
+def train_stream(ctx, pool, total: int, n: int, res: int, label_iters: int, chunk: int = 64,
+                 batch: int = 8, base: int = 32, lr: float = 1e-3, seed: int = 1000, log=print,
+                 deadline: float | None = None):
+    """Train on `total` (target, label) pairs generated and consumed chunk by
+    chunk, each pair used once: a worker pool builds the next chunks' targets
+    on the host while the GPU runs the current chunk's `label_iters` WGS
+    iterations from the quadratic start and trains on its pairs (res x res
+    inputs, labels and weights stay on the device).  Stops early at
+    `deadline` (time.perf_counter()); returns the network, the loss history
+    and the pairs used."""
+    from collections import deque
+
+    net = nat.WsNet(ctx, res, base, batch, seed)
+    ill = illumination(n)
+    steps_total = max(1, total // batch)
+    jobs = [(seed + c0, min(chunk, total - c0), n) for c0 in range(0, total, chunk)]
+    pending = deque(pool.apply_async(_gen_chunk, (j,)) for j in jobs[:4])
+    nxt = 4
+    hist, used, step = [], 0, 0
+    t0 = time.perf_counter()
+    for q, job in enumerate(jobs):
+        data = pending.popleft().get()
+        if nxt < len(jobs):
+            pending.append(pool.apply_async(_gen_chunk, (jobs[nxt],)))
+            nxt += 1
+        m = job[1]
+        b = nat.GsBatch(ctx, [n] * m, nat.ALG_WGS, 0.9, with_roi=True)
+        for i, (t, roi) in enumerate(data):
+            b.set(i, ill, t, roi)
+        start_quadratic(ctx, b, m, n, res)
+        b.run(label_iters, graph=True)
+        inp = ctx.field(res, res, m, "f32")
+        lab = ctx.field(res, res, m, "f32")
+        wts = ctx.field(res, res, m, "f32")
+        b.warm_inputs(res, inp.ptrs()[0])
+        b.warm_labels(res, lab.ptrs()[0], lab.ptrs()[1], wts.ptrs()[0])
+        b.close()
+        stride = batch * res * res * 4
+        for k in range(m // batch):
+            o = k * stride
+            loss, _ = net.loss_grad(batch, inp.ptrs()[0] + o, lab.ptrs()[0] + o, lab.ptrs()[1] + o,
+                                    wts.ptrs()[0] + o)
+            sl = lr * min(1.0, (step + 1) / 50) * (0.5 * (1 + np.cos(np.pi * min(step, steps_total) / steps_total)))
+            net.adam_step(sl)
+            hist.append(loss)
+            step += 1
+        used += m
+        for f in (inp, lab, wts):
+            f.free()
+        if q % 10 == 0 or q == len(jobs) - 1:
+            log(f"  pairs {used}/{total}, step {step}: loss {np.mean(hist[-40:]):.4f} "
+                f"({time.perf_counter() - t0:.0f} s)")
+        if deadline is not None and time.perf_counter() > deadline:
+            log(f"  stopping at the time limit after {used} pairs")
+            break
+    return net, hist, used
 
 
 def first_below(h, thr):
@@ -362,6 +433,41 @@ def summarize(res: dict, thr: float = 0.01) -> dict:
     return out
 
 
+def main_stream(args) -> int:
+    """The learned warm start at full scale: `--train` pairs streamed through
+    training (see train_stream), then the evaluation; the network's
+    parameters are saved with --save."""
+    import multiprocessing as mp
+
+    t_start = time.perf_counter()
+    # the worker processes start before this process touches the GPU
+    pool = mp.get_context("fork").Pool(args.workers)
+    ctx = nat.Context(0)
+    deadline = t_start + args.max_seconds if args.max_seconds else None
+    print(f"streaming {args.train} pairs at {args.n}^2 ({args.label_iters} WGS iterations each), "
+          f"network output {args.res}^2", flush=True)
+    net, hist, used = train_stream(ctx, pool, args.train, args.n, args.res, args.label_iters,
+                                   log=lambda s: print(s, flush=True), deadline=deadline)
+    pool.close()
+    t_train = time.perf_counter() - t_start
+    if args.save:
+        np.save(args.save, net.params())
+    print("evaluation", flush=True)
+    res = evaluate(ctx, net, args.eval, args.n, args.eval_iters, seed=99999, thr=args.threshold,
+                   log=lambda s: print(s, flush=True), modes=("plain", "quadratic", "warm", "restart@25"),
+                   label_res=[args.n])
+    res["training"] = {"pairs": used, "label_iterations": args.label_iters, "steps": len(hist),
+                       "batch": 8, "network_res": args.res, "params": net.nparams,
+                       "loss_first_last": [float(np.mean(hist[:40])), float(np.mean(hist[-40:]))],
+                       "seconds": round(t_train, 1)}
+    res["summary"] = summarize(res, args.threshold)
+    print(json.dumps({"training": res["training"], "summary": res["summary"]}, indent=1), flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(res, f)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1024)
@@ -375,7 +481,14 @@ def main(argv=None) -> int:
     ap.add_argument("--label-res", type=int, nargs="*", default=None,
                     help="resolutions of the label-bound starts (default: the network's and n)")
     ap.add_argument("--out", default=None, help="write the result JSON here")
+    ap.add_argument("--stream", action="store_true",
+                    help="stream the pairs through training (full-scale runs; see train_stream)")
+    ap.add_argument("--workers", type=int, default=8, help="host processes building targets (--stream)")
+    ap.add_argument("--max-seconds", type=float, default=None, help="stop streaming after this long")
+    ap.add_argument("--save", default=None, help="save the trained parameters (.npy, --stream)")
     args = ap.parse_args(argv)
+    if args.stream:
+        return main_stream(args)
     label_res = args.label_res if args.label_res is not None else [args.res, args.n]
     ctx = nat.Context(0)
     t0 = time.perf_counter()

@@ -287,6 +287,12 @@ int doe_warmstart_apply(doe_gs_batch* batch, doe_wsnet* net, float* scratch);
  * floats interleaved re, im, schedule order) instead of a network, e.g. a
  * quadratic target-plane phase. */
 int doe_warmstart_apply_phasor(doe_gs_batch* batch, int res, const float* phasor);
+/* The same hand-off from a quadratic target-plane phase computed on the
+ * device: per design exp(i a |u - u0|^2) over the pooled target, u0 its
+ * centroid, a = spread * pi / r with r its RMS radius in full-resolution
+ * pixels (at least 4), so a target point at radius r lands near element
+ * radius spread * n (scratch: count*res*res*3 floats, as doe_warmstart_apply). */
+int doe_warmstart_apply_quadratic(doe_gs_batch* batch, int res, double spread, float* scratch);
 /* Restart from the best iterate (no counterpart in the reference): every
  * design's initial phase becomes arg(F^-1(t * phasor(F(illumination *
  * e^{i best phase})))) — one GS projection of its best iterate — so the next
