@@ -110,18 +110,40 @@ struct LinePlan {
   static constexpr int first() { return radix<REV>(0); }
   template <bool REV>
   static constexpr int last() { return radix<REV>(NST - 1); }
-  // Position (along the line) held by register i of thread t at entry/exit.
+  // Position (along the line) held by register i of thread t at entry/exit:
+  // t plus a per-register constant.
+  template <bool REV>
+  static constexpr int in_off(int i) {
+    return (i / first<REV>()) * T + (i % first<REV>()) * (N / first<REV>());
+  }
+  template <bool REV>
+  static constexpr int out_off(int i) {
+    return (i / last<REV>()) * T + (i % last<REV>()) * (N / last<REV>());
+  }
   template <bool REV>
   __device__ __forceinline__ static int in_pos(int t, int i) {
-    constexpr int R = first<REV>();
-    return t + (i / R) * T + (i % R) * (N / R);
+    return t + in_off<REV>(i);
   }
   template <bool REV>
   __device__ __forceinline__ static int out_pos(int t, int i) {
-    constexpr int R = last<REV>();
-    return t + (i / R) * T + (i % R) * (N / R);
+    return t + out_off<REV>(i);
   }
 };
+
+// How a stage obtains the twiddles of its NB butterfly groups.  Group b of
+// thread t multiplies input r by W_N^(r*m_b), m_b = ((t + b*TT) % NS) * (N / (NS*R)).
+//   TW_SAME: TT is a multiple of NS, so m_b = (t % NS) * N/(NS*R) for every b:
+//            one base twiddle and one set of powers per stage.
+//   TW_EACH: each group has its own base twiddle.
+// Every base twiddle is loaded from the table when the line transform starts,
+// so the loads' latency overlaps the first stage.
+enum { TW_NONE = 0, TW_SAME = 1, TW_EACH = 2 };
+template <int TT, int NS>
+constexpr int tw_mode() {
+  if (NS <= 1) return TW_NONE;
+  if (TT % NS == 0) return TW_SAME;
+  return TW_EACH;
+}
 
 // One full line FFT with a per-stage thread geometry.  `geo` says, for each
 // stage s, which line position index t the thread works on (geo.t<s>()),
@@ -140,11 +162,27 @@ __device__ __forceinline__ void fft_line_g(T (&vr)[P], T (&vi)[P], Geo& geo, T* 
                                            const T* __restrict__ twr, const T* __restrict__ twi) {
   using PL = LinePlan<N, P>;
   constexpr int TT = PL::T;
+  // base twiddles of every stage and group, loaded up front (slot s*P + b)
+  T pwr[PL::NST * P], pwi[PL::NST * P];
+  static_for<0, PL::NST>([&](auto SI) {
+    constexpr int s = decltype(SI)::value;
+    constexpr int R = PL::template radix<REV>(s);
+    constexpr int NS = PL::template ns<REV>(s);
+    constexpr int MODE = tw_mode<TT, NS>();
+    constexpr int NL = MODE == TW_SAME ? 1 : MODE == TW_EACH ? P / R : 0;
+    static_for<0, NL>([&](auto BI) {
+      constexpr int b = decltype(BI)::value;
+      const int m = ((geo.template t<s>() + b * TT) % NS) * (N / (NS * R));
+      pwr[s * P + b] = twr[m];
+      pwi[s * P + b] = INV ? -twi[m] : twi[m];
+    });
+  });
   static_for<0, PL::NST>([&](auto SI) {
     constexpr int s = decltype(SI)::value;
     constexpr int R = PL::template radix<REV>(s);
     constexpr int NS = PL::template ns<REV>(s);
     constexpr int NB = P / R;
+    constexpr int MODE = tw_mode<TT, NS>();
     const int t = geo.template t<s>();
     if constexpr (s > 0) {
       constexpr int RP = PL::template radix<REV>(s - 1);
@@ -188,21 +226,37 @@ __device__ __forceinline__ void fft_line_g(T (&vr)[P], T (&vi)[P], Geo& geo, T* 
       }
       geo.template after_read<s>(vr, vi);
     }
+    // powers W^r (r < R) of a base twiddle
+    auto powers = [&](T (&wr)[R], T (&wi)[R]) {
+      static_for<2, R>([&](auto RI) {
+        constexpr int r = decltype(RI)::value;
+        constexpr int a = (r % 2 == 0) ? r / 2 : r - 1;
+        constexpr int c = (r % 2 == 0) ? r / 2 : 1;
+        wr[r] = wr[a] * wr[c] - wi[a] * wi[c];
+        wi[r] = wr[a] * wi[c] + wi[a] * wr[c];
+      });
+    };
+    T swr[R], swi[R];  // TW_SAME: shared by the stage's groups
+    if constexpr (MODE == TW_SAME) {
+      swr[1] = pwr[s * P];
+      swi[1] = pwi[s * P];
+      powers(swr, swi);
+    }
     static_for<0, NB>([&](auto BI) {
       constexpr int b = decltype(BI)::value;
-      if constexpr (NS > 1) {
-        const int j = t + b * TT;
-        const int m = (j % NS) * (N / (NS * R));
+      if constexpr (MODE != TW_NONE) {
         T wr[R], wi[R];
-        wr[1] = twr[m];
-        wi[1] = INV ? -twi[m] : twi[m];
-        static_for<2, R>([&](auto RI) {
-          constexpr int r = decltype(RI)::value;
-          constexpr int a = (r % 2 == 0) ? r / 2 : r - 1;
-          constexpr int c = (r % 2 == 0) ? r / 2 : 1;
-          wr[r] = wr[a] * wr[c] - wi[a] * wi[c];
-          wi[r] = wr[a] * wi[c] + wi[a] * wr[c];
-        });
+        if constexpr (MODE == TW_SAME) {
+          static_for<1, R>([&](auto RI) {
+            constexpr int r = decltype(RI)::value;
+            wr[r] = swr[r];
+            wi[r] = swi[r];
+          });
+        } else {
+          wr[1] = pwr[s * P + b];
+          wi[1] = pwi[s * P + b];
+          powers(wr, wi);
+        }
         static_for<1, R>([&](auto RI) {
           constexpr int r = decltype(RI)::value;
           const T xr = vr[b * R + r], xi = vi[b * R + r];
@@ -260,7 +314,7 @@ struct ColGeom {
   static constexpr int C = cmax(1, cmin(CMAX, cmin(BUDGET / N, 1024 / T_LINE)));
   static constexpr int WG = C * T_LINE;
   static constexpr bool NEED_LDS = LinePlan<N, P>::NST > 1;
-  static constexpr int LDS_ELEMS = NEED_LDS ? (C * N + (C * N) / 32 + 1) : 1;
+  static constexpr int LDS_ELEMS = !NEED_LDS ? 1 : C % 32 == 0 ? C * N : C * N + (C * N) / 32 + 1;
   static constexpr int LDS_IM = PLANES == 2 ? LDS_ELEMS : 1;
 };
 
@@ -285,12 +339,19 @@ struct RowLIdx {
   int base;
   __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
 };
+// Column tiles of 32 or more columns need no padding: a half-wave's 32
+// lanes hold 32 adjacent columns of one line position, one per bank, so
+// every register's LDS address is a per-thread base plus a constant.
+// Narrower tiles pad one element per 32 against bank conflicts.
 template <int C>
 struct ColLIdx {
   int col;
   __device__ __forceinline__ int operator()(int pos) const {
     const int a = pos * C + col;
-    return a + (a >> 5);
+    if constexpr (C % 32 == 0)
+      return a;
+    else
+      return a + (a >> 5);
   }
 };
 

@@ -346,6 +346,7 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     OpAsmH h{};
     std::vector<AsmField> hp(1);
     hp[0].two_pi_z = 2 * kPi * plan->z;
+    hp[0].z = plan->z;
     hp[0].lim_x = hp[0].lim_y = hp[0].inner_x = hp[0].inner_y = 0;
     hp[0].band = 0;  // the design loop never band-limits (doe.js:352, 400, 419)
     hp[0].zero = plan->z == 0 ? 1 : 0;
@@ -484,6 +485,7 @@ int run_gs_multiplane(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum
   std::vector<AsmField> hp(P);
   for (int q = 0; q < P; ++q) {
     hp[q].two_pi_z = 2 * kPi * planes[q].z;
+    hp[q].z = planes[q].z;
     hp[q].lim_x = hp[q].lim_y = hp[q].inner_x = hp[q].inner_y = 0;
     hp[q].band = 0;  // bandLimit: false (doe.js:580, 605)
     hp[q].zero = planes[q].z == 0 ? 1 : 0;

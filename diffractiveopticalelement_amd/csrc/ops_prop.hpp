@@ -63,10 +63,26 @@ __device__ __forceinline__ double soft_window(double v, double lim, double inner
 // Per-field parameters for the angular-spectrum transfer function.
 struct AsmField {
   double two_pi_z;  // (2*pi)*z, formed on the host as the reference does
+  double z;         // z itself: the phase in revolutions is z * sqrt(arg)
   double lim_x, inner_x, lim_y, inner_y;
   int band;  // band limit active (finite local frequency limit)
   int zero;  // z == 0: identity (propagate.js:113)
 };
+
+// sqrt(x) for a positive normal double: the hardware reciprocal square-root
+// estimate refined by Newton-Raphson steps for 1/sqrt and sqrt together
+// (x*y0, y0/2; one coupled correction; one residual correction), good to an
+// ulp or two; the FP32 transfer function only needs ~1e-14 relative.
+__device__ __forceinline__ double sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  return fma(d, h, g);
+}
 
 // Multiply the 2-D spectrum (natural FFT order) by the angular-spectrum
 // transfer function H of propagate.js:57-102.  The reference builds H on the
@@ -74,20 +90,36 @@ struct AsmField {
 // circular shift commutes with the circular convolution, that equals
 // F^-1 diag(ifftshift H) F, i.e. H sampled at the signed natural frequency
 // index, which is what is evaluated here.  `conj` gives H(-z) = conj(H(z)).
+//
+// FP64 fields evaluate H per point exactly in the reference's operation
+// order (h_at).  FP32 fields split the work by axis: the fused column pass
+// (one workgroup = 32 columns x one column block of the grid) first fills an
+// LDS table with fy^2 and the y window for each of its rows, each thread
+// takes its column's fx terms (inv_l2 - fx^2, the x window, 1/N of the
+// inverse transform) once, and a point costs a table read, a subtraction,
+// the square root and the phase reduction in FP64, and the hardware sine and
+// cosine.  The argument keeps the reference's order, (inv_l2 - fx*fx) -
+// fy*fy, so it equals the FP64 path's bit for bit; the windows are rounded
+// to FP32 after the reference's FP64 formula.
 struct OpAsmH {
   static constexpr int NACC = 0;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
-  // FP32 column tiles.  Lines of 256 and 512 points: 16 points per thread x 16 columns,
-  // both LDS planes (512 threads at 128 registers, two workgroups per CU);
-  // measured on 16 x 2048^2: 367-371 us per pass against 478 us for 32
-  // points x 32 columns at one workgroup per CU (472 us with one plane), 410
-  // us for 8-column tiles and 375 us for 16 x 16 with one plane.  Longer
-  // lines would spill at 16 points per thread (1024-thread workgroups) and
-  // keep 32 points per thread with both planes (the FP64 transfer-function
-  // argument fits the 256-register budget unspilled).
+  // FP32 column tiles: 32 points per thread x 32 columns, one LDS plane (two
+  // 512-thread workgroups per CU for 512-point lines); 32-column tiles
+  // address LDS without padding.  Longer lines keep 32 points per thread
+  // with both planes.
   template <typename TT, int N>
-  using ColGeomT = typename std::conditional<(N <= 512), ColGeom<TT, N, 16, 16, 2>,
+  using ColGeomT = typename std::conditional<(N <= 512), ColGeom<TT, N, 32, 32, 1>,
                                              ColGeom<TT, N, 32, 64, 2>>::type;
+  // FP32 lines up to 2048 points (the table has to fit the LDS beside the
+  // line planes): per-line table of fy^2 (N doubles) and the y window (N
+  // floats); longer lines evaluate h_at per point
+  template <typename TT, int N>
+  static constexpr bool TABLED = sizeof(TT) == 4 && N <= 2048;
+  template <typename TT, int N>
+  static constexpr int LINE_TAB = TABLED<TT, N> ? N + (N + 1) / 2 : 0;
+  template <typename TT, int N>
+  static constexpr bool SCALES_INV = TABLED<TT, N>;
   const AsmField* fp;
   double dfx, dfy, inv_l2;
   int nx, ny, evanescent;
@@ -108,6 +140,64 @@ struct OpAsmH {
     if (conj) hi = -hi;
     cmul(re, im, hr, hi);
   }
+
+  // ---- FP32 column pass: per-line table, per-column constants ----------
+  template <typename T>
+  struct ColCtx {
+    double ax;   // inv_l2 - fx*fx
+    double zr;   // +-z: revolutions per unit of sqrt(arg); the sign conjugates H
+    float wxs;   // x window / N (the inverse column transform's scale)
+    float invn;  // 1 / N
+    int zero;    // z == 0: H = 1
+  };
+  template <typename T, int N>
+  __device__ __forceinline__ void line_table(double* tab, int f, int k1, int n1cnt, int tid,
+                                             int nthr) const {
+#pragma clang fp contract(off)
+    const AsmField p = fp[f];
+    float* wtab = reinterpret_cast<float*>(tab + N);
+    for (int q = tid; q < N; q += nthr) {
+      const int ky = k1 + n1cnt * q;
+      const int sy = ky < (ny >> 1) ? ky : ky - ny;
+      const double fy = double(sy) * dfy;
+      tab[q] = fy * fy;
+      wtab[q] = p.band ? float(soft_window(fabs(fy), p.lim_y, p.inner_y)) : 1.0f;
+    }
+  }
+  template <typename T, int N>
+  __device__ __forceinline__ ColCtx<T> col_ctx(int f, int kx) const {
+#pragma clang fp contract(off)
+    const AsmField p = fp[f];
+    const int sx = kx < (nx >> 1) ? kx : kx - nx;
+    const double fx = double(sx) * dfx;
+    ColCtx<T> c;
+    c.ax = inv_l2 - fx * fx;
+    c.zr = conj ? -p.z : p.z;
+    const double wx = p.band ? soft_window(fabs(fx), p.lim_x, p.inner_x) : 1.0;
+    c.invn = 1.0f / float(N);
+    c.wxs = float(wx) * c.invn;
+    c.zero = p.zero;
+    return c;
+  }
+  // H at line position q of the column: branch-free (selects only), so the
+  // caller's registers are never updated conditionally
+  template <typename T, int N>
+  __device__ __forceinline__ void mid_line(const ColCtx<T>& c, const double* tab, int q, int, int,
+                                           int, T& re, T& im, double*) const {
+    static_assert(sizeof(T) == 4, "the table-driven transfer function is the FP32 path");
+    const double arg = c.ax - tab[q];
+    const float wv = c.wxs * reinterpret_cast<const float*>(tab + N)[q];
+    // outside the band limit (w = 0) and for evanescent waves H is 0; the
+    // square root's argument is kept normal (1e-300: a phase below 1e-140)
+    const float w = (arg >= 0.0 || !evanescent) ? wv : 0.0f;
+    const double sq = sqrt_pos(fmax(fabs(arg), 1e-300));
+    const double rev = c.zr * sq;
+    const float fr = float(rev - rint(rev));
+    const float hr = c.zero ? c.invn : w * __builtin_amdgcn_cosf(fr);
+    const float hi = c.zero ? 0.0f : w * __builtin_amdgcn_sinf(fr);
+    cmul(re, im, hr, hi);
+  }
+
   template <typename T>
   __device__ __forceinline__ void h_at(int f, int kx, int ky, T& hr, T& hi) const {
 #pragma clang fp contract(off)

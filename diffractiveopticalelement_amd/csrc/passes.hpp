@@ -220,6 +220,35 @@ struct OpHooks<Op, std::void_t<decltype(Op::HAS_PRE), decltype(Op::HAS_POST)>> {
   static constexpr bool pre = Op::HAS_PRE, post = Op::HAS_POST;
 };
 
+// Ops may prepare, for a fused column pass (MODE 1), a per-line table in LDS
+// (Op::template LINE_TAB<T, N>: doubles per workgroup, filled by
+// line_table before the first transform) and per-column constants
+// (col_ctx, once per thread); their mid hook is then
+// mid_line(ctx, table, line position, field, x, y, re, im, acc).  Such ops
+// also fold the inverse transform's 1/N into the mid hook when they say so
+// (Op::template SCALES_INV<T, N>).
+template <class Op, typename T, int N, class = void>
+struct OpLineTab {
+  static constexpr int value = 0;
+};
+template <class Op, typename T, int N>
+struct OpLineTab<Op, T, N, std::void_t<decltype(Op::template LINE_TAB<T, N>)>> {
+  static constexpr int value = Op::template LINE_TAB<T, N>;
+};
+template <class Op, typename T, int N, class = void>
+struct OpScalesInv : std::false_type {};
+template <class Op, typename T, int N>
+struct OpScalesInv<Op, T, N, std::void_t<decltype(Op::template SCALES_INV<T, N>)>>
+    : std::integral_constant<bool, Op::template SCALES_INV<T, N>> {};
+template <class Op, typename T, bool CC>
+struct OpCtxOf {
+  using type = int;
+};
+template <class Op, typename T>
+struct OpCtxOf<Op, T, true> {
+  using type = typename Op::template ColCtx<T>;
+};
+
 template <int N1>
 __device__ __forceinline__ constexpr int brev_lane(int l) {
   return N1 == 4 ? ((l & 1) << 1) | ((l >> 1) & 1) : l;
@@ -598,7 +627,7 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
                                          const S* __restrict__ iim, S* __restrict__ ore,
                                          S* __restrict__ oim, int nx, int tiles, int n1cnt,
                                          const T* __restrict__ twr, const T* __restrict__ twi,
-                                         int lb, T* lre, T* lim) {
+                                         int lb, T* lre, T* lim, double* ltab) {
   constexpr int P = CF::P;
   constexpr int C = CF::C;
   using PL = LinePlan<N2, P>;
@@ -621,20 +650,27 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
   double acc[NA];
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+  // Row of line position p = t + c (c a per-register constant): a workgroup-
+  // uniform part (k1 and c) and a per-thread part (t), so each access is a
+  // uniform base (scalar registers) plus one 32-bit per-thread offset.
+  const int toff_in = (NAT_IN ? n1cnt * t : t) * nx + x;
+  const int toff_out = (NAT_OUT ? n1cnt * t : t) * nx + x;
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-      const int p = PL::template in_pos<false>(t, i);
-      const int row = NAT_IN ? k1 + n1cnt * p : N2 * k1 + p;
-      const long a = fbase + (long)row * nx + x;
+      const int c = PL::template in_off<false>(i);
+      const int p = t + c;
+      const long urow = NAT_IN ? long(k1) + long(n1cnt) * c : long(N2) * k1 + c;
+      const S* ur = ire + (fbase + urow * nx);
+      const S* ui = iim + (fbase + urow * nx);
       // a row segment narrower than a 128-byte line shares its line with the
       // neighbouring tile, which finds it in L2 only after an ordinary load
       if constexpr (C * sizeof(S) < 128) {
-        vr[i] = ld<T>(ire + a);
-        vi[i] = ld<T>(iim + a);
+        vr[i] = ld<T>(ur + toff_in);
+        vi[i] = ld<T>(ui + toff_in);
       } else {
-        vr[i] = ld_field<T>(ire + a);
-        vi[i] = ld_field<T>(iim + a);
+        vr[i] = ld_field<T>(ur + toff_in);
+        vi[i] = ld_field<T>(ui + toff_in);
       }
       op.pre(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
     }
@@ -654,11 +690,32 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
         aux[i] = op.load_aux(f, x, k1 + n1cnt * PL::template out_pos<false>(t, i));
     }
   }
+  // per-line table and per-column constants of the mid hook
+  constexpr bool CC = MODE == 1 && OpLineTab<Op, T, N2>::value > 0;
+  typename OpCtxOf<Op, T, CC>::type cc{};
+  if constexpr (CC) {
+    op.template line_table<T, N2>(ltab, f, k1, n1cnt, tid, CF::WG);
+    cc = op.template col_ctx<T, N2>(f, x);
+    if constexpr (PL::NST == 1) __syncthreads();  // no exchange barrier before the mid hook
+  }
   const ColLIdx<C> li{col};
   fft_line<T, N2, P, INV1, false, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   constexpr T invN = T(1) / T(N2);
   constexpr bool REV_OUT = MODE == 1;
-  if constexpr (MODE == 1) {
+  if constexpr (MODE == 1 && CC) {
+    // table-driven hooks read only LDS and registers: no guard for columns
+    // past the grid, whose lines are zero
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      const int p = PL::template out_pos<false>(t, i);
+      if constexpr (INV1) {
+        vr[i] *= invN;
+        vi[i] *= invN;
+      }
+      op.template mid_line<T, N2>(cc, ltab, p, f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
+    }
+    fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
+  } else if constexpr (MODE == 1) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < P; ++i) {
@@ -676,19 +733,21 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
     fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   }
   constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
+  // the mid hook may already carry the inverse transform's 1/N
+  constexpr bool SCALE_LAST = INV_LAST && !(MODE == 1 && INV2 && OpScalesInv<Op, T, N2>::value);
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const int p = PL::template out_pos<REV_OUT>(t, i);
-      if constexpr (INV_LAST) {
+      if constexpr (SCALE_LAST) {
         vr[i] *= invN;
         vi[i] *= invN;
       }
       op.post(f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
-      const int row = NAT_OUT ? k1 + n1cnt * p : N2 * k1 + p;
-      const long a = fbase + (long)row * nx + x;
-      st(ore + a, vr[i]);
-      st(oim + a, vi[i]);
+      const int c = PL::template out_off<REV_OUT>(i);
+      const long urow = NAT_OUT ? long(k1) + long(n1cnt) * c : long(N2) * k1 + c;
+      st(ore + (fbase + urow * nx) + toff_out, vr[i]);
+      st(oim + (fbase + urow * nx) + toff_out, vi[i]);
     }
   }
   // slot lb = f * per_field + (k1, tile): field f's slots are contiguous
@@ -703,9 +762,11 @@ __global__ void __launch_bounds__((CF::WG))
                const T* __restrict__ twi, int rev) {
   __shared__ T lre[CF::LDS_ELEMS];
   __shared__ T lim[CF::LDS_IM];
+  constexpr int NT = MODE == 1 ? OpLineTab<Op, T, N2>::value : 0;
+  __shared__ double ltab[NT > 0 ? NT : 1];
   const int lb = xcd_remap(rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x), gridDim.x);
   col_body<T, S, N2, MODE, INV1, INV2, Op, CF>(op, ire, iim, ore, oim, nx, tiles, n1cnt, twr, twi,
-                                               lb, lre, lim);
+                                               lb, lre, lim, ltab);
 }
 
 // Elementwise pass over a batch (no FFT): op.apply(field, x, y, k).

@@ -33,6 +33,7 @@ static AsmField asm_field_params(int nx, int ny, double dx, double dy, double la
   p.inner_y = p.lim_y * (1 - soft_edge);
   p.band = band_limit ? 1 : 0;
   p.two_pi_z = 2 * kPi * z;
+  p.z = z;
   p.zero = (z == 0) ? 1 : 0;
   return p;
 }
