@@ -254,56 +254,62 @@ __device__ __forceinline__ constexpr int brev_lane(int l) {
   return N1 == 4 ? ((l & 1) << 1) | ((l >> 1) & 1) : l;
 }
 
-template <typename T>
-__device__ __forceinline__ T shx(T v, int mask) {
+// Lane exchange within a quad of lanes by DPP (no LDS traffic): MASK 1
+// swaps lanes 0<->1 and 2<->3 (quad_perm [1,0,3,2]), MASK 2 swaps the pairs
+// (quad_perm [2,3,0,1]).
+template <int MASK, typename T>
+__device__ __forceinline__ T shx(T v) {
+  constexpr int CTRL = MASK == 1 ? 0xB1 : 0x4E;
   if constexpr (sizeof(T) == 4) {
-    return __shfl_xor(v, mask, 64);
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
   } else {
-    return __shfl_xor(v, mask, 64);
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
   }
 }
 
-// One radix-2 lane butterfly: lower lane (bit clear) u+v, upper lane v-u.
+// One radix-2 lane butterfly: lower lane (bit clear) u+v, upper lane v-u,
+// i.e. sgn*u + v with sgn = +1 / -1 (exact: one rounding, as the sum).
+template <int MASK, typename T>
+__device__ __forceinline__ void lane_bfly(T& ur, T& ui, T sgn) {
+  const T vr = shx<MASK>(ur), vi = shx<MASK>(ui);
+  ur = fma(sgn, ur, vr);
+  ui = fma(sgn, ui, vi);
+}
+
+// Per-lane constants of the N1-point lane DFT (lane slot n1).
 template <typename T>
-__device__ __forceinline__ void lane_bfly(T& ur, T& ui, int mask, bool upper) {
-  const T vr = shx(ur, mask), vi = shx(ui, mask);
-  if (!upper) {
-    ur = ur + vr;
-    ui = ui + vi;
-  } else {
-    ur = vr - ur;
-    ui = vi - ui;
-  }
-}
+struct LaneDft {
+  T s1, s2;  // butterfly signs for the lane bits 1 and 2
+  bool rot;  // n1 == 3: the W4 twiddle between the two butterflies
+  __device__ __forceinline__ explicit LaneDft(int n1)
+      : s1((n1 & 1) ? T(-1) : T(1)), s2((n1 & 2) ? T(-1) : T(1)), rot(n1 == 3) {}
+};
 
-// multiply by W4^e (e in {0,1}) forward, or its conjugate for INV
+// multiply by W4 = -i (forward) or +i (inverse) where `rot`, by selects
 template <bool INV, typename T>
-__device__ __forceinline__ void mul_w4(T& ur, T& ui, int e) {
-  if (e) {
-    const T r = ur;
-    if (!INV) {  // * (-i)
-      ur = ui;
-      ui = -r;
-    } else {  // * (+i)
-      ur = -ui;
-      ui = r;
-    }
-  }
+__device__ __forceinline__ void mul_w4(T& ur, T& ui, bool rot) {
+  const T r = ur, i = ui;
+  ur = rot ? (INV ? -i : i) : r;
+  ui = rot ? (INV ? r : -r) : i;
 }
 
 template <typename T, int N1, bool INV, bool DIT>
-__device__ __forceinline__ void lane_dft(T& ur, T& ui, int n1) {
+__device__ __forceinline__ void lane_dft(T& ur, T& ui, const LaneDft<T>& ld) {
   if constexpr (N1 == 2) {
-    lane_bfly(ur, ui, 1, (n1 & 1) != 0);
+    lane_bfly<1>(ur, ui, ld.s1);
   } else if constexpr (N1 == 4) {
     if constexpr (!DIT) {
-      lane_bfly(ur, ui, 2, (n1 & 2) != 0);
-      if (n1 & 2) mul_w4<INV>(ur, ui, n1 & 1);
-      lane_bfly(ur, ui, 1, (n1 & 1) != 0);
+      lane_bfly<2>(ur, ui, ld.s2);
+      mul_w4<INV>(ur, ui, ld.rot);
+      lane_bfly<1>(ur, ui, ld.s1);
     } else {
-      lane_bfly(ur, ui, 1, (n1 & 1) != 0);
-      if (n1 & 2) mul_w4<INV>(ur, ui, n1 & 1);
-      lane_bfly(ur, ui, 2, (n1 & 2) != 0);
+      lane_bfly<1>(ur, ui, ld.s1);
+      mul_w4<INV>(ur, ui, ld.rot);
+      lane_bfly<2>(ur, ui, ld.s2);
     }
   }
 }
@@ -315,12 +321,8 @@ __device__ __forceinline__ void lane_dft(T& ur, T& ui, int n1) {
 // twiddle, inverse DFT across the lanes scaled by 1/N1 (lane n1 then holds
 // the natural row slot bitrev(n1)).
 template <typename T, int P, int N1, bool INV>
-__device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, int n2,
-                                            const T* __restrict__ twyr,
-                                            const T* __restrict__ twyi) {
-  const int k1 = INV ? n1 : brev_lane<N1>(n1);
-  const T wr = twyr[n2 * k1];
-  const T wi = INV ? -twyi[n2 * k1] : twyi[n2 * k1];
+__device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, T wr, T wi) {
+  const LaneDft<T> ld(n1);
   constexpr T sc = T(1) / T(N1);
 #pragma unroll
   for (int i = 0; i < P; ++i) {
@@ -328,11 +330,11 @@ __device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, int 
       const T a = vr[i] * wr - vi[i] * wi;
       vi[i] = vr[i] * wi + vi[i] * wr;
       vr[i] = a;
-      lane_dft<T, N1, true, false>(vr[i], vi[i], n1);
+      lane_dft<T, N1, true, false>(vr[i], vi[i], ld);
       vr[i] *= sc;
       vi[i] *= sc;
     } else {
-      lane_dft<T, N1, false, false>(vr[i], vi[i], n1);
+      lane_dft<T, N1, false, false>(vr[i], vi[i], ld);
       const T a = vr[i] * wr - vi[i] * wi;
       vi[i] = vr[i] * wi + vi[i] * wr;
       vr[i] = a;
@@ -352,9 +354,8 @@ __device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, int 
 // 2 inverse.
 template <int NX, int NST, int LSTRIDE, int CROSS, typename T, int N1>
 struct RowLaneGeo {
-  int tc, lc, ti, li, lb, n1, n2;
-  const T* twyr;
-  const T* twyi;
+  int tc, lc, ti, li, lb, n1;
+  T cwr, cwi;  // the cross-row twiddle of this lane (loaded before the transform)
   static constexpr int S_IN = 1, S_OUT = NST - 1;
   static_assert(NST >= 3, "needs two exchanges");
   static constexpr bool inter(int s) { return s >= S_IN && s < S_OUT; }
@@ -372,8 +373,8 @@ struct RowLaneGeo {
   }
   template <int S, typename U, int PP>
   __device__ __forceinline__ void after_read(U (&vr)[PP], U (&vi)[PP]) const {
-    if constexpr (S == S_IN && CROSS == 1) cross_lanes<U, PP, N1, false>(vr, vi, n1, n2, twyr, twyi);
-    if constexpr (S == S_IN && CROSS == 2) cross_lanes<U, PP, N1, true>(vr, vi, n1, n2, twyr, twyi);
+    if constexpr (S == S_IN && CROSS == 1) cross_lanes<U, PP, N1, false>(vr, vi, n1, cwr, cwi);
+    if constexpr (S == S_IN && CROSS == 2) cross_lanes<U, PP, N1, true>(vr, vi, n1, cwr, cwi);
   }
 };
 
@@ -539,8 +540,16 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
       constexpr int GT = N1 * CF::T_LINE;  // threads of one group
       const int gt = tid - g * GT;
       const int ni = gt % N1;
+      // the lane's cross-row twiddle W_ny^(n2*k1): forward k1 = bitrev(slot),
+      // inverse k1 = slot, conjugated
+      T cwr = T(1), cwi = T(0);
+      if constexpr (CROSS != 0) {
+        const int k1 = CROSS == 2 ? ni : brev_lane<N1>(ni);
+        cwr = twyr[n2 * k1];
+        cwi = CROSS == 2 ? -twyi[n2 * k1] : twyi[n2 * k1];
+      }
       RowLaneGeo<NX, PL::NST, CF::LSTRIDE, CROSS, T, N1> geo{
-          t, l, gt / N1, g * N1 + ni, g * N1 + brev_lane<N1>(ni), ni, n2, twyr, twyi};
+          t, l, gt / N1, g * N1 + ni, g * N1 + brev_lane<N1>(ni), ni, cwr, cwi};
       fft_line_g<T, NX, P, INV, REV, CF::PLANES>(vr, vi, geo, lre, lim, twr, twi);
     } else {
       const RowLIdx li{l * CF::LSTRIDE};
