@@ -237,6 +237,34 @@ struct GsWork {
   }
 };
 
+// Run a design loop's launches (`f` enqueues them on the context stream) as
+// one captured graph, so a long loop of small passes is not bound by the
+// host's launch rate; eagerly when per-launch profiling is on or the loop is
+// short.  The loops capture kernel launches only (no memsets or copies).
+template <class F>
+static int run_captured(doe_ctx* ctx, long launches, F&& f) {
+  if ((ctx->flags & DOE_CTX_PROFILE) || launches < 16) return f();
+  hipGraph_t g = nullptr;
+  DOE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = f();
+  const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  DOE_HIP(ce);
+  hipGraphExec_t ge = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  DOE_HIP(ie);
+  const hipError_t le = hipGraphLaunch(ge, ctx->stream);
+  const hipError_t se = hipStreamSynchronize(ctx->stream);
+  (void)hipGraphExecDestroy(ge);
+  DOE_HIP(le);
+  DOE_HIP(se);
+  return DOE_OK;
+}
+
 template <typename T>
 int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const double* tgt,
                   const double* roi, const double* init_phase, unsigned seed, double* phase_out,
@@ -335,13 +363,18 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     OpFFInit<T> op0{b, W.phi0};
     DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ff_init_row", op0, re, im, re, im, 1,
                                                             pl, sz)));
-    for (int it = 0; it < K; ++it) {
-      OpFFB<T> opb{b, ctx->partials, it, it == K - 1 ? 1 : 0};
-      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, 1, pl, sz)));
-      OpFFA<T> opa{b, it == K - 1 ? 1 : 0};
-      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ff_row_phase", opa, re, im, re, im, 1,
-                                                           pl, sz)));
-    }
+    rc = run_captured(ctx, 3L * K, [&]() -> int {
+      for (int it = 0; it < K; ++it) {
+        OpFFB<T> opb{b, ctx->partials, it, it == K - 1 ? 1 : 0};
+        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ff_col_project", opb, re, im, re, im, 1, pl,
+                                                sz)));
+        OpFFA<T> opa{b, it == K - 1 ? 1 : 0};
+        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ff_row_phase", opa, re, im, re, im,
+                                                             1, pl, sz)));
+      }
+      return DOE_OK;
+    });
+    if (rc) return rc;
   } else {
     OpAsmH h{};
     std::vector<AsmField> hp(1);
@@ -368,30 +401,35 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
     DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_init_row", op0, re, im, re, im, 1,
                                                            pl, sz)));
     DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
-    for (int it = 0; it < K; ++it) {
-      if (!weighted || it == 0) {
-        OpIPMP<T> op{b, ctx->partials, it};
-        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_project", op, re, im, re, im,
-                                                             1, pl, sz)));
-      } else {
-        OpIPMeas<T> om{b, ctx->partials, it - 1};
-        DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re,
-                                                              im, 1, pl, sz)));
-        OpIPProj<T> op{b, ctx->partials, it};
-        DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_row_reproject", op, re, im, re,
+    rc = run_captured(ctx, 7L * K, [&]() -> int {
+      for (int it = 0; it < K; ++it) {
+        if (!weighted || it == 0) {
+          OpIPMP<T> op{b, ctx->partials, it};
+          DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_project", op, re, im, re,
                                                                im, 1, pl, sz)));
+        } else {
+          OpIPMeas<T> om{b, ctx->partials, it - 1};
+          DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re,
+                                                                im, 1, pl, sz)));
+          OpIPProj<T> op{b, ctx->partials, it};
+          DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "ip_row_reproject", op, re, im,
+                                                                 re, im, 1, pl, sz)));
+        }
+        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_hconj", hc, re, im, re, im, 1, pl,
+                                                sz)));
+        OpIPPhase<T> opp{b};
+        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_phase", opp, re, im, re, im,
+                                                             1, pl, sz)));
+        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
       }
-      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_hconj", hc, re, im, re, im, 1, pl, sz)));
-      OpIPPhase<T> opp{b};
-      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "ip_row_phase", opp, re, im, re, im, 1,
-                                                           pl, sz)));
-      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "ip_col_h", h, re, im, re, im, 1, pl, sz)));
-    }
-    // closing measurement: GS final forward amplitude (doe.js:418-426); WGS
-    // best check + weight update of the last iteration.
-    OpIPMeas<T> om{b, ctx->partials, K - 1};
-    DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re, im, 1,
-                                                          pl, sz)));
+      // closing measurement: GS final forward amplitude (doe.js:418-426); WGS
+      // best check + weight update of the last iteration.
+      OpIPMeas<T> om{b, ctx->partials, K - 1};
+      DOE_HIP((row_pass<T, T, 0, true, false, true, false>(ctx, "ip_row_measure", om, re, im, re, im,
+                                                            1, pl, sz)));
+      return DOE_OK;
+    });
+    if (rc) return rc;
   }
 
   // ---- outputs ----------------------------------------------------------
@@ -542,28 +580,32 @@ int run_gs_multiplane(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum
   DOE_HIP((row_pass<T, T, 0, false, false, false, true>(ctx, "mp_init_row", op0, re, im, re, im, 1, pl,
                                                          sz)));
   if (K > 0) DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_h", hfor(0, 0), re, im, re, im, 1, pl, sz)));
-  for (int it = 0; it < K; ++it) {
-    for (int q = 0; q < P; ++q) {
-      GsBuf<T> bq = b;
-      bq.tgt = tgt_d + size_t(q) * n;
-      bq.roi = roi_d + size_t(q) * n;
-      bq.err_hist = W.hist + hk * size_t(q);
-      OpIPMP<T> op{bq, ctx->partials, it};
-      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_project", op, re, im, re, im, 1,
-                                                           pl, sz)));
-      DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_hconj", hfor(q, 1), re, im, re, im, 1, pl,
-                                              sz)));
-      OpIPPhase<T> opp{bq};
-      DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_phase", opp, re, im, re, im, 1,
-                                                           pl, sz)));
-      const bool last = it == K - 1 && q == P - 1;
-      if (!last) {
-        const int nq = q + 1 < P ? q + 1 : 0;
-        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_h", hfor(nq, 0), re, im, re, im, 1, pl,
+  rc = run_captured(ctx, 4L * K * P, [&]() -> int {
+    for (int it = 0; it < K; ++it) {
+      for (int q = 0; q < P; ++q) {
+        GsBuf<T> bq = b;
+        bq.tgt = tgt_d + size_t(q) * n;
+        bq.roi = roi_d + size_t(q) * n;
+        bq.err_hist = W.hist + hk * size_t(q);
+        OpIPMP<T> op{bq, ctx->partials, it};
+        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_project", op, re, im, re, im, 1,
+                                                             pl, sz)));
+        DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_hconj", hfor(q, 1), re, im, re, im, 1, pl,
                                                 sz)));
+        OpIPPhase<T> opp{bq};
+        DOE_HIP((row_pass<T, T, 1, true, false, true, true>(ctx, "mp_row_phase", opp, re, im, re, im, 1,
+                                                             pl, sz)));
+        const bool last = it == K - 1 && q == P - 1;
+        if (!last) {
+          const int nq = q + 1 < P ? q + 1 : 0;
+          DOE_HIP((col_pass<T, T, 1, false, true>(ctx, "mp_col_h", hfor(nq, 0), re, im, re, im, 1, pl,
+                                                  sz)));
+        }
       }
     }
-  }
+    return DOE_OK;
+  });
+  if (rc) return rc;
   DOE_HIP(hipStreamSynchronize(ctx->stream));
   rc = download_t<T>(ctx, K > 0 ? ph : W.phi0, phase_out, n);
   if (rc) return rc;

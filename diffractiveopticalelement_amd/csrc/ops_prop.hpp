@@ -120,6 +120,8 @@ struct OpAsmH {
   static constexpr int LINE_TAB = TABLED<TT, N> ? N + (N + 1) / 2 : 0;
   template <typename TT, int N>
   static constexpr bool SCALES_INV = TABLED<TT, N>;
+  template <typename TT, int N>
+  static constexpr bool COL_CTX = TABLED<TT, N>;
   const AsmField* fp;
   double dfx, dfy, inv_l2;
   int nx, ny, evanescent;
@@ -181,9 +183,9 @@ struct OpAsmH {
   }
   // H at line position q of the column: branch-free (selects only), so the
   // caller's registers are never updated conditionally
-  template <typename T, int N>
+  template <typename T, int N, class A, class X>
   __device__ __forceinline__ void mid_line(const ColCtx<T>& c, const double* tab, int q, int, int,
-                                           int, T& re, T& im, double*) const {
+                                           int, T& re, T& im, A*, bool, const X&) const {
     static_assert(sizeof(T) == 4, "the table-driven transfer function is the FP32 path");
     const double arg = c.ax - tab[q];
     const float wv = c.wxs * reinterpret_cast<const float*>(tab + N)[q];

@@ -220,13 +220,20 @@ struct OpHooks<Op, std::void_t<decltype(Op::HAS_PRE), decltype(Op::HAS_POST)>> {
   static constexpr bool pre = Op::HAS_PRE, post = Op::HAS_POST;
 };
 
-// Ops may prepare, for a fused column pass (MODE 1), a per-line table in LDS
+// Ops may prepare, for a fused column pass (MODE 1, Op::template COL_CTX<T,
+// N>), per-column constants (col_ctx, once per thread: uniform values the
+// hook would otherwise reload per point) and a per-line table in LDS
 // (Op::template LINE_TAB<T, N>: doubles per workgroup, filled by
-// line_table before the first transform) and per-column constants
-// (col_ctx, once per thread); their mid hook is then
-// mid_line(ctx, table, line position, field, x, y, re, im, acc).  Such ops
-// also fold the inverse transform's 1/N into the mid hook when they say so
+// line_table before the first transform); their mid hook is then
+// mid_line(ctx, table, line position, field, x, y, re, im, acc, active,
+// aux) and must leave memory alone where `active` is false.  Such ops also
+// fold the inverse transform's 1/N into the mid hook when they say so
 // (Op::template SCALES_INV<T, N>).
+template <class Op, typename T, int N, class = void>
+struct OpColCtx : std::false_type {};
+template <class Op, typename T, int N>
+struct OpColCtx<Op, T, N, std::void_t<decltype(Op::template COL_CTX<T, N>)>>
+    : std::integral_constant<bool, Op::template COL_CTX<T, N>> {};
 template <class Op, typename T, int N, class = void>
 struct OpLineTab {
   static constexpr int value = 0;
@@ -700,20 +707,23 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
     }
   }
   // per-line table and per-column constants of the mid hook
-  constexpr bool CC = MODE == 1 && OpLineTab<Op, T, N2>::value > 0;
+  constexpr bool CC = MODE == 1 && OpColCtx<Op, T, N2>::value;
   typename OpCtxOf<Op, T, CC>::type cc{};
   if constexpr (CC) {
-    op.template line_table<T, N2>(ltab, f, k1, n1cnt, tid, CF::WG);
+    if constexpr (OpLineTab<Op, T, N2>::value > 0) {
+      op.template line_table<T, N2>(ltab, f, k1, n1cnt, tid, CF::WG);
+      if constexpr (PL::NST == 1) __syncthreads();  // no exchange barrier before the mid hook
+    }
     cc = op.template col_ctx<T, N2>(f, x);
-    if constexpr (PL::NST == 1) __syncthreads();  // no exchange barrier before the mid hook
   }
   const ColLIdx<C> li{col};
   fft_line<T, N2, P, INV1, false, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   constexpr T invN = T(1) / T(N2);
   constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1 && CC) {
-    // table-driven hooks read only LDS and registers: no guard for columns
-    // past the grid, whose lines are zero
+    // the hook runs for every column (columns past the grid hold zero lines)
+    // and keeps its memory accesses to active ones: no branch around the
+    // registers it updates
 #pragma unroll
     for (int i = 0; i < P; ++i) {
       const int p = PL::template out_pos<false>(t, i);
@@ -721,7 +731,8 @@ __device__ __forceinline__ void col_body(const Op& op, const S* __restrict__ ire
         vr[i] *= invN;
         vi[i] *= invN;
       }
-      op.template mid_line<T, N2>(cc, ltab, p, f, x, k1 + n1cnt * p, vr[i], vi[i], acc);
+      op.template mid_line<T, N2>(cc, ltab, p, f, x, k1 + n1cnt * p, vr[i], vi[i], acc, active,
+                                  aux[PF ? i : 0]);
     }
     fft_line<T, N2, P, INV2, true, CF::PLANES>(vr, vi, t, lre, lim, li, twr, twi);
   } else if constexpr (MODE == 1) {
