@@ -195,24 +195,35 @@ __device__ __forceinline__ void fft_line_g(T (&vr)[P], T (&vi)[P], Geo& geo, T* 
         constexpr bool WRE = decltype(wr_c)::value;
         constexpr bool WIM = decltype(wi_c)::value;
         __syncthreads();
+        // Positions are a per-thread part plus a compile-time part; the
+        // index maps add the latter as a constant offset where their padding
+        // allows (widx_c / ridx_c), so most accesses need no address math.
+        // Writes: with TT a multiple of NSP, position (j/NSP)*NSP*RP + j%NSP
+        // + q*NSP of j = tp + b*TT is W(tp) + b*TT*RP + q*NSP, W(tp) a
+        // multiple of RP when NSP = 1.
         static_for<0, NBP>([&](auto BI) {
           constexpr int b = decltype(BI)::value;
-          const int j = tp + b * TT;
-          const int base = (j / NSP) * NSP * RP + (j % NSP);
           static_for<0, RP>([&](auto QI) {
             constexpr int q = decltype(QI)::value;
-            const int li = geo.template widx<s>(base + q * NSP);
+            int li;
+            if constexpr (TT % NSP == 0) {
+              const int w = (tp / NSP) * NSP * RP + (tp % NSP);
+              li = geo.template widx_c<s, b * TT * RP + q * NSP, (NSP == 1 ? RP : 1)>(w);
+            } else {
+              const int j = tp + b * TT;
+              li = geo.template widx<s>((j / NSP) * NSP * RP + (j % NSP) + q * NSP);
+            }
             if constexpr (WRE) lds_re[li] = vr[b * RP + q];
             if constexpr (WIM) (PLANES == 1 ? lds_re : lds_im)[li] = vi[b * RP + q];
           });
         });
         __syncthreads();
+        // reads: position t + b*TT + r*(N/R)
         static_for<0, NB>([&](auto BI) {
           constexpr int b = decltype(BI)::value;
-          const int j = t + b * TT;
           static_for<0, R>([&](auto RI) {
             constexpr int r = decltype(RI)::value;
-            const int li = geo.template ridx<s>(j + r * (N / R));
+            const int li = geo.template ridx_c<s, b * TT + r * (N / R), 1>(t);
             if constexpr (WRE) vr[b * R + r] = lds_re[li];
             if constexpr (WIM) vi[b * R + r] = (PLANES == 1 ? lds_re : lds_im)[li];
           });
@@ -280,6 +291,11 @@ struct FixedGeo {
   __device__ __forceinline__ int widx(int pos) const { return lidx(pos); }
   template <int S>
   __device__ __forceinline__ int ridx(int pos) const { return lidx(pos); }
+  // index of position rt + CC, rt a multiple of M
+  template <int S, int CC, int M>
+  __device__ __forceinline__ int widx_c(int rt) const { return lidx.template at<CC, M>(rt); }
+  template <int S, int CC, int M>
+  __device__ __forceinline__ int ridx_c(int rt) const { return lidx.template at<CC, M>(rt); }
   template <int S, typename T, int P>
   __device__ __forceinline__ void after_read(T (&)[P], T (&)[P]) const {}
 };
@@ -334,10 +350,20 @@ struct ColCfgSel {
 template <typename T, int N, bool WIDE_OK = false>
 using ColCfg = typename ColCfgSel<T, N, WIDE_OK>::type;
 
-// LDS index maps.
+// LDS index maps.  at<CC, M>(rt) is the index of position rt + CC for a
+// per-thread rt that is a multiple of M: a padded map p + (p >> k) splits
+// into map(rt) + map(CC) when either part is a multiple of 2^k.
 struct RowLIdx {
   int base;
   __device__ __forceinline__ int operator()(int pos) const { return base + pos + (pos >> 4); }
+  template <int CC, int M>
+  __device__ __forceinline__ int at(int rt) const {
+    constexpr bool SPLIT = CC % 16 == 0 || M % 16 == 0;
+    if constexpr (SPLIT)
+      return base + (rt + (rt >> 4)) + (CC + (CC >> 4));
+    else
+      return (*this)(rt + CC);
+  }
 };
 // Column tiles of 32 or more columns need no padding: a half-wave's 32
 // lanes hold 32 adjacent columns of one line position, one per bank, so
@@ -352,6 +378,19 @@ struct ColLIdx {
       return a;
     else
       return a + (a >> 5);
+  }
+  template <int CC, int M>
+  __device__ __forceinline__ int at(int rt) const {
+    // a = rt*C + col + CC*C: the padding splits when CC*C is a multiple of 32
+    constexpr int A = CC * C;
+    if constexpr (C % 32 == 0) {
+      return rt * C + col + A;
+    } else if constexpr (A % 32 == 0) {
+      const int a = rt * C + col;
+      return a + (a >> 5) + (A + (A >> 5));
+    } else {
+      return (*this)(rt + CC);
+    }
   }
 };
 

@@ -369,6 +369,15 @@ struct RowLaneGeo {
   __device__ __forceinline__ static int lidx(int line, int pos) {
     return line * LSTRIDE + pos + (pos >> 4);
   }
+  // position rt + CC of `line`, rt a multiple of M: the padding splits into a
+  // per-thread part and a constant when either part is a multiple of 16
+  template <int CC, int M>
+  __device__ __forceinline__ static int lidx_c(int line, int rt) {
+    if constexpr (CC % 16 == 0 || M % 16 == 0)
+      return line * LSTRIDE + (rt + (rt >> 4)) + (CC + (CC >> 4));
+    else
+      return lidx(line, rt + CC);
+  }
   template <int S>
   __device__ __forceinline__ int t() const { return inter(S) ? ti : tc; }
   template <int S>
@@ -377,6 +386,15 @@ struct RowLaneGeo {
   __device__ __forceinline__ int widx(int pos) const {
     if constexpr (inter(S - 1)) return lidx((S - 1 == S_IN && CROSS != 0) ? lb : li, pos);
     else return lidx(lc, pos);
+  }
+  template <int S, int CC, int M>
+  __device__ __forceinline__ int ridx_c(int rt) const {
+    return lidx_c<CC, M>(inter(S) ? li : lc, rt);
+  }
+  template <int S, int CC, int M>
+  __device__ __forceinline__ int widx_c(int rt) const {
+    if constexpr (inter(S - 1)) return lidx_c<CC, M>((S - 1 == S_IN && CROSS != 0) ? lb : li, rt);
+    else return lidx_c<CC, M>(lc, rt);
   }
   template <int S, typename U, int PP>
   __device__ __forceinline__ void after_read(U (&vr)[PP], U (&vi)[PP]) const {
