@@ -325,21 +325,20 @@ __device__ __forceinline__ void lane_dft(T& ur, T& ui, const LaneDft<T>& ld) {
 // lanes (lane n1 holds row slot n1 of its group, the same positions in every
 // lane).  Forward: DFT across the lanes (lane n1 then holds k1 = bitrev(n1)),
 // then the twiddle W_ny^(n2*k1).  Inverse: lane n1 holds k1 = n1; conjugate
-// twiddle, inverse DFT across the lanes scaled by 1/N1 (lane n1 then holds
-// the natural row slot bitrev(n1)).
+// twiddle, inverse DFT across the lanes, unscaled (the caller applies 1/N1;
+// lane n1 then holds the natural row slot bitrev(n1)).
 template <typename T, int P, int N1, bool INV>
 __device__ __forceinline__ void cross_lanes(T (&vr)[P], T (&vi)[P], int n1, T wr, T wi) {
   const LaneDft<T> ld(n1);
-  constexpr T sc = T(1) / T(N1);
 #pragma unroll
   for (int i = 0; i < P; ++i) {
     if constexpr (INV) {
+      // the inverse step's 1/N1 is applied by the row pass together with
+      // the row transform's 1/NX (one exact power-of-two scaling)
       const T a = vr[i] * wr - vi[i] * wi;
       vi[i] = vr[i] * wi + vi[i] * wr;
       vr[i] = a;
       lane_dft<T, N1, true, false>(vr[i], vi[i], ld);
-      vr[i] *= sc;
-      vi[i] *= sc;
     } else {
       lane_dft<T, N1, false, false>(vr[i], vi[i], ld);
       const T a = vr[i] * wr - vi[i] * wi;
@@ -583,14 +582,19 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
   };
   line_fft(std::bool_constant<INV1>{}, std::false_type{}, std::integral_constant<int, CROSS1>{});
   constexpr T invN = T(1) / T(NX);
+  // the first transform's scale: 1/NX if inverse, times the 1/N1 of an
+  // inverse cross-row step on lanes (powers of two, so one multiplication
+  // gives the two scalings' result)
+  constexpr bool SC1 = INV1 || CROSS1 == 2;
+  constexpr T sc1 = (INV1 ? T(1) / T(NX) : T(1)) * (CROSS1 == 2 ? T(1) / T(N1) : T(1));
   constexpr bool REV_OUT = MODE == 1;
   if constexpr (MODE == 1) {
     if (active) {
 #pragma unroll
       for (int i = 0; i < P; ++i) {
-        if constexpr (INV1) {
-          vr[i] *= invN;
-          vi[i] *= invN;
+        if constexpr (SC1) {
+          vr[i] *= sc1;
+          vi[i] *= sc1;
         }
         if constexpr (PF)
           op.mid_aux(f, PL::template out_pos<false>(t, i), y, vr[i], vi[i], acc, aux[i]);
@@ -600,13 +604,16 @@ __device__ __forceinline__ void row_body(const Op& op, const S* __restrict__ ire
     }
     line_fft(std::bool_constant<INV2>{}, std::true_type{}, std::integral_constant<int, CROSS2>{});
   }
-  constexpr bool INV_LAST = MODE == 1 ? INV2 : INV1;
+  // the last transform's scale (a second transform's cross-row step is
+  // always the forward one)
+  constexpr bool SC_LAST = MODE == 1 ? INV2 : SC1;
+  constexpr T sc_last = MODE == 1 ? invN : sc1;
   if (active) {
 #pragma unroll
     for (int i = 0; i < P; ++i) {
-      if constexpr (INV_LAST) {
-        vr[i] *= invN;
-        vi[i] *= invN;
+      if constexpr (SC_LAST) {
+        vr[i] *= sc_last;
+        vi[i] *= sc_last;
       }
       op.post(f, PL::template out_pos<REV_OUT>(t, i), y, vr[i], vi[i], acc);
     }

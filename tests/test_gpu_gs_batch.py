@@ -1,17 +1,17 @@
 """Batched far-field GS/WGS over designs of uneven sizes (the persistent
 batch engine, `doe_gs_batch_*`).
 
-Parity: each design of a batch must reproduce the single-design far-field
-loop (`doe_gs_run`, FP32), which is itself pinned against the FP64 oracle
-(test_gpu_gs.py: FP64 iterates within 1e-9, FP32 first ten iterations within
-1e-4, published results).  The batch runs the same pass kernels on the same
-per-design geometry, so the results are expected to agree to rounding of the
-same operations (bitwise in practice).  The graph-replayed loop is checked
-against the eagerly launched one on every replay.
+Parity: each design of a batch is checked against the FP64 oracle's
+fourierCgh / WGS loop directly (first ten iterations' error and efficiency
+within 1e-4, the FP32 gate), and must reproduce the single-design far-field
+loop (`doe_gs_run`, FP32) to rounding of the same operations (bitwise in
+practice), over longer runs too.  The graph-replayed loop is checked against
+the eagerly launched one on every replay.
 """
 import numpy as np
 import pytest
 
+import oracle_js as oj
 from diffractiveopticalelement_amd import native as nat
 from test_gpu_gs import problem
 
@@ -64,6 +64,31 @@ def test_batch_matches_single_designs(ctx, alg, with_roi):
             assert close(g["weights"], r["weights"], 1e-4 * np.max(r["weights"])), i
         else:
             assert np.isnan(g["rmse"])
+    b.close()
+
+
+@pytest.mark.parametrize("alg", ["gs", "wgs"])
+def test_batch_designs_against_the_oracle(ctx, alg):
+    # a mixed batch (sides 64, 128, 64, 256) against the oracle's far-field
+    # loop on each design's inputs (doe.js:455-546, 639-730)
+    sides = [64, 128, 64, 256]
+    iters = 10
+    print(f"gs batch vs oracle sides={sides} alg={alg}", flush=True)
+    b = nat.GsBatch(ctx, sides, nat.ALG_WGS if alg == "wgs" else nat.ALG_GS, 0.9, True)
+    inputs = []
+    for i, n in enumerate(sides):
+        ill, t, roi, phi0 = problem(n, 8e-6, 300 + i)
+        b.set(i, ill, t, roi, phi0)
+        inputs.append((ill, t, roi, phi0))
+    b.run(iters, graph=True)
+    for i, n in enumerate(sides):
+        ill, t, roi, phi0 = inputs[i]
+        args = {"mode": "far-field", "algorithm": alg, "nx": n, "ny": n, "dx": 8e-6, "dy": 8e-6,
+                "lambda": LAM, "z": 0.03, "iterations": iters, "beta": 0.9}
+        _, ref = oj.run("gs", args, {"illum": ill, "target": t, "roi": roi, "phase0": phi0})
+        g = b.get(i)
+        assert close(g["error"], ref["error"], 1e-4), (i, n)
+        assert close(g["efficiency"], ref["efficiency"], 1e-4), (i, n)
     b.close()
 
 
