@@ -387,7 +387,8 @@ def c3_targets(n: int, rng, polygon: bool) -> np.ndarray:
     return I
 
 
-def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7, graph: bool = True) -> dict:
+def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7, graph: bool = True,
+           per_class: bool = False) -> dict:
     """Batched uneven far-field WGS: `count` designs, sides drawn from
     {256, 512, 1024, 2048}, half spot arrays and half filled polygons, binary,
     power-matched to a top-hat illumination; `iters` WGS iterations in one
@@ -407,10 +408,10 @@ def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7, graph: bool =
         b.set(i, ills[n], t)
     b.random_init(seed)
     setup_s = time.perf_counter() - t0
-    b.run(5, graph=graph)  # warm-up (captures a 5-iteration graph)
+    b.run(5, graph=graph, per_class=per_class)  # warm-up (captures a 5-iteration graph)
     ctx.sync()
     ctx.timer_start()
-    b.run(iters, graph=graph)
+    b.run(iters, graph=graph, per_class=per_class)
     ms = ctx.timer_stop()
     pts = sum(n * n for n in sides)
     best = [b.get(i)["error"][-1] for i in range(0, count, max(1, count // 8))]
@@ -418,7 +419,9 @@ def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7, graph: bool =
     return {
         "config": {"workload": f"{count} far-field WGS designs, sides from {{256, 512, 1024, 2048}}^2 "
                                f"(seed {seed}), spot arrays and filled polygons, {iters} iterations, "
-                               "FP32, " + ("hipGraph-captured" if graph else "eager launches"),
+                               "FP32, " + ("hipGraph-captured" if graph else "eager launches")
+                               + (", one launch per pass per size class" if per_class
+                                  else ", one launch per pass for the whole batch"),
                    "points": pts,
                    "sides": {str(n): sides.count(n) for n in sorted(set(sides))}},
         "ms": round(ms, 2), "designs_per_s": round(count / (ms * 1e-3), 1),
@@ -597,6 +600,9 @@ def main() -> int:
     ap.add_argument("--no-c3", action="store_true", help="skip the batched uneven GS leg")
     ap.add_argument("--c3-count", type=int, default=256, help="designs of the batched uneven GS leg")
     ap.add_argument("--c3-iters", type=int, default=200, help="iterations of the batched uneven GS leg")
+    ap.add_argument("--c3-per-class", action="store_true",
+                    help="run the batched uneven GS leg with one launch per pass per size class "
+                         "instead of one per pass for the whole batch")
     ap.add_argument("--c3-eager", action="store_true",
                     help="launch the batched uneven GS iterations one by one instead of from a graph "
                          "(for profilers that count per dispatch)")
@@ -711,7 +717,8 @@ def main() -> int:
     errors = None if args.no_errors else errors_leg(ctx, field, zs)
     variants = None if args.no_variants else storage_variants(ctx, field, args.steps)
     field.free()
-    c3 = None if args.no_c3 else c3_leg(ctx, args.c3_count, args.c3_iters, graph=not args.c3_eager)
+    c3 = None if args.no_c3 else c3_leg(ctx, args.c3_count, args.c3_iters, graph=not args.c3_eager,
+                                       per_class=args.c3_per_class)
     targets = None if args.no_targets else targets_leg(ctx)
     fab = None if args.no_targets else fab_leg(ctx)
     c1 = None if args.no_c1 else c1_leg()

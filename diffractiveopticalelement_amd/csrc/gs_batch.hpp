@@ -41,6 +41,7 @@ struct doe_gs_batch {
   int last_k = 0;  // iterations of the last run
   hipGraphExec_t graph = nullptr;
   int graph_k = -1;
+  bool graph_merged = false;  // the captured loop's launch layout
 
   ~doe_gs_batch() {
     if (graph) hipGraphExecDestroy(graph);

@@ -8,6 +8,7 @@ there is no CPU fallback anywhere on this path.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -192,6 +193,7 @@ SIGNATURES = [
 ]
 
 GSB_GRAPH = 1
+GSB_PER_CLASS = 2
 
 _lib: C.CDLL | None = None
 
@@ -241,9 +243,16 @@ class Context:
         h = C.c_void_p()
         check(self.lib.doe_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h)))
         self.h = h
+        # batches and networks hold the native context: closed before it
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
 
     def close(self) -> None:
         if self.h:
+            for child in list(self._children):
+                child.close()
             self.lib.doe_ctx_destroy(self.h)
             self.h = None
 
@@ -569,6 +578,7 @@ class GsBatch:
                                           int(with_roi), C.byref(h)))
         self.h = h
         self.iterations = 0
+        ctx._adopt(self)
 
     def set(self, index: int, illumination, target_amp, roi=None, init_phase=None) -> None:
         ill = _darr(illumination).ravel()
@@ -582,8 +592,11 @@ class GsBatch:
     def random_init(self, seed: int) -> None:
         check(self.ctx.lib.doe_gs_batch_random_init(self.h, C.c_ulonglong(seed)))
 
-    def run(self, iterations: int, graph: bool = True) -> None:
-        check(self.ctx.lib.doe_gs_batch_run(self.h, int(iterations), GSB_GRAPH if graph else 0))
+    def run(self, iterations: int, graph: bool = True, per_class: bool = False) -> None:
+        """per_class: one launch per pass per size class (DOE_GSB_PER_CLASS)
+        instead of one per pass for the whole batch."""
+        flags = (GSB_GRAPH if graph else 0) | (GSB_PER_CLASS if per_class else 0)
+        check(self.ctx.lib.doe_gs_batch_run(self.h, int(iterations), flags))
         self.iterations = int(iterations)
 
     def get(self, index: int) -> dict:
@@ -649,6 +662,7 @@ class WsNet:
         check(ctx.lib.doe_wsnet_create(ctx.h, res, base, max_batch, C.c_ulonglong(seed),
                                        C.byref(h)))
         self.h = h
+        ctx._adopt(self)
         self.res, self.base, self.max_batch = res, base, max_batch
         n = C.c_long()
         check(ctx.lib.doe_wsnet_param_count(self.h, C.byref(n)))

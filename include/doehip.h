@@ -49,7 +49,9 @@ int doe_version(void);
 const char* doe_last_error(void);
 const char* doe_status_string(int status);
 int doe_device_count(int* count);
-/* stream: an existing hipStream_t to launch on, or NULL for a private one. */
+/* stream: an existing hipStream_t to launch on, or NULL for a private one.
+ * Batches (doe_gs_batch) and networks (doe_wsnet) keep their context: destroy
+ * them before it. */
 int doe_ctx_create(int device, void* stream, doe_ctx** out);
 int doe_ctx_destroy(doe_ctx* ctx);
 int doe_ctx_sync(doe_ctx* ctx);
@@ -215,11 +217,15 @@ int doe_gs_multiplane_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* i
  * fourierCgh (src/doe.js:639-730), under DOE_ALG_WGS with the weight update
  * and best-iterate tracking of weightedGerchbergSaxton (doe.js:455-546), with
  * the same results as doe_gs_run in DOE_MODE_FAR_FIELD / DOE_F32 on that
- * design alone.  A batch holds `count` square designs of sides 256, 512,
- * 1024 or 2048, mixed freely; every iteration of the whole batch is two
- * persistent kernel launches driven by a per-design size table, and with
- * DOE_GSB_GRAPH the iteration loop is captured once as a hipGraph and
- * replayed.  Inputs per design (doe_gs_batch_set): illumination, target
+ * design alone.  A batch holds `count` square designs of power-of-two sides
+ * 64 ... 4096, mixed freely, grouped into size classes.  Every iteration is
+ * one column-pass, one finalize and one row-pass launch for the whole batch,
+ * each workgroup finding its size class in a table of the classes (side,
+ * planes, state, first workgroup) and running that side's pass body; with
+ * DOE_GSB_PER_CLASS, or when a side exceeds 2048, it is one column-pass and
+ * one row-pass launch per size class instead.  With DOE_GSB_GRAPH
+ * the iteration loop is captured once as a hipGraph and replayed.  Inputs
+ * per design (doe_gs_batch_set): illumination, target
  * amplitude, ROI (only if the batch was created with_roi; then required),
  * initial phase (NULL keeps the current one; doe_gs_batch_random_init draws
  * every design's phase on the device from a counter-based splitmix64 stream,
@@ -229,6 +235,7 @@ int doe_gs_multiplane_run(doe_ctx* ctx, const doe_gs_plan* plan, const double* i
  * reported iterate, weights, best rmse (NaN for GS). */
 typedef struct doe_gs_batch doe_gs_batch;
 #define DOE_GSB_GRAPH 1
+#define DOE_GSB_PER_CLASS 2
 int doe_gs_batch_create(doe_ctx* ctx, int count, const int* sides, int algorithm, double beta,
                         int with_roi, doe_gs_batch** out);
 int doe_gs_batch_destroy(doe_gs_batch* batch);

@@ -13,7 +13,7 @@ import pytest
 
 import oracle_js as oj
 from diffractiveopticalelement_amd import native as nat
-from test_gpu_gs import problem
+from test_gpu_gs import phase_err, problem
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,42 @@ def test_batch_designs_against_the_oracle(ctx, alg):
         assert close(g["error"], ref["error"], 1e-4), (i, n)
         assert close(g["efficiency"], ref["efficiency"], 1e-4), (i, n)
     b.close()
+
+
+@pytest.mark.parametrize("alg", [nat.ALG_GS, nat.ALG_WGS])
+def test_merged_launches_match_per_class_launches(ctx, alg):
+    # one launch per pass for the whole batch (a table of size classes, the
+    # default) gives the per-class launches' designs: the same line
+    # transforms, other workgroup tilings (the FP64 sums fold in another order)
+    sides = [256, 1024, 64, 512, 256, 2048, 128, 64]
+    iters = 12
+    print(f"gs batch merged vs per-class sides={sides} alg={alg}", flush=True)
+    b = nat.GsBatch(ctx, sides, alg, 0.9, True)
+    try:
+        ills = []
+        for i, n in enumerate(sides):
+            ill, t, roi, phi0 = problem(n, 8e-6, 500 + i)
+            b.set(i, ill, t, roi, phi0)
+            ills.append(ill.ravel())
+        b.run(iters, graph=True, per_class=True)
+        ref = [b.get(i) for i in range(len(sides))]
+        for graph in (False, True):
+            b.run(iters, graph=graph)
+            for i in range(len(sides)):
+                g = b.get(i)
+                assert close(g["error"], ref[i]["error"], 1e-6), (graph, i)
+                assert close(g["efficiency"], ref[i]["efficiency"], 1e-6), (graph, i)
+                # the element phase where the illumination carries it (outside
+                # the aperture it multiplies zero and is not determined)
+                assert phase_err(g["phase"], ref[i]["phase"], ills[i]) <= 1e-5, (graph, i)
+                if alg == nat.ALG_WGS:
+                    # weights compound per-point ratios (t/a)^0.9, clamped to
+                    # [1/4, 4]: compared in relative L2 (a point whose measured
+                    # amplitude is near zero may sit on either side of a clamp)
+                    dw = np.linalg.norm(g["weights"] - ref[i]["weights"])
+                    assert dw <= 1e-4 * np.linalg.norm(ref[i]["weights"]), (graph, i, dw)
+    finally:
+        b.close()
 
 
 def test_graph_replays_match_eager(ctx):
