@@ -350,8 +350,8 @@ int run_gs(doe_ctx* ctx, const doe_gs_plan* plan, const double* illum, const dou
   Plan2 pl;
   rc = make_plan2(ctx, nx, ny, sizeof(T), col_split<T>(nx, ny), &pl);
   if (rc) return rc;
-  // reduction scratch: partials per workgroup (row or column pass), one
-  // arrival counter per field
+  // reduction scratch: partial sums per workgroup of a reducing row or
+  // column pass
   rc = ensure_reduce(ctx, size_t(std::max(ny, 4 * nx)) * R_NFF);
   if (rc) return rc;
   T* re = W.fre;
