@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "gs_batch.hpp"
+#include "merged.hpp"
 
 namespace doe {
 
@@ -63,139 +64,21 @@ inline const char* gsb_tag(int n, int pass) {
   return pass == 0 ? col[i] : (pass == 1 ? row[i] : ini[i]);
 }
 
-// ---------------------------------------------------------------------------
 // One launch per pass over every size class (the default; DOE_GSB_PER_CLASS
-// selects the per-class launches further down).  A table of the
-// batch's size classes (side, pass operation, planes, twiddles, and the first
-// workgroup of each class) drives one column-pass launch, one finalize launch
-// and one row-pass launch per iteration; each workgroup finds its class from
-// the table (a uniform scan of at most 7 entries) and runs that side's pass
-// body.  Every body is given a 512-thread geometry (row passes: 512 / (NX/16
-// x N1) row groups per workgroup; column passes: as many columns as fill 512
-// threads), so one block size and one LDS budget serve them all.  Sides
-// 64 ... 2048 (a 4096-point row needs 1024 threads).
-// ---------------------------------------------------------------------------
-constexpr int kMaxCls = 6;  // sides 64 ... 2048
-constexpr int kMWG = 512;
-
-// Column bodies keep the per-class passes' line transforms (16 points per
-// thread with both LDS planes below 256-point lines, 32 with one plane
-// from 256), widened to 512 threads by their column count.
-template <int N2>
-using MColCF = typename std::conditional<
-    (N2 < 256), ColGeom<float, N2, 16, 512 / (N2 / 16), 2>,
-    typename std::conditional<(N2 <= 512), ColGeom<float, N2, 32, 512 / (N2 / 32), 1>,
-                              ColGeom<float, N2, 32, 16, 2>>::type>::type;
-template <int NX>
-constexpr int m_n1() {
-  return NX == 1024 ? 2 : (NX == 2048 ? 4 : 1);
-}
-template <int NX>
-using MRowCF = RowCfg<float, NX, m_n1<NX>(), 2, kMWG>;
-
-struct GsbColEntry {
-  OpFFB<float> op;
-  float* re;
-  float* im;
-  const float* twr;
-  const float* twi;
-  int n, nx, tiles, n1cnt;
-};
-struct GsbRowEntry {
-  OpFFA<float> op;
-  float* re;
-  float* im;
-  const float* twr;
-  const float* twi;
-  const float* twyr;
-  const float* twyi;
-  long groups;
-  int n;
-};
-template <class E>
-struct GsbTable {
-  int ncls;
-  int first[kMaxCls + 1];  // first workgroup of each class; first[ncls] = grid
-  E e[kMaxCls];
-};
-
-__device__ __forceinline__ int gsb_class(const int* first, int ncls, int b) {
-  int c = 0;
-  while (c + 1 < ncls && b >= first[c + 1]) ++c;
-  return c;
-}
-
-template <int N2>
-constexpr int mcol_lds() {
-  return MColCF<N2>::LDS_ELEMS + MColCF<N2>::LDS_IM;
-}
-template <int NX>
-constexpr int mrow_lds() {
-  return MRowCF<NX>::LDS_ELEMS + MRowCF<NX>::LDS_IM;
-}
-constexpr int kMColLds = cmax(cmax(mcol_lds<64>(), mcol_lds<128>()),
-                              cmax(mcol_lds<256>(), mcol_lds<512>()));
-constexpr int kMRowLds = cmax(cmax(cmax(mrow_lds<64>(), mrow_lds<128>()), cmax(mrow_lds<256>(), mrow_lds<512>())),
-                              cmax(mrow_lds<1024>(), mrow_lds<2048>()));
-
-__global__ void __launch_bounds__(kMWG) k_gsb_col_merged(GsbTable<GsbColEntry> tab, int rev) {
-  __shared__ float lds[kMColLds];
-  const int b = rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x);
-  const int c = gsb_class(tab.first, tab.ncls, b);
-  const GsbColEntry& E = tab.e[c];
-  const int nb = tab.first[c + 1] - tab.first[c];
-  const int lb = xcd_remap(b - tab.first[c], nb);
-  auto body = [&](auto NC) {
-    constexpr int N2 = decltype(NC)::value;
-    using CF = MColCF<N2>;
-    static_assert(CF::WG == kMWG, "merged column bodies are 512 threads");
-    col_body<float, float, N2, 1, false, true, OpFFB<float>, CF>(
-        E.op, E.re, E.im, E.re, E.im, E.nx, E.tiles, E.n1cnt, E.twr, E.twi, lb, lds,
-        lds + CF::LDS_ELEMS, nullptr);
-  };
-  switch (E.n / (E.n >= 2048 ? 4 : (E.n == 1024 ? 2 : 1))) {  // N2 of the side
-    case 64: body(std::integral_constant<int, 64>{}); break;
-    case 128: body(std::integral_constant<int, 128>{}); break;
-    case 256: body(std::integral_constant<int, 256>{}); break;
-    default: body(std::integral_constant<int, 512>{}); break;
-  }
-}
-
-__global__ void __launch_bounds__(kMWG) k_gsb_row_merged(GsbTable<GsbRowEntry> tab, int rev) {
-  __shared__ float lds[kMRowLds];
-  const int b = rev ? int(gridDim.x) - 1 - int(blockIdx.x) : int(blockIdx.x);
-  const int c = gsb_class(tab.first, tab.ncls, b);
-  const GsbRowEntry& E = tab.e[c];
-  const int lb = b - tab.first[c];
-  auto body = [&](auto NC) {
-    constexpr int NX = decltype(NC)::value;
-    using CF = MRowCF<NX>;
-    static_assert(CF::WG == kMWG, "merged row bodies are 512 threads");
-    row_body<float, float, NX, m_n1<NX>(), 1, true, false, true, true, OpFFA<float>, CF>(
-        E.op, E.re, E.im, E.re, E.im, E.n, E.groups, E.twr, E.twi, E.twyr, E.twyi, lb, lds,
-        lds + CF::LDS_ELEMS);
-  };
-  switch (E.n) {
-    case 64: body(std::integral_constant<int, 64>{}); break;
-    case 128: body(std::integral_constant<int, 128>{}); break;
-    case 256: body(std::integral_constant<int, 256>{}); break;
-    case 512: body(std::integral_constant<int, 512>{}); break;
-    case 1024: body(std::integral_constant<int, 1024>{}); break;
-    default: body(std::integral_constant<int, 2048>{}); break;
-  }
-}
-
+// selects the per-class launches further down): the table-driven pass
+// kernels of merged.hpp with the far-field operations, and a finalize
+// kernel over every design.
 // Fold every design's column-pass partials and finalise (one workgroup per
 // design; design d of the batch's class order).
 struct GsbFinEntry {
   OpFFB<float> op;
   int per_field;  // column-pass workgroups per design
 };
-__global__ void __launch_bounds__(256) k_gsb_fin_merged(GsbTable<GsbFinEntry> tab) {
+__global__ void __launch_bounds__(256) k_gsb_fin_merged(MTable<GsbFinEntry> tab) {
   constexpr int NA = OpFFB<float>::NACC;
   __shared__ double sh[16 * NA];
   const int d = blockIdx.x;
-  const int c = gsb_class(tab.first, tab.ncls, d);
+  const int c = m_entry(tab.first, tab.n, d);
   const GsbFinEntry& E = tab.e[c];
   const int f = d - tab.first[c];
   const int tid = threadIdx.x;
@@ -312,28 +195,10 @@ hipError_t gsb_iterations(doe_gs_batch* B, int K) {
   return hipSuccess;
 }
 
-// Host side of the merged passes: per-side geometry of the bodies above.
-int mcol_cols(int n2) {
-  int c = 0;
-  with_n_gs(n2, [&](auto NC) {
-    constexpr int N2 = decltype(NC)::value;
-    if constexpr (N2 <= 512) c = MColCF<N2>::C;
-  });
-  return c;
-}
-int mrow_groups(int nx) {
-  int g = 0;
-  with_n_gs(nx, [&](auto NC) {
-    constexpr int NX = decltype(NC)::value;
-    if constexpr (NX <= 2048) g = MRowCF<NX>::G;
-  });
-  return g;
-}
-
 bool gsb_merged_ok(const doe_gs_batch* B) {
   for (const auto& c : B->cls)
     if (c.n > 2048) return false;
-  return int(B->cls.size()) <= kMaxCls;
+  return int(B->cls.size()) <= kMergedMax;
 }
 
 // The iteration loop with one column-pass, one finalize and one row-pass
@@ -341,11 +206,11 @@ bool gsb_merged_ok(const doe_gs_batch* B) {
 hipError_t gsb_iterations_merged(doe_gs_batch* B, int K) {
   doe_ctx* ctx = B->ctx;
   const bool gs = B->algorithm == DOE_ALG_GS;
-  GsbTable<GsbColEntry> ct{};
-  GsbTable<GsbRowEntry> rt{};
-  GsbTable<GsbFinEntry> ft{};
+  MTable<MColEntry<OpFFB<float>>> ct{};
+  MTable<MRowEntry<OpFFA<float>>> rt{};
+  MTable<GsbFinEntry> ft{};
   const int ncls = int(B->cls.size());
-  ct.ncls = rt.ncls = ft.ncls = ncls;
+  ct.n = rt.n = ft.n = ncls;
   int cb = 0, rb = 0, fb = 0;
   for (int k = 0; k < ncls; ++k) {
     auto& c = B->cls[k];
@@ -358,14 +223,16 @@ hipError_t gsb_iterations_merged(doe_gs_batch* B, int K) {
     float* re = B->re + c.off;
     float* im = B->im + c.off;
     ct.first[k] = cb;
-    ct.e[k] = GsbColEntry{OpFFB<float>{c.b, c.partials, 0, 0}, re, im, (const float*)c.pl.tc.re,
-                          (const float*)c.pl.tc.im, c.n, c.n, tiles, n1};
+    ct.e[k] = MColEntry<OpFFB<float>>{OpFFB<float>{c.b, c.partials, 0, 0}, re, im, re, im,
+                                      (const float*)c.pl.tc.re, (const float*)c.pl.tc.im, n2,
+                                      c.n, tiles, n1};
     cb += tiles * n1 * nd;
     const long groups = long(c.n / n1) * nd;
     rt.first[k] = rb;
-    rt.e[k] = GsbRowEntry{OpFFA<float>{c.b, 0}, re, im, (const float*)c.pl.tx.re,
-                          (const float*)c.pl.tx.im, (const float*)c.pl.ty.re,
-                          (const float*)c.pl.ty.im, groups, c.n};
+    rt.e[k] = MRowEntry<OpFFA<float>>{OpFFA<float>{c.b, 0}, re, im, re, im,
+                                      (const float*)c.pl.tx.re, (const float*)c.pl.tx.im,
+                                      (const float*)c.pl.ty.re, (const float*)c.pl.ty.im, groups,
+                                      c.n};
     rb += int((groups + g - 1) / g);
     ft.first[k] = fb;
     ft.e[k] = GsbFinEntry{OpFFB<float>{c.b, c.partials, 0, 0}, tiles * n1};
@@ -384,13 +251,13 @@ hipError_t gsb_iterations_merged(doe_gs_batch* B, int K) {
       rt.e[k].op.write_phase = last;
     }
     prof_begin(ctx, "gsb_col_merged", long(B->total));
-    hipLaunchKernelGGL(k_gsb_col_merged, dim3(unsigned(cb)), dim3(kMWG), 0, ctx->stream, ct,
-                       next_dir(ctx));
+    hipLaunchKernelGGL((k_col_merged<OpFFB<float>, 1, false, true>), dim3(unsigned(cb)),
+                       dim3(kMWG), 0, ctx->stream, ct, next_dir(ctx));
     hipLaunchKernelGGL(k_gsb_fin_merged, dim3(unsigned(fb)), dim3(256), 0, ctx->stream, ft);
     prof_end(ctx);
     prof_begin(ctx, "gsb_row_merged", long(B->total));
-    hipLaunchKernelGGL(k_gsb_row_merged, dim3(unsigned(rb)), dim3(kMWG), 0, ctx->stream, rt,
-                       next_dir(ctx));
+    hipLaunchKernelGGL((k_row_merged<OpFFA<float>, 1, true, false, true, true>),
+                       dim3(unsigned(rb)), dim3(kMWG), 0, ctx->stream, rt, next_dir(ctx));
     prof_end(ctx);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

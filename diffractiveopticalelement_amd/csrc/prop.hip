@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "merged.hpp"
 #include "ops_prop.hpp"
 
 namespace doe {
@@ -146,6 +147,65 @@ DOE_PROP_EXTERN(float, __half)
 DOE_PROP_EXTERN(float, __hip_bfloat16)
 #elif DOE_PART == 1
 DOE_PROP_INST(float, float)
+
+// Angular-spectrum propagation of several FP32 batches of square fields of
+// different sides (64 ... 2048), at most kMergedMax per call: the rows
+// forward, the fused column pass and the rows inverse each run as one launch
+// for all of them (merged.hpp), the same three passes as run_asm per batch.
+int run_asm_multi_f32(doe_ctx* ctx, int count, const doe_field* const* in, doe_field* const* out,
+                      const OpAsmH* ops) {
+  MTable<MRowEntry<OpNone>> rf{}, ri{};
+  MTable<MColEntry<OpAsmH>> cc{};
+  rf.n = ri.n = cc.n = count;
+  int rb = 0, cb = 0;
+  for (int k = 0; k < count; ++k) {
+    const int n = in[k]->nx;
+    Plan2 pl;
+    int rc = make_plan2(ctx, n, n, sizeof(float), col_split<float>(n, n), &pl);
+    if (rc) return rc;
+    const int n1 = pl.n1, n2 = n / n1;
+    const int cols = mcol_cols(n2), g = mrow_groups(n);
+    DOE_REQUIRE(cols > 0 && g > 0 && n1 == (n == 1024 ? 2 : (n == 2048 ? 4 : 1)), DOE_ERR_UNSUPPORTED,
+                "doe_asm_multi: sides 64 ... 2048");
+    const float* ire = (const float*)in[k]->re;
+    const float* iim = (const float*)in[k]->im;
+    float* re = (float*)out[k]->re;
+    float* im = (float*)out[k]->im;
+    const long groups = long(n / n1) * in[k]->batch;
+    const int tiles = (n + cols - 1) / cols;
+    const OpNone none{};
+    rf.first[k] = ri.first[k] = rb;
+    rf.e[k] = MRowEntry<OpNone>{none, ire, iim, re, im, (const float*)pl.tx.re,
+                                (const float*)pl.tx.im, (const float*)pl.ty.re,
+                                (const float*)pl.ty.im, groups, n};
+    ri.e[k] = MRowEntry<OpNone>{none, re, im, re, im, (const float*)pl.tx.re,
+                                (const float*)pl.tx.im, (const float*)pl.ty.re,
+                                (const float*)pl.ty.im, groups, n};
+    rb += int((groups + g - 1) / g);
+    cc.first[k] = cb;
+    cc.e[k] = MColEntry<OpAsmH>{ops[k], re, im, re, im, (const float*)pl.tc.re,
+                                (const float*)pl.tc.im, n2, n, tiles, n1};
+    cb += tiles * n1 * in[k]->batch;
+  }
+  rf.first[count] = ri.first[count] = rb;
+  cc.first[count] = cb;
+  long pts = 0;
+  for (int k = 0; k < count; ++k) pts += long(in[k]->count());
+  prof_begin(ctx, "asm_multi_row_fwd", pts);
+  hipLaunchKernelGGL((k_row_merged<OpNone, 0, false, false, false, true>), dim3(unsigned(rb)),
+                     dim3(kMWG), 0, ctx->stream, rf, next_dir(ctx));
+  prof_end(ctx);
+  prof_begin(ctx, "asm_multi_col_h", pts);
+  hipLaunchKernelGGL((k_col_merged<OpAsmH, 1, false, true>), dim3(unsigned(cb)), dim3(kMWG), 0,
+                     ctx->stream, cc, next_dir(ctx));
+  prof_end(ctx);
+  prof_begin(ctx, "asm_multi_row_inv", pts);
+  hipLaunchKernelGGL((k_row_merged<OpNone, 0, true, false, true, false>), dim3(unsigned(rb)),
+                     dim3(kMWG), 0, ctx->stream, ri, next_dir(ctx));
+  prof_end(ctx);
+  DOE_HIP(hipGetLastError());
+  return DOE_OK;
+}
 #elif DOE_PART == 2
 DOE_PROP_INST(double, double)
 #elif DOE_PART == 3
@@ -207,6 +267,11 @@ struct OpAsmTable {
 #if DOE_PART == 0
 using namespace doe;
 
+namespace doe {
+int run_asm_multi_f32(doe_ctx* ctx, int count, const doe_field* const* in, doe_field* const* out,
+                      const OpAsmH* ops);
+}
+
 extern "C" {
 
 int doe_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double dy, double lambda,
@@ -238,6 +303,57 @@ int doe_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double
   op.evanescent = evanescent ? 1 : 0;
   rc = DOE_DISPATCH(run_asm, in->dtype, ctx, in, out, op);
   if (rc) return rc;
+  DOE_HIP(finish(ctx));
+  return DOE_OK;
+}
+
+int doe_asm_multi(doe_ctx* ctx, int count, const doe_field* const* in, doe_field* const* out,
+                  const double* dx, const double* dy, double lambda, const double* const* z,
+                  int band_limit, int evanescent, double soft_edge) {
+  DOE_REQUIRE(ctx && in && out && dx && dy && z, DOE_ERR_INVALID, "doe_asm_multi: null argument");
+  DOE_REQUIRE(count >= 0, DOE_ERR_INVALID, "doe_asm_multi: negative count");
+  long nparams = 0;
+  for (int k = 0; k < count; ++k) {
+    int rc = check_pair(ctx, in[k], out[k]);
+    if (rc) return rc;
+    DOE_REQUIRE(z[k], DOE_ERR_INVALID, "doe_asm_multi: null z");
+    DOE_REQUIRE(in[k]->dtype == DOE_F32 && in[k]->nx == in[k]->ny && in[k]->nx >= 64 &&
+                    in[k]->nx <= 2048,
+                DOE_ERR_UNSUPPORTED, "doe_asm_multi: FP32 square fields of sides 64 ... 2048");
+    nparams += in[k]->batch;
+  }
+  if (count == 0) return DOE_OK;
+  DOE_HIP(hipSetDevice(ctx->device));
+  // every field's transfer-function parameters in one block
+  std::vector<AsmField> params;
+  params.reserve(size_t(nparams));
+  for (int k = 0; k < count; ++k)
+    for (int b = 0; b < in[k]->batch; ++b)
+      params.push_back(asm_field_params(in[k]->nx, in[k]->ny, dx[k], dy[k], lambda, z[k][b],
+                                        band_limit, soft_edge));
+  void* dp = nullptr;
+  int rc = upload_params(ctx, params.data(), params.size() * sizeof(AsmField), &dp);
+  if (rc) return rc;
+  std::vector<OpAsmH> ops(static_cast<size_t>(count));
+  long off = 0;
+  for (int k = 0; k < count; ++k) {
+    OpAsmH& op = ops[k];
+    op = OpAsmH{};
+    op.fp = (const AsmField*)dp + off;
+    op.dfx = 1 / (in[k]->nx * dx[k]);
+    op.dfy = 1 / (in[k]->ny * dy[k]);
+    op.inv_l2 = 1 / (lambda * lambda);
+    op.nx = in[k]->nx;
+    op.ny = in[k]->ny;
+    op.evanescent = evanescent ? 1 : 0;
+    off += in[k]->batch;
+  }
+  // up to kMergedMax batches per launch triple
+  for (int k0 = 0; k0 < count; k0 += kMergedMax) {
+    const int nk = std::min(kMergedMax, count - k0);
+    rc = run_asm_multi_f32(ctx, nk, in + k0, out + k0, ops.data() + k0);
+    if (rc) return rc;
+  }
   DOE_HIP(finish(ctx));
   return DOE_OK;
 }

@@ -132,6 +132,8 @@ SIGNATURES = [
                                  C.c_long, C.c_int, C.c_int]),
     ("doe_asm", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p, C.c_int,
                           C.c_int, C.c_double]),
+    ("doe_asm_multi", C.c_int, [_VP, C.c_int, _VP, _VP, c_double_p, c_double_p, C.c_double, _VP,
+                                C.c_int, C.c_int, C.c_double]),
     ("doe_asm_transfer", C.c_int, [_VP, _VP, C.c_double, C.c_double, C.c_double, C.c_double,
                                    C.c_int, C.c_int, C.c_double, c_double_p]),
     ("doe_fresnel", C.c_int, [_VP, _VP, _VP, C.c_double, C.c_double, C.c_double, c_double_p,
@@ -351,6 +353,21 @@ class Context:
         zz = _darr(np.broadcast_to(np.asarray(z, dtype=np.float64), (fin.batch,)))
         check(self.lib.doe_asm(self.h, fin.h, fout.h, dx, dy, lam, _dptr(zz), int(band_limit),
                                int(evanescent), soft_edge))
+
+    def asm_multi(self, fins, fouts, dxs, dys, lam, zs, band_limit=True, evanescent=True,
+                  soft_edge=0.02) -> None:
+        """doe_asm_multi: batches of different (square) sides in one launch per pass;
+        zs[k] holds batch k's distances."""
+        k = len(fins)
+        ins = (C.c_void_p * k)(*[f.h.value if hasattr(f.h, "value") else f.h for f in fins])
+        outs = (C.c_void_p * k)(*[f.h.value if hasattr(f.h, "value") else f.h for f in fouts])
+        dxa = _darr(dxs)
+        dya = _darr(dys)
+        zarr = [_darr(np.broadcast_to(np.asarray(z, dtype=np.float64), (f.batch,))) for z, f in
+                zip(zs, fins)]
+        zp = (c_double_p * k)(*[_dptr(a) for a in zarr])
+        check(self.lib.doe_asm_multi(self.h, k, ins, outs, _dptr(dxa), _dptr(dya), lam, zp,
+                                     int(band_limit), int(evanescent), soft_edge))
 
     def asm_transfer(self, nx, ny, dx, dy, lam, z, band_limit=True, evanescent=True,
                      soft_edge=0.02, dtype="f64"):

@@ -141,6 +141,16 @@ int doe_fft1d_host(doe_ctx* ctx, double* re, double* im, long len, long n, long 
  * them as a pure phase (ref: propagate.js:88-95).  z[b] == 0 copies. */
 int doe_asm(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, double dy,
             double lambda, const double* z, int band_limit, int evanescent, double soft_edge);
+/* Many propagations of uneven sizes (no counterpart in the reference, which
+ * propagates one ComplexField per call, propagate.js:105-133): count batches
+ * in[k] (FP32 square fields of sides 64 ... 2048, each batch one side) to
+ * out[k] (same shapes; may equal in[k]), batch k with pitch dx[k] x dy[k]
+ * and distances z[k][0 .. batch-1], as doe_asm per batch (z = 0: H = 1).
+ * Each of the three passes is one launch for up to 6 batches, driven by a
+ * table of the batches' sides, planes and transfer-function parameters. */
+int doe_asm_multi(doe_ctx* ctx, int count, const doe_field* const* in, doe_field* const* out,
+                  const double* dx, const double* dy, double lambda, const double* const* z,
+                  int band_limit, int evanescent, double soft_edge);
 /* Transfer function H on the centred grid (ref: propagate.js:57-102) into a
  * batch-1 field; also returns the propagating fraction. */
 int doe_asm_transfer(doe_ctx* ctx, doe_field* out, double dx, double dy, double lambda, double z,

@@ -135,6 +135,51 @@ def test_asm_vs_oracle(ctx, dtype, n, dx, z, band, evan):
     fo.free()
 
 
+def test_asm_multi_uneven_sizes(ctx):
+    # batches of different sides (one launch per pass for all of them): each
+    # equals doe_asm on its own batch (FP32 rounding apart where the line
+    # transforms are split differently), and the small ones the FP64 oracle
+    sides = [256, 64, 1024, 512, 128, 2048]
+    batch = [2, 3, 1, 2, 1, 1]
+    dxs = [8e-6, 8e-6, 4e-6, 6e-6, 8e-6, 4e-6]
+    print(f"asm multi sides={sides} batches={batch}", flush=True)
+    rng = np.random.default_rng(77)
+    fins, fouts, zs, us = [], [], [], []
+    for n, b in zip(sides, batch):
+        u = rng.standard_normal((b, n, n)) + 1j * rng.standard_normal((b, n, n))
+        us.append(u)
+        fins.append(ctx.upload(u.real, u.imag, "f32"))
+        fouts.append(ctx.field(n, n, b, "f32"))
+        zs.append(rng.uniform(0.01, 0.08, b))
+    ctx.asm_multi(fins, fouts, dxs, dxs, LAM, zs, band_limit=True)
+    for k, (n, b) in enumerate(zip(sides, batch)):
+        re, im = fouts[k].download()
+        got = re + 1j * im
+        g1 = ctx.field(n, n, b, "f32")
+        ctx.asm(fins[k], g1, dxs[k], dxs[k], LAM, zs[k], band_limit=True)
+        r1, i1 = g1.download()
+        assert rel(got, r1 + 1j * i1) < 1e-6, (n, rel(got, r1 + 1j * i1))
+        g1.free()
+        if n <= 128:
+            out, _ = oj.run("asm", {"nx": n, "ny": n, "dx": dxs[k], "dy": dxs[k], "lambda": LAM,
+                                    "z": float(zs[k][0]), "bandLimit": True, "evanescent": True,
+                                    "softEdge": 0.02},
+                            {"re": us[k][0].real.ravel(), "im": us[k][0].imag.ravel()})
+            ref = (out["re"] + 1j * out["im"]).reshape(n, n)
+            assert rel(got[0], ref) < TOL["f32"], n
+    # in place, and an unsupported side is refused
+    ctx.asm_multi(fins[:2], fins[:2], dxs[:2], dxs[:2], LAM, zs[:2], band_limit=True)
+    re, im = fins[1].download()
+    r0, i0 = fouts[1].download()
+    assert rel(re + 1j * im, r0 + 1j * i0) < 1e-6
+    big = ctx.field(4096, 4096, 1, "f32")
+    with pytest.raises(nat.DoeError):
+        ctx.asm_multi([big], [big], [4e-6], [4e-6], LAM, [[0.05]])
+    big.free()
+    for f in fins + fouts:
+        f.free()
+
+
 def test_asm_transfer_table_vs_oracle(ctx):
     n, dx, z = 128, 8e-6, 0.05
     out, res = oj.run("transfer", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "z": z,
