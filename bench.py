@@ -173,16 +173,27 @@ def storage_variants(ctx, field, steps: int) -> dict:
             dst.upload(re, im)
             dst.free()
         ctx.autotune_fft2d(N, N, B, dt, reps=2)  # the configurations that exist for this storage
+        # forward and inverse alternate (as in the main step), so half-width
+        # storage stays finite
         ctx.fft2d(g)
+        ctx.fft2d(g, inverse=True)
         ctx.sync()
-        reps = max(3, steps // 2)
+        reps = 2 * max(2, steps // 4)
         ctx.timer_start()
-        for _ in range(reps):
-            ctx.fft2d(g)
+        for i in range(reps):
+            ctx.fft2d(g, inverse=bool(i & 1))
         ms = ctx.timer_stop() / reps
+        # per-kernel durations from a separate profiled pair of calls
+        ctx.set_flags(asynchronous=True, profile=True)
+        for i in range(4):
+            ctx.fft2d(g, inverse=bool(i & 1))
+        ctx.sync()
+        prof = ctx.profile_report()
+        ctx.set_flags(asynchronous=True, profile=False)
         # two passes, each reading and writing both planes
         out[dt] = {"ms_per_step": round(ms, 4), "gbs_alg": round(8 * size * B * N * N / (ms * 1e-3) / 1e9, 1),
-                   "bytes_per_pt": 8 * size, "tuning": ctx.tune_get(N, N, B, dt)}
+                   "bytes_per_pt": 8 * size, "tuning": ctx.tune_get(N, N, B, dt),
+                   "kernels_avg_us": {t: round(tot / c * 1e3, 2) for t, (c, tot, _) in prof.items()}}
         g.free()
     return out
 
