@@ -182,6 +182,22 @@ __global__ void k_copy(const S* __restrict__ a, S* __restrict__ b, long n) {
   if (i < n) b[i] = a[i];
 }
 
+// Uniform noise in [-amp, amp) from a hash of the index: the autotuner's
+// scratch data.  Zero-filled planes run 5-10 % faster than any real field
+// (fewer toggling bits, higher sustained clock), which misranks variants.
+template <typename S>
+__global__ void k_fill_noise(S* __restrict__ p, long n, unsigned seed, double amp) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = unsigned(i) * 0x9E3779B1u ^ (unsigned(i >> 32) + seed) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  st<double>(p + i, amp * (double(h) * (2.0 / 4294967296.0) - 1.0));
+}
+
 template <class F>
 static int with_dtype(int dt, F&& f) {
   switch (dt) {
@@ -591,7 +607,7 @@ int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int
                  int rowplanes, long chunk_bytes) {
   DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
   DOE_REQUIRE(n1 == 0 || n1 == 1 || n1 == 2 || n1 == 4, DOE_ERR_INVALID, "tune: n1 must be 0, 1, 2 or 4");
-  DOE_REQUIRE(colvar >= 0 && colvar <= 3, DOE_ERR_INVALID, "tune: colvar must be 0..3");
+  DOE_REQUIRE(colvar >= 0 && colvar <= 5, DOE_ERR_INVALID, "tune: colvar must be 0..5");
   DOE_REQUIRE(rowplanes == 1 || rowplanes == 2, DOE_ERR_INVALID, "tune: rowplanes must be 1 or 2");
   doe_tune t;
   t.n1 = n1;
@@ -626,19 +642,29 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
     doe_field_free(f);
     return rc;
   }
-  const size_t bytes = f->count() * dtype_size(dtype);
-  if (hipMemsetAsync(f->re, 0, bytes, ctx->stream) != hipSuccess ||
-      hipMemsetAsync(f->im, 0, bytes, ctx->stream) != hipSuccess) {
+  // noise of a size that no variant overflows in half-width storage
+  // (forward and inverse alternate)
+  const long cnt = long(f->count());
+  rc = with_dtype(dtype, [&](auto tag) -> int {
+    using S = decltype(tag);
+    const unsigned grid = unsigned((cnt + 255) / 256);
+    hipLaunchKernelGGL(k_fill_noise<S>, dim3(grid), dim3(256), 0, ctx->stream, (S*)f->re, cnt,
+                       1u, 0.01);
+    hipLaunchKernelGGL(k_fill_noise<S>, dim3(grid), dim3(256), 0, ctx->stream, (S*)f->im, cnt,
+                       2u, 0.01);
+    DOE_HIP(hipGetLastError());
+    return DOE_OK;
+  });
+  if (rc) {
     doe_field_free(f);
-    set_error("autotune: hipMemsetAsync failed");
-    return DOE_ERR_HIP;
+    return rc;
   }
   const auto key = std::make_tuple(nx, ny, batch, dtype);
   const bool had = ctx->tune.count(key) > 0;
   const doe_tune saved_tune = had ? ctx->tune[key] : doe_tune{};
   const int saved_flags = ctx->flags;
   ctx->flags = DOE_CTX_ASYNC;
-  const bool f32 = dtype == DOE_F32;
+  const bool f32 = dtype != DOE_F64;  // FP32 arithmetic: every row / column variant
   const bool split_dtype = dtype != DOE_F64;
   const long fb = long(nx) * ny * 2 * long(dtype_size(dtype));
   std::vector<long> chunks = {0};
@@ -651,7 +677,7 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
       continue;
     for (int rp : {2, 1}) {
       if (rp == 1 && !f32) continue;
-      for (int cv = 0; cv <= (f32 ? 3 : 0); ++cv) {
+      for (int cv = 0; cv <= (f32 ? 5 : 0); ++cv) {
         for (long ch : chunks) {
           doe_tune t;
           t.n1 = n1;

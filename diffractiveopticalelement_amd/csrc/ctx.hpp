@@ -55,8 +55,8 @@ struct Twiddle {
 // One tuned configuration of a batched fft2d shape (see doe_autotune_fft2d).
 struct doe_tune {
   int n1 = 0;         // column split (0: the built-in choice)
-  int colvar = 0;     // column tile variant (FP32; 0: the built-in choice)
-  int rowplanes = 2;  // LDS planes of the row pass (FP32 split passes)
+  int colvar = 0;     // column tile variant (FP32 arithmetic; 0: the built-in choice)
+  int rowplanes = 2;  // LDS planes of the row pass (FP32 arithmetic)
   long chunk = -1;    // chunk bytes (-1: the context's)
 };
 
@@ -259,17 +259,38 @@ inline hipError_t launch_row(doe_ctx* ctx, const char* tag, const Op& op, const 
   }
 }
 
-// Column tile variants the tuner chooses among (FP32): 0 the op's default,
-// 1: 16 points x up to 64 columns, both planes; 2: 32 points x up to 64
-// columns, one plane; 3: 32 points x up to 32 columns, one plane.
+// Column tile variants the tuner chooses among (FP32 arithmetic): 0 the op's
+// default, 1: 16 points x up to 64 columns, both planes; 2: 32 points x up
+// to 64 columns, one plane; 3: 32 points x up to 32 columns, one plane;
+// 4: 32 points x up to 16 columns, one plane (small workgroups, several per
+// CU); 5: 16 points x up to 32 columns, one plane.
+template <typename T, int N2, class Op, int CV>
+struct ColVarSel {
+  using type = typename OpColCfg<T, N2, Op>::type;
+};
+template <typename T, int N2, class Op>
+struct ColVarSel<T, N2, Op, 1> {
+  using type = ColGeom<T, N2, 16, 64, 2>;
+};
+template <typename T, int N2, class Op>
+struct ColVarSel<T, N2, Op, 2> {
+  using type = ColGeom<T, N2, 32, 64, 1>;
+};
+template <typename T, int N2, class Op>
+struct ColVarSel<T, N2, Op, 3> {
+  using type = ColGeom<T, N2, 32, 32, 1>;
+};
+template <typename T, int N2, class Op>
+struct ColVarSel<T, N2, Op, 4> {
+  using type = ColGeom<T, N2, 32, 16, 1>;
+};
+template <typename T, int N2, class Op>
+struct ColVarSel<T, N2, Op, 5> {
+  using type = ColGeom<T, N2, 16, 32, 1>;
+};
 template <typename T, int N2, class Op, int CV>
 struct ColVar {
-  using type = typename std::conditional<
-      CV == 1, ColGeom<T, N2, 16, 64, 2>,
-      typename std::conditional<
-          CV == 2, ColGeom<T, N2, 32, 64, 1>,
-          typename std::conditional<CV == 3, ColGeom<T, N2, 32, 32, 1>,
-                                    typename OpColCfg<T, N2, Op>::type>::type>::type>::type;
+  using type = typename ColVarSel<T, N2, Op, CV>::type;
 };
 
 template <typename T, typename S, int N2, int MODE, bool INV1, bool INV2, class Op, int CV = 0>
@@ -318,9 +339,21 @@ inline hipError_t col_pass(doe_ctx* ctx, const char* tag, const Op& op, const S*
 }
 
 // The tuned configuration for an fft2d shape, or the built-in one.
+// Half-width storage (FP16 / BF16) of 2048+ lines defaults to plain passes
+// with one-plane rows: at 4 B per point the passes are VALU-bound, and the
+// plain row pass (58 VGPRs, 8 waves per SIMD) beats the split's cross-row
+// network (16 x 2048^2 of noise: 0.248-0.257 ms against 0.26-0.27 ms for
+// the split with two-plane rows; the tuner's best, the split with one-plane
+// rows in 128 MiB chunks, 0.246 ms).
 inline doe_tune tune_for(const doe_ctx* ctx, int nx, int ny, int batch, int dtype) {
   auto it = ctx->tune.find(std::make_tuple(nx, ny, batch, dtype));
-  return it == ctx->tune.end() ? doe_tune{} : it->second;
+  if (it != ctx->tune.end()) return it->second;
+  doe_tune t{};
+  if ((dtype == DOE_F16 || dtype == DOE_BF16) && nx >= 2048 && ny >= 2048) {
+    t.n1 = 1;
+    t.rowplanes = 1;
+  }
+  return t;
 }
 
 // size dispatchers usable as the `with_sizes` argument

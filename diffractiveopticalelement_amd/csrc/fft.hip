@@ -31,21 +31,26 @@ static hipError_t fft2d_chunk(doe_ctx* ctx, const Plan2& pl, S* re, S* im, S* xr
   return e;
 }
 
-// Variants exist for FP32 only; other precisions use the built-in choice.
+// Variants exist for FP32 arithmetic (FP32, FP16 and BF16 storage); FP64
+// uses the built-in choice.
 template <typename T, typename S>
 static hipError_t fft2d_chunk_var(doe_ctx* ctx, const doe_tune& tu, const Plan2& pl, S* re, S* im,
                                   S* xre, S* xim, int nb, int inverse) {
-  if constexpr (std::is_same<T, float>::value && std::is_same<S, float>::value) {
+  if constexpr (std::is_same<T, float>::value) {
     const int key = (tu.rowplanes == 1 ? 10 : 0) + tu.colvar;
     switch (key) {
       case 0: return fft2d_chunk<T, S, 2, 0>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 1: return fft2d_chunk<T, S, 2, 1>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 2: return fft2d_chunk<T, S, 2, 2>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 3: return fft2d_chunk<T, S, 2, 3>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 4: return fft2d_chunk<T, S, 2, 4>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 5: return fft2d_chunk<T, S, 2, 5>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 10: return fft2d_chunk<T, S, 1, 0>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 11: return fft2d_chunk<T, S, 1, 1>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 12: return fft2d_chunk<T, S, 1, 2>(ctx, pl, re, im, xre, xim, nb, inverse);
       case 13: return fft2d_chunk<T, S, 1, 3>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 14: return fft2d_chunk<T, S, 1, 4>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 15: return fft2d_chunk<T, S, 1, 5>(ctx, pl, re, im, xre, xim, nb, inverse);
       default: return hipErrorInvalidValue;
     }
   } else {

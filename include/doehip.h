@@ -105,13 +105,17 @@ int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
 
 /* fft2d tuning (no counterpart in the reference).  A context keeps one
  * configuration per (nx, ny, batch, dtype): the column split n1 (1, 2 or 4;
- * 0 = built-in), the FP32 column tile variant colvar (0 built-in; 1: 16
- * points x 64 columns, both LDS planes; 2: 32 points x 64 columns, one plane;
- * 3: 32 points x 32 columns, one plane), the LDS planes of FP32 row passes
- * (1 or 2) and the chunk size in bytes (-1 = the context's).  Variants are
- * built for sides 256..8192.  doe_autotune_fft2d times every configuration
- * that exists for the shape (forward + inverse, `reps` times, on a scratch
- * field), keeps the fastest and writes one line per candidate
+ * 0 = built-in), the column tile variant colvar (0 built-in; 1: 16 points x
+ * 64 columns, both LDS planes; 2: 32 points x 64 columns, one plane; 3: 32
+ * points x 32 columns, one plane; 4: 32 points x 16 columns, one plane; 5: 16
+ * points x 32 columns, one plane), the LDS planes of the row passes (1 or 2)
+ * and the chunk size in bytes (-1 = the context's).  colvar and rowplanes
+ * apply to FP32 arithmetic (FP32, FP16 and BF16 storage); FP64 fields use the
+ * built-in passes.  Untuned FP16 / BF16 shapes with both sides 2048 or more
+ * default to n1 = 1 and one-plane rows.  Variants are built for sides
+ * 256..8192.  doe_autotune_fft2d times every configuration that exists for
+ * the shape (forward + inverse, `reps` times, on a scratch field of uniform
+ * noise), keeps the fastest and writes one line per candidate
  * ("n1=.. colvar=.. rowplanes=.. chunk=.. ms=..", ms per transform) and a
  * final "best ..." line into `report`. */
 int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int colvar,

@@ -47,7 +47,7 @@ def test_every_variant_computes_fft2d(ctx, nx, ny, batch):
     ref = np.fft.fft2(u)
     f = ctx.field(nx, ny, batch, "f32")
     ran = 0
-    for n1, cv, rp, ch in itertools.product((1, 2, 4), (0, 1, 2, 3), (2, 1), (0, 8 << 20)):
+    for n1, cv, rp, ch in itertools.product((1, 2, 4), range(6), (2, 1), (0, 8 << 20)):
         if not valid(nx, ny, n1, cv, rp):
             continue
         ctx.tune_set(nx, ny, batch, "f32", n1, cv, rp, ch)
@@ -61,6 +61,34 @@ def test_every_variant_computes_fft2d(ctx, nx, ny, batch):
         ran += 1
     assert ran >= 8
     ctx.tune_set(nx, ny, batch, "f32")  # back to the built-in configuration
+    f.free()
+
+
+# half-width storage: FP32 arithmetic, rounding to the storage type after
+# each pass (the storage gates of the fft2d parity tests)
+HALF_TOL = {"f16": 2e-3, "bf16": 1.5e-2}
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+def test_half_width_variants_compute_fft2d(ctx, dt):
+    nx, ny, batch = 1024, 1024, 2
+    print(f"tune variants {dt} {nx}x{ny}x{batch}", flush=True)
+    rng = np.random.default_rng(17)
+    u = (rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))) * 0.01
+    ref = np.fft.fft2(u)
+    f = ctx.field(nx, ny, batch, dt)
+    ran = 0
+    for n1, cv, rp in itertools.product((1, 2, 4), range(6), (2, 1)):
+        if not valid(nx, ny, n1, cv, rp):
+            continue
+        ctx.tune_set(nx, ny, batch, dt, n1, cv, rp, 0)
+        f.upload(u.real, u.imag)
+        ctx.fft2d(f)
+        re, im = f.download()
+        assert rel(re + 1j * im, ref) < HALF_TOL[dt], (dt, n1, cv, rp)
+        ran += 1
+    assert ran >= 30
+    ctx.tune_set(nx, ny, batch, dt)
     f.free()
 
 
@@ -91,3 +119,9 @@ def test_tune_argument_errors(ctx):
     # an untuned shape reports the built-in choice
     t = ctx.tune_get(2048, 2048, 16, "f32")
     assert t["n1"] == 4 and t["colvar"] == 0 and t["rowplanes"] == 2
+    # half-width storage of 2048+ lines: plain passes, one-plane rows
+    for dt in ("f16", "bf16"):
+        t = ctx.tune_get(2048, 2048, 16, dt)
+        assert t["n1"] == 1 and t["colvar"] == 0 and t["rowplanes"] == 1
+    t = ctx.tune_get(1024, 1024, 16, "f16")
+    assert t["n1"] == 2 and t["rowplanes"] == 2
