@@ -622,7 +622,7 @@ int doe_tune_get(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int* n1, in
                  int* rowplanes, long* chunk_bytes) {
   DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
   const doe_tune t = tune_for(ctx, nx, ny, batch, dtype);
-  const int builtin_n1 = dtype == DOE_F64 ? col_split<double>(nx, ny) : col_split<float>(nx, ny);
+  const int builtin_n1 = dtype == DOE_F64 ? fft_split<double>(nx, ny) : fft_split<float>(nx, ny);
   if (n1) *n1 = t.n1 > 0 ? t.n1 : builtin_n1;
   if (colvar) *colvar = t.colvar;
   if (rowplanes) *rowplanes = t.rowplanes;
@@ -664,8 +664,8 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   const doe_tune saved_tune = had ? ctx->tune[key] : doe_tune{};
   const int saved_flags = ctx->flags;
   ctx->flags = DOE_CTX_ASYNC;
-  const bool f32 = dtype != DOE_F64;  // FP32 arithmetic: every row / column variant
-  const bool split_dtype = dtype != DOE_F64;
+  const bool f32 = dtype != DOE_F64;  // FP32 arithmetic: every column variant
+  const long split_max = f32 ? 16384 : 8192;  // N1 rows of a row group
   const long fb = long(nx) * ny * 2 * long(dtype_size(dtype));
   std::vector<long> chunks = {0};
   if (batch > 1 && fb * batch > (128L << 20)) chunks.push_back(128L << 20);
@@ -673,11 +673,9 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   double best_ms = 1e30;
   doe_tune best{};
   for (int n1 : {1, 2, 4}) {
-    if (n1 > 1 && (!split_dtype || ny % n1 || ny / n1 < 2 || nx < 256 || long(nx) * n1 > 16384))
-      continue;
+    if (n1 > 1 && (ny % n1 || ny / n1 < 2 || nx < 256 || long(nx) * n1 > split_max)) continue;
     for (int rp : {2, 1}) {
-      if (rp == 1 && !f32) continue;
-      for (int cv = 0; cv <= (f32 ? 5 : 0); ++cv) {
+      for (int cv = 0; cv <= (f32 ? 5 : 1); ++cv) {
         for (long ch : chunks) {
           doe_tune t;
           t.n1 = n1;

@@ -31,8 +31,8 @@ static hipError_t fft2d_chunk(doe_ctx* ctx, const Plan2& pl, S* re, S* im, S* xr
   return e;
 }
 
-// Variants exist for FP32 arithmetic (FP32, FP16 and BF16 storage); FP64
-// uses the built-in choice.
+// Row and column tile variants (six column tiles for FP32 arithmetic, two
+// for FP64).
 template <typename T, typename S>
 static hipError_t fft2d_chunk_var(doe_ctx* ctx, const doe_tune& tu, const Plan2& pl, S* re, S* im,
                                   S* xre, S* xim, int nb, int inverse) {
@@ -54,7 +54,16 @@ static hipError_t fft2d_chunk_var(doe_ctx* ctx, const doe_tune& tu, const Plan2&
       default: return hipErrorInvalidValue;
     }
   } else {
-    return fft2d_chunk<T, S, 2, 0>(ctx, pl, re, im, xre, xim, nb, inverse);
+    // FP64: the built-in tiles and the two-plane column tile, with either row
+    // pass
+    const int key = (tu.rowplanes == 1 ? 10 : 0) + tu.colvar;
+    switch (key) {
+      case 0: return fft2d_chunk<T, S, 2, 0>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 1: return fft2d_chunk<T, S, 2, 1>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 10: return fft2d_chunk<T, S, 1, 0>(ctx, pl, re, im, xre, xim, nb, inverse);
+      case 11: return fft2d_chunk<T, S, 1, 1>(ctx, pl, re, im, xre, xim, nb, inverse);
+      default: return hipErrorInvalidValue;
+    }
   }
 }
 
@@ -62,7 +71,7 @@ template <typename T, typename S>
 int run_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
   const doe_tune tu = tune_for(ctx, f->nx, f->ny, f->batch, f->dtype);
   Plan2 pl;
-  const int n1 = tu.n1 > 0 ? tu.n1 : col_split<T>(f->nx, f->ny);
+  const int n1 = tu.n1 > 0 ? tu.n1 : fft_split<T>(f->nx, f->ny);
   int rc = make_plan2(ctx, f->nx, f->ny, sizeof(T), n1, &pl);
   if (rc) return rc;
   const size_t plane = size_t(f->nx) * size_t(f->ny);

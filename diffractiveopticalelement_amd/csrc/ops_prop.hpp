@@ -43,6 +43,7 @@ struct OpNone {
   static constexpr int NACC = 0;
   static constexpr bool HAS_PRE = false, HAS_POST = false;
   static constexpr bool COL_WIDE = true;
+  static constexpr bool FP64_SPLIT = true;
   // the op for a chunk of the batch starting at field f0
   OpNone at(int) const { return *this; }
   template <typename T>

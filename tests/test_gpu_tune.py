@@ -92,6 +92,35 @@ def test_half_width_variants_compute_fft2d(ctx, dt):
     f.free()
 
 
+@pytest.mark.parametrize("nx,ny,batch", [(1024, 1024, 2), (2048, 2048, 1), (4096, 512, 1)])
+def test_fp64_variants_compute_fft2d(ctx, nx, ny, batch):
+    # FP64: the split (N1 rows of at most 8192 points per row group), the
+    # two column tiles and both row passes
+    print(f"tune variants f64 {nx}x{ny}x{batch}", flush=True)
+    rng = np.random.default_rng(23)
+    u = rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))
+    ref = np.fft.fft2(u)
+    f = ctx.field(nx, ny, batch, "f64")
+    ran = 0
+    for n1, cv, rp in itertools.product((1, 2, 4), (0, 1), (2, 1)):
+        if n1 > 1 and nx * n1 > 8192:
+            continue
+        if not valid(nx, ny, n1, cv, rp):
+            continue
+        ctx.tune_set(nx, ny, batch, "f64", n1, cv, rp, 0)
+        f.upload(u.real, u.imag)
+        ctx.fft2d(f)
+        re, im = f.download()
+        assert rel(re + 1j * im, ref) < 1e-13, (n1, cv, rp)
+        ctx.fft2d(f, inverse=True)
+        re, im = f.download()
+        assert rel(re + 1j * im, u) < 1e-14, (n1, cv, rp)
+        ran += 1
+    assert ran >= 6
+    ctx.tune_set(nx, ny, batch, "f64")
+    f.free()
+
+
 def test_autotune_picks_a_working_configuration(ctx):
     nx, ny, batch = 1024, 1024, 2
     rep = ctx.autotune_fft2d(nx, ny, batch, "f32", reps=2)
@@ -125,3 +154,9 @@ def test_tune_argument_errors(ctx):
         assert t["n1"] == 1 and t["colvar"] == 0 and t["rowplanes"] == 1
     t = ctx.tune_get(1024, 1024, 16, "f16")
     assert t["n1"] == 2 and t["rowplanes"] == 2
+    # FP64: columns of 1024+ points split in two, with one-plane rows
+    t = ctx.tune_get(2048, 2048, 16, "f64")
+    assert t["n1"] == 2 and t["rowplanes"] == 1
+    assert ctx.tune_get(4096, 4096, 1, "f64")["n1"] == 2
+    t = ctx.tune_get(8192, 4096, 1, "f64")
+    assert t["n1"] == 1 and t["rowplanes"] == 2
