@@ -341,13 +341,13 @@ struct ColGeom {
 // 231 us for 64-column tiles at one workgroup per CU and 240 us for 16
 // points per thread with both planes).  Heavier hooks would spill at 32
 // points per thread.
-// FP64 lines of 512 points and more for such passes: 16 points per thread,
-// one plane, up to 16 columns (128-byte row segments; the two-plane tile
-// held 4 columns of a 2048-point line).
+// FP64 lines of 512 points and more, any hooks: 16 points per thread, one
+// plane, up to 16 columns (128-byte row segments; the two-plane tile held 4
+// columns of a 2048-point line).
 template <typename T, int N, bool WIDE_OK = false>
 struct ColCfgSel {
   static constexpr bool WIDE = WIDE_OK && sizeof(T) == 4 && N >= 512;
-  static constexpr bool WIDE64 = WIDE_OK && sizeof(T) == 8 && N >= 512;
+  static constexpr bool WIDE64 = sizeof(T) == 8 && N >= 512;
   using type = typename std::conditional<
       WIDE, ColGeom<T, N, 32, 32, 1>,
       typename std::conditional<WIDE64, ColGeom<T, N, 16, 16, 1>,
