@@ -102,28 +102,52 @@ hipEvent_t prof_event(doe_ctx* ctx) {
 
 int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev) {
   // Identical parameter blocks (repeated calls with the same geometry) reuse
-  // the device copy without a transfer or a stream synchronisation.
-  if (ctx->dparams && bytes == ctx->params_host.size() &&
+  // the device copy without a transfer.
+  if (ctx->pcur >= 0 && bytes == ctx->params_host.size() &&
       std::memcmp(ctx->params_host.data(), host, bytes) == 0) {
-    *dev = ctx->dparams;
+    *dev = ctx->pslot[ctx->pcur].dev;
     return DOE_OK;
   }
-  if (bytes > ctx->dparams_cap) {
-    if (ctx->dparams) {
-      DOE_HIP(hipStreamSynchronize(ctx->stream));
-      DOE_HIP(hipFree(ctx->dparams));
-    }
+  // the current block is free again once the work queued so far has ended
+  if (ctx->pcur >= 0) {
+    doe_ctx::ParamSlot& old = ctx->pslot[ctx->pcur];
+    if (!old.done) DOE_HIP(hipEventCreateWithFlags(&old.done, hipEventDisableTiming));
+    DOE_HIP(hipEventRecord(old.done, ctx->stream));
+    old.pending = true;
+  }
+  const int k = (ctx->pcur + 1) % doe_ctx::kParamSlots;
+  doe_ctx::ParamSlot& sl = ctx->pslot[k];
+  if (sl.pending) {
+    DOE_HIP(hipEventSynchronize(sl.done));  // kernels that read this slot have ended
+    sl.pending = false;
+  }
+  ctx->pcur = -1;  // nothing valid is cached until the copy below is queued
+  ctx->params_host.clear();
+  if (bytes > sl.cap) {
+    if (sl.dev) DOE_HIP(hipFree(sl.dev));
+    if (sl.pin) DOE_HIP(hipHostFree(sl.pin));
+    sl.dev = sl.pin = nullptr;
+    sl.cap = 0;
     size_t cap = 4096;
     while (cap < bytes) cap <<= 1;
-    DOE_HIP(hipMalloc(&ctx->dparams, cap));
-    ctx->dparams_cap = cap;
+    DOE_HIP(hipMalloc(&sl.dev, cap));
+    if (cap <= doe_ctx::kParamPinMax) DOE_HIP(hipHostMalloc(&sl.pin, cap, hipHostMallocDefault));
+    sl.cap = cap;
   }
-  // Ordered on the context stream; the host buffer is consumed before return.
-  DOE_HIP(hipStreamSynchronize(ctx->stream));  // previous users of the block are done
-  DOE_HIP(hipMemcpyAsync(ctx->dparams, host, bytes, hipMemcpyHostToDevice, ctx->stream));
-  DOE_HIP(hipStreamSynchronize(ctx->stream));
+  if (sl.pin) {
+    // pinned staging: the copy is queued in stream order and the host call
+    // returns at once; the staging is rewritten only after the slot's event
+    std::memcpy(sl.pin, host, bytes);
+    DOE_HIP(hipMemcpyAsync(sl.dev, sl.pin, bytes, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    // large blocks (rasters, images): from the caller's memory, consumed
+    // before return
+    DOE_HIP(hipMemcpyAsync(sl.dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    DOE_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->pcur = k;
   ctx->params_host.assign((const char*)host, (const char*)host + bytes);
-  *dev = ctx->dparams;
+  *dev = sl.dev;
   return DOE_OK;
 }
 
@@ -378,7 +402,11 @@ int doe_ctx_destroy(doe_ctx* ctx) {
     hipFree(kv.second.re);
     hipFree(kv.second.im);
   }
-  if (ctx->dparams) hipFree(ctx->dparams);
+  for (auto& sl : ctx->pslot) {
+    if (sl.dev) hipFree(sl.dev);
+    if (sl.pin) hipHostFree(sl.pin);
+    if (sl.done) hipEventDestroy(sl.done);
+  }
   if (ctx->stage) hipFree(ctx->stage);
   if (ctx->partials) hipFree(ctx->partials);
   if (ctx->work_re) hipFree(ctx->work_re);

@@ -75,10 +75,22 @@ struct doe_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::map<std::pair<int, int>, doe::Twiddle> tw;  // (N, sizeof(T)) -> table
-  // device parameter block, refilled per call (small)
-  void* dparams = nullptr;
-  size_t dparams_cap = 0;
-  std::vector<char> params_host;  // last uploaded parameter block
+  // device parameter blocks (per-field geometry and the like), a ring of
+  // slots filled in stream order: a slot is refilled only after the work
+  // queued behind its previous block has ended, so consecutive calls with
+  // different parameters do not drain the stream
+  struct ParamSlot {
+    void* dev = nullptr;
+    void* pin = nullptr;  // pinned staging (blocks up to kParamPinMax)
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  };
+  static constexpr int kParamSlots = 8;
+  static constexpr size_t kParamPinMax = size_t(1) << 20;
+  ParamSlot pslot[kParamSlots];
+  int pcur = -1;                  // slot of the last uploaded block
+  std::vector<char> params_host;  // its contents
   // staging for FP64 host <-> device conversion
   void* stage = nullptr;
   size_t stage_cap = 0;

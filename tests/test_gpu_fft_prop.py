@@ -344,3 +344,41 @@ def test_half_storage_propagators(ctx, dtype):
     ctx.asm(fi, fo, dx, dx, LAM, z)
     re, im = fo.download()
     assert rel((re[0] + 1j * im[0]).ravel(), out["re"] + 1j * out["im"]) < 3 * TOL[dtype]
+
+
+def test_parameter_blocks_in_flight(ctx):
+    # Calls with different per-field parameters queue without draining the
+    # stream (a ring of parameter slots, each refilled only after the work
+    # behind its previous block has ended): 20 asynchronous ASM / Fresnel
+    # calls with distinct distances, more than the ring holds, each equal
+    # bit for bit to the same call run on its own.
+    print("parameter blocks in flight", flush=True)
+    n, lam, dx = 128, 633e-9, 4e-6
+    rng = np.random.default_rng(31)
+    u = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    fin = ctx.upload(u.real, u.imag, "f32")
+    zs = [(0.01 + 0.003 * i, 0.02 + 0.002 * i) for i in range(20)]
+    outs = [ctx.field(n, n, 2, "f32") for _ in zs]
+    ctx.set_flags(asynchronous=True)
+    try:
+        for i, z in enumerate(zs):
+            if i % 3 == 2:
+                ctx.fresnel(fin, outs[i], dx, dx, lam, z)
+            else:
+                ctx.asm(fin, outs[i], dx, dx, lam, z, band_limit=bool(i % 2))
+        ctx.sync()
+    finally:
+        ctx.set_flags(asynchronous=False)
+    got = [o.download() for o in outs]
+    ref = ctx.field(n, n, 2, "f32")
+    for i, z in enumerate(zs):
+        if i % 3 == 2:
+            ctx.fresnel(fin, ref, dx, dx, lam, z)
+        else:
+            ctx.asm(fin, ref, dx, dx, lam, z, band_limit=bool(i % 2))
+        want = ref.download()
+        assert np.array_equal(got[i][0], want[0]) and np.array_equal(got[i][1], want[1]), i
+    for o in outs:
+        o.free()
+    ref.free()
+    fin.free()
