@@ -247,7 +247,16 @@ struct ChirpField {
   double laz;       // lambda*|z|  (divisor kept for the FP64 check mode)
   double dx_in, dy_in, dx_out, dy_out;
   double pre_r, pre_i;  // complex prefactor applied on the output side
+  // FP32 fields: the chirp phase in revolutions per squared grid index,
+  // dx^2 / (2 lambda |z|) on the input and output grids (set by chirp_revs)
+  double rx_in, ry_in, rx_out, ry_out;
 };
+inline void chirp_revs(ChirpField& p) {
+  p.rx_in = 0.5 * p.inv_laz * p.dx_in * p.dx_in;
+  p.ry_in = 0.5 * p.inv_laz * p.dy_in * p.dy_in;
+  p.rx_out = 0.5 * p.inv_laz * p.dx_out * p.dx_out;
+  p.ry_out = 0.5 * p.inv_laz * p.dy_out * p.dy_out;
+}
 
 // mode bits for OpChirp
 enum : int {
@@ -286,22 +295,30 @@ struct OpChirp {
     const ChirpField p = fp[f];
     T cr = T(1), ci = T(0);
     if (mode & CH_QUAD) {
-      const double px = (mode & CH_OUTGRID) ? p.dx_out : p.dx_in;
-      const double py = (mode & CH_OUTGRID) ? p.dy_out : p.dy_in;
-      const double x = double(i - (nx >> 1)) * px;  // (i - nx/2) * dx
-      const double y = double(j - (ny >> 1)) * py;
-      // FP64 fields keep propagate.js:189's division; FP32 fields multiply by
-      // the reciprocal (one rounding apart: below 1e-11 rad at the largest
-      // phase of a 2048^2 grid, far under FP32's resolution) and skip the
-      // division's reciprocal-and-refine sequence at every point.
-      double q;
       if constexpr (sizeof(T) == 8) {
-        q = (kPi * (x * x + y * y)) / p.laz;
+        // FP64 fields: propagate.js:189's operation order
+        const double px = (mode & CH_OUTGRID) ? p.dx_out : p.dx_in;
+        const double py = (mode & CH_OUTGRID) ? p.dy_out : p.dy_in;
+        const double x = double(i - (nx >> 1)) * px;  // (i - nx/2) * dx
+        const double y = double(j - (ny >> 1)) * py;
+        double q = (kPi * (x * x + y * y)) / p.laz;
+        if (mode & CH_MINUS_HALF_PI) q = q - kPi / 2;
+        cis<T>(q, cr, ci);
       } else {
-        q = (kPi * (x * x + y * y)) * p.inv_laz;
+        // FP32 fields: the phase in revolutions from the exact integer
+        // squares of the grid indices, rx*ix^2 + ry*iy^2 (one FMA; the
+        // per-row or per-column term is loop-invariant), then the fraction
+        // and the hardware sine / cosine.  Within 1e-11 rad of the
+        // reference's argument at the largest phase of a 4096^2 grid.
+        const int ix = i - (nx >> 1), iy = j - (ny >> 1);
+        const double rx = (mode & CH_OUTGRID) ? p.rx_out : p.rx_in;
+        const double ry = (mode & CH_OUTGRID) ? p.ry_out : p.ry_in;
+        double rev = fma(rx, double(ix * ix), ry * double(iy * iy));
+        if (mode & CH_MINUS_HALF_PI) rev = rev - 0.25;
+        const float fr = float(rev - rint(rev));
+        cr = __builtin_amdgcn_cosf(fr);
+        ci = __builtin_amdgcn_sinf(fr);
       }
-      if (mode & CH_MINUS_HALF_PI) q = q - kPi / 2;
-      cis<T>(q, cr, ci);
       if (mode & CH_CONJ) ci = -ci;
     }
     if (mode & CH_PREFAC) {

@@ -448,6 +448,7 @@ int doe_fresnel(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, do
       if (out_dx) out_dx[b] = dx;
       if (out_dy) out_dy[b] = dy;
     }
+    chirp_revs(p);
   }
   void* dp = nullptr;
   rc = upload_params(ctx, params.data(), params.size() * sizeof(ChirpField), &dp);
@@ -486,6 +487,7 @@ int doe_lens(doe_ctx* ctx, const doe_field* in, doe_field* out, double dx, doubl
     p.pre_i = 0.0;
     if (out_dx) out_dx[b] = p.dx_out;
     if (out_dy) out_dy[b] = p.dy_out;
+    chirp_revs(p);
   }
   void* dp = nullptr;
   rc = upload_params(ctx, params.data(), params.size() * sizeof(ChirpField), &dp);
