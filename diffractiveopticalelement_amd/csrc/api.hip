@@ -636,7 +636,10 @@ int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int
   DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
   DOE_REQUIRE(n1 == 0 || n1 == 1 || n1 == 2 || n1 == 4, DOE_ERR_INVALID, "tune: n1 must be 0, 1, 2 or 4");
   DOE_REQUIRE(colvar >= 0 && colvar <= 5, DOE_ERR_INVALID, "tune: colvar must be 0..5");
-  DOE_REQUIRE(rowplanes == 1 || rowplanes == 2, DOE_ERR_INVALID, "tune: rowplanes must be 1 or 2");
+  DOE_REQUIRE(dtype != DOE_F64 || colvar <= 1, DOE_ERR_INVALID,
+              "tune: colvar of FP64 fields must be 0 or 1");
+  DOE_REQUIRE(rowplanes >= 0 && rowplanes <= 2, DOE_ERR_INVALID,
+              "tune: rowplanes must be 0 (built-in), 1 or 2");
   doe_tune t;
   t.n1 = n1;
   t.colvar = colvar;
@@ -650,8 +653,7 @@ int doe_tune_get(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int* n1, in
                  int* rowplanes, long* chunk_bytes) {
   DOE_REQUIRE(ctx, DOE_ERR_INVALID, "null context");
   const doe_tune t = tune_for(ctx, nx, ny, batch, dtype);
-  const int builtin_n1 = dtype == DOE_F64 ? fft_split<double>(nx, ny) : fft_split<float>(nx, ny);
-  if (n1) *n1 = t.n1 > 0 ? t.n1 : builtin_n1;
+  if (n1) *n1 = t.n1;
   if (colvar) *colvar = t.colvar;
   if (rowplanes) *rowplanes = t.rowplanes;
   if (chunk_bytes) *chunk_bytes = t.chunk >= 0 ? t.chunk : ctx->chunk_bytes;

@@ -56,7 +56,7 @@ struct Twiddle {
 struct doe_tune {
   int n1 = 0;         // column split (0: the built-in choice)
   int colvar = 0;     // column tile variant (FP32 arithmetic; 0: the built-in choice)
-  int rowplanes = 2;  // LDS planes of the row pass (FP32 arithmetic)
+  int rowplanes = 0;  // LDS planes of the row pass (0: the built-in choice)
   long chunk = -1;    // chunk bytes (-1: the context's)
 };
 
@@ -368,23 +368,26 @@ inline hipError_t col_pass(doe_ctx* ctx, const char* tag, const Op& op, const S*
   return e;
 }
 
-// The tuned configuration for an fft2d shape, or the built-in one.
+// The tuned configuration for an fft2d shape, or the built-in one; fields
+// left at 0 (n1, rowplanes) take the built-in choice, so the result always
+// names the split and the row pass that run.
 // Half-width storage (FP16 / BF16) of 2048+ lines defaults to plain passes
 // with one-plane rows: at 4 B per point the passes are VALU-bound, and the
 // plain row pass (58 VGPRs, 8 waves per SIMD) beats the split's cross-row
 // network (16 x 2048^2 of noise: 0.248-0.257 ms against 0.26-0.27 ms for
 // the split with two-plane rows; the tuner's best, the split with one-plane
-// rows in 128 MiB chunks, 0.246 ms).
+// rows in 128 MiB chunks, 0.246 ms).  FP64 split rows of 1024+ points take
+// one LDS plane (16 x 2048^2: 0.855 against 0.875 ms per transform).
 inline doe_tune tune_for(const doe_ctx* ctx, int nx, int ny, int batch, int dtype) {
-  auto it = ctx->tune.find(std::make_tuple(nx, ny, batch, dtype));
-  if (it != ctx->tune.end()) return it->second;
   doe_tune t{};
-  if ((dtype == DOE_F16 || dtype == DOE_BF16) && nx >= 2048 && ny >= 2048) {
-    t.n1 = 1;
-    t.rowplanes = 1;
-  }
-  // FP64 split rows: one LDS plane (0.855 against 0.875 ms per transform)
-  if (dtype == DOE_F64 && fft_split<double>(nx, ny) > 1) t.rowplanes = 1;
+  auto it = ctx->tune.find(std::make_tuple(nx, ny, batch, dtype));
+  if (it != ctx->tune.end()) t = it->second;
+  const bool half = dtype == DOE_F16 || dtype == DOE_BF16;
+  const bool half_big = half && nx >= 2048 && ny >= 2048;
+  if (t.n1 == 0)
+    t.n1 = half_big ? 1 : (dtype == DOE_F64 ? fft_split<double>(nx, ny) : fft_split<float>(nx, ny));
+  if (t.rowplanes == 0)
+    t.rowplanes = half_big || (dtype == DOE_F64 && t.n1 > 1 && nx >= 1024) ? 1 : 2;
   return t;
 }
 

@@ -279,7 +279,7 @@ class Context:
             out[tag] = (int(cnt), float(ms), int(pts))
         return out
 
-    def tune_set(self, nx, ny, batch, dtype, n1=0, colvar=0, rowplanes=2, chunk=-1) -> None:
+    def tune_set(self, nx, ny, batch, dtype, n1=0, colvar=0, rowplanes=0, chunk=-1) -> None:
         check(self.lib.doe_tune_set(self.h, nx, ny, batch, DTYPES[dtype], n1, colvar, rowplanes,
                                     chunk))
 

@@ -108,11 +108,15 @@ int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
  * 0 = built-in), the column tile variant colvar (0 built-in; 1: 16 points x
  * 64 columns, both LDS planes; 2: 32 points x 64 columns, one plane; 3: 32
  * points x 32 columns, one plane; 4: 32 points x 16 columns, one plane; 5: 16
- * points x 32 columns, one plane), the LDS planes of the row passes (1 or 2)
- * and the chunk size in bytes (-1 = the context's).  colvar and rowplanes
- * apply to FP32 arithmetic (FP32, FP16 and BF16 storage); FP64 fields use the
- * built-in passes.  Untuned FP16 / BF16 shapes with both sides 2048 or more
- * default to n1 = 1 and one-plane rows.  Variants are built for sides
+ * points x 32 columns, one plane), the LDS planes of the row passes (1 or 2;
+ * 0 = built-in) and the chunk size in bytes (-1 = the context's).
+ * doe_tune_get reports the configuration that runs, built-in choices
+ * resolved.  A split the shape cannot take fails the transform with
+ * DOE_ERR_UNSUPPORTED.  colvar and rowplanes
+ * apply to FP32 arithmetic (FP32, FP16 and BF16 storage); FP64 fields take
+ * colvar 0 or 1.  Untuned FP16 / BF16 shapes with both sides 2048 or more
+ * default to n1 = 1 and one-plane rows; FP64 columns of 1024..8192 points
+ * split in two (one-plane rows from 1024-point rows).  Variants are built for sides
  * 256..8192.  doe_autotune_fft2d times every configuration that exists for
  * the shape (forward + inverse, `reps` times, on a scratch field of uniform
  * noise), keeps the fastest and writes one line per candidate

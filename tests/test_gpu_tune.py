@@ -145,6 +145,15 @@ def test_tune_argument_errors(ctx):
         ctx.tune_set(512, 512, 1, "f32", n1=3)
     with pytest.raises(nat.DoeError, match="colvar"):
         ctx.tune_set(512, 512, 1, "f32", colvar=7)
+    with pytest.raises(nat.DoeError, match="colvar of FP64"):
+        ctx.tune_set(512, 512, 1, "f64", colvar=2)
+    # a split the shape cannot take is refused when the transform runs
+    f = ctx.field(128, 256, 1, "f32")
+    ctx.tune_set(128, 256, 1, "f32", n1=4)
+    with pytest.raises(nat.DoeError, match="column split 4 does not fit 128x256"):
+        ctx.fft2d(f)
+    ctx.tune_set(128, 256, 1, "f32")
+    f.free()
     # an untuned shape reports the built-in choice
     t = ctx.tune_get(2048, 2048, 16, "f32")
     assert t["n1"] == 4 and t["colvar"] == 0 and t["rowplanes"] == 2
