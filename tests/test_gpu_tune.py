@@ -102,19 +102,19 @@ def test_fp64_variants_compute_fft2d(ctx, nx, ny, batch):
     ref = np.fft.fft2(u)
     f = ctx.field(nx, ny, batch, "f64")
     ran = 0
-    for n1, cv, rp in itertools.product((1, 2, 4), (0, 1), (2, 1)):
+    for n1, cv, rp, ch in itertools.product((1, 2, 4), (0, 1), (2, 1), (0, 8 << 20)):
         if n1 > 1 and nx * n1 > 8192:
             continue
         if not valid(nx, ny, n1, cv, rp):
             continue
-        ctx.tune_set(nx, ny, batch, "f64", n1, cv, rp, 0)
+        ctx.tune_set(nx, ny, batch, "f64", n1, cv, rp, ch)
         f.upload(u.real, u.imag)
         ctx.fft2d(f)
         re, im = f.download()
-        assert rel(re + 1j * im, ref) < 1e-13, (n1, cv, rp)
+        assert rel(re + 1j * im, ref) < 1e-13, (n1, cv, rp, ch)
         ctx.fft2d(f, inverse=True)
         re, im = f.download()
-        assert rel(re + 1j * im, u) < 1e-14, (n1, cv, rp)
+        assert rel(re + 1j * im, u) < 1e-14, (n1, cv, rp, ch)
         ran += 1
     assert ran >= 6
     ctx.tune_set(nx, ny, batch, "f64")
