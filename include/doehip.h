@@ -60,8 +60,11 @@ int doe_ctx_stream(doe_ctx* ctx, void** stream);
 /* Measurement plumbing (no counterpart in the reference, which times with
  * wall clocks only, tools/bench.mjs:39-47).  DOE_CTX_ASYNC: entry points on
  * device-resident fields return without synchronising the stream.
- * DOE_CTX_PROFILE: every kernel launch is bracketed by HIP events on the
- * context stream; doe_ctx_profile_report synchronises and writes one line
+ * DOE_CTX_PROFILE: every kernel launch is timed on the context stream (the
+ * row and column passes by start / stop events carried on their own
+ * dispatches, other launches by events recorded around them; launches into
+ * a stream under capture are not timed); doe_ctx_profile_report
+ * synchronises and writes one line
  * per kernel tag, "tag count total_ms total_points" (grid points summed over
  * the launches), then clears the records.
  * doe_timer_start/stop bracket a region with two events on the stream. */
