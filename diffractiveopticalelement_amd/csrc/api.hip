@@ -101,28 +101,40 @@ hipEvent_t prof_event(doe_ctx* ctx) {
 }
 
 int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev) {
-  // Identical parameter blocks (repeated calls with the same geometry) reuse
-  // the device copy without a transfer.
-  if (ctx->pcur >= 0 && bytes == ctx->params_host.size() &&
-      std::memcmp(ctx->params_host.data(), host, bytes) == 0) {
+  auto holds = [&](const doe_ctx::ParamSlot& sl) {
+    return sl.host.size() == bytes && std::memcmp(sl.host.data(), host, bytes) == 0;
+  };
+  // the block handed out last, unchanged: no transfer
+  if (ctx->pcur >= 0 && holds(ctx->pslot[ctx->pcur])) {
     *dev = ctx->pslot[ctx->pcur].dev;
     return DOE_OK;
   }
-  // the current block is free again once the work queued so far has ended
+  // the current slot is free again once the work queued so far has ended
   if (ctx->pcur >= 0) {
     doe_ctx::ParamSlot& old = ctx->pslot[ctx->pcur];
     if (!old.done) DOE_HIP(hipEventCreateWithFlags(&old.done, hipEventDisableTiming));
     DOE_HIP(hipEventRecord(old.done, ctx->stream));
     old.pending = true;
   }
-  const int k = (ctx->pcur + 1) % doe_ctx::kParamSlots;
+  // a block still held by another slot (calls that alternate between a few
+  // parameter sets): read it there, no transfer
+  for (int k = 0; k < doe_ctx::kParamSlots; ++k) {
+    if (k != ctx->pcur && ctx->pslot[k].dev && holds(ctx->pslot[k])) {
+      ctx->pcur = k;
+      *dev = ctx->pslot[k].dev;
+      return DOE_OK;
+    }
+  }
+  int k = ctx->pnext;
+  if (k == ctx->pcur) k = (k + 1) % doe_ctx::kParamSlots;
+  ctx->pnext = (k + 1) % doe_ctx::kParamSlots;
   doe_ctx::ParamSlot& sl = ctx->pslot[k];
   if (sl.pending) {
     DOE_HIP(hipEventSynchronize(sl.done));  // kernels that read this slot have ended
     sl.pending = false;
   }
-  ctx->pcur = -1;  // nothing valid is cached until the copy below is queued
-  ctx->params_host.clear();
+  ctx->pcur = -1;  // nothing is handed out until the copy below is queued
+  sl.host.clear();
   if (bytes > sl.cap) {
     if (sl.dev) DOE_HIP(hipFree(sl.dev));
     if (sl.pin) DOE_HIP(hipHostFree(sl.pin));
@@ -145,8 +157,8 @@ int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev) {
     DOE_HIP(hipMemcpyAsync(sl.dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
     DOE_HIP(hipStreamSynchronize(ctx->stream));
   }
+  sl.host.assign((const char*)host, (const char*)host + bytes);
   ctx->pcur = k;
-  ctx->params_host.assign((const char*)host, (const char*)host + bytes);
   *dev = sl.dev;
   return DOE_OK;
 }

@@ -85,14 +85,15 @@ struct doe_ctx {
     void* dev = nullptr;
     void* pin = nullptr;  // pinned staging (blocks up to kParamPinMax)
     size_t cap = 0;
-    hipEvent_t done = nullptr;
+    hipEvent_t done = nullptr;  // recorded each time the slot stops being current
     bool pending = false;
+    std::vector<char> host;  // the block the slot holds (empty: none)
   };
   static constexpr int kParamSlots = 8;
   static constexpr size_t kParamPinMax = size_t(1) << 20;
   ParamSlot pslot[kParamSlots];
-  int pcur = -1;                  // slot of the last uploaded block
-  std::vector<char> params_host;  // its contents
+  int pcur = -1;   // slot of the block handed out last
+  int pnext = 0;   // next slot to refill
   // staging for FP64 host <-> device conversion
   void* stage = nullptr;
   size_t stage_cap = 0;

@@ -351,7 +351,8 @@ def test_parameter_blocks_in_flight(ctx):
     # stream (a ring of parameter slots, each refilled only after the work
     # behind its previous block has ended): 20 asynchronous ASM / Fresnel
     # calls with distinct distances, more than the ring holds, each equal
-    # bit for bit to the same call run on its own.
+    # bit for bit to the same call run on its own; then three of them in
+    # turn, whose blocks the slots still hold.
     print("parameter blocks in flight", flush=True)
     n, lam, dx = 128, 633e-9, 4e-6
     rng = np.random.default_rng(31)
@@ -370,6 +371,24 @@ def test_parameter_blocks_in_flight(ctx):
     finally:
         ctx.set_flags(asynchronous=False)
     got = [o.download() for o in outs]
+    # blocks that repeat (a few parameter sets in turn) are read where a slot
+    # still holds them: the same outputs again, in a cycle of three calls
+    ctx.set_flags(asynchronous=True)
+    try:
+        for r in range(12):
+            i = r % 3
+            o = outs[i]
+            o.upload(np.zeros((2, n, n)), np.zeros((2, n, n)))
+            if i % 3 == 2:
+                ctx.fresnel(fin, o, dx, dx, lam, zs[i])
+            else:
+                ctx.asm(fin, o, dx, dx, lam, zs[i], band_limit=bool(i % 2))
+        ctx.sync()
+    finally:
+        ctx.set_flags(asynchronous=False)
+    for i in range(3):
+        again = outs[i].download()
+        assert np.array_equal(again[0], got[i][0]) and np.array_equal(again[1], got[i][1]), i
     ref = ctx.field(n, n, 2, "f32")
     for i, z in enumerate(zs):
         if i % 3 == 2:
