@@ -126,8 +126,10 @@ namespace doe {
 
 // Fetch (creating on first use) the twiddle table for N in precision T.
 int get_twiddle(doe_ctx* ctx, int n, int tsize, Twiddle* out);
-// Ensure a device parameter block of `bytes` and upload `host` into it on the
-// context stream (synchronous w.r.t. the host buffer).
+// A device copy of the parameter block `host` (`bytes`), queued on the
+// context stream; the host buffer may be reused on return.  Kernels only
+// read the block: a later call with the same bytes may be handed the same
+// device copy.
 int upload_params(doe_ctx* ctx, const void* host, size_t bytes, void** dev);
 // Ensure `n` doubles of reduction partials (ctx->partials).
 int ensure_reduce(doe_ctx* ctx, size_t n);

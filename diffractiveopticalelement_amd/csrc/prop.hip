@@ -368,13 +368,15 @@ int doe_asm_transfer(doe_ctx* ctx, doe_field* out, double dx, double dy, double 
   DOE_HIP(hipSetDevice(ctx->device));
   AsmField p = asm_field_params(out->nx, out->ny, dx, dy, lambda, z, band_limit, soft_edge);
   p.zero = 0;  // the table is built for any z, including 0
-  struct Blk {
-    AsmField p;
-    unsigned long long count;
-  } blk{p, 0ull};
+  // parameter blocks are read-only for kernels (a later call may read the
+  // same block again); the count goes to the reduction scratch
   void* dp = nullptr;
-  rc = upload_params(ctx, &blk, sizeof(blk), &dp);
+  rc = upload_params(ctx, &p, sizeof(p), &dp);
   if (rc) return rc;
+  rc = ensure_reduce(ctx, 1);
+  if (rc) return rc;
+  unsigned long long* cnt = reinterpret_cast<unsigned long long*>(ctx->partials);
+  DOE_HIP(hipMemsetAsync(cnt, 0, sizeof(*cnt), ctx->stream));
   OpAsmH h{};
   h.fp = (const AsmField*)dp;
   h.dfx = 1 / (out->nx * dx);
@@ -383,7 +385,6 @@ int doe_asm_transfer(doe_ctx* ctx, doe_field* out, double dx, double dy, double 
   h.nx = out->nx;
   h.ny = out->ny;
   h.evanescent = evanescent ? 1 : 0;
-  unsigned long long* cnt = (unsigned long long*)((char*)dp + offsetof(Blk, count));
   const long total = (long)out->nx * out->ny;
   const unsigned grid = unsigned((total + 255) / 256);
   if (out->dtype == DOE_F64) {

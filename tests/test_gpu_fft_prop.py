@@ -401,3 +401,15 @@ def test_parameter_blocks_in_flight(ctx):
         o.free()
     ref.free()
     fin.free()
+
+
+def test_asm_transfer_fraction_on_repeated_calls(ctx):
+    # the propagating fraction is counted afresh by every call, whether or
+    # not the call's parameter block is one a previous call uploaded
+    n, dx = 64, 8e-6
+    first = ctx.asm_transfer(n, n, dx, dx, LAM, 0.05, True, True, 0.01, "f64")[2]
+    other = ctx.asm_transfer(n, n, dx, dx, LAM, 0.02, True, True, 0.01, "f64")[2]
+    for _ in range(3):
+        assert ctx.asm_transfer(n, n, dx, dx, LAM, 0.05, True, True, 0.01, "f64")[2] == first
+        assert ctx.asm_transfer(n, n, dx, dx, LAM, 0.02, True, True, 0.01, "f64")[2] == other
+    assert 0 < first <= 1 and 0 < other <= 1
