@@ -707,7 +707,6 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   const int saved_flags = ctx->flags;
   ctx->flags = DOE_CTX_ASYNC;
   const bool f32 = dtype != DOE_F64;  // FP32 arithmetic: every column variant
-  const long split_max = f32 ? 16384 : 8192;  // N1 rows of a row group
   const long fb = long(nx) * ny * 2 * long(dtype_size(dtype));
   std::vector<long> chunks = {0};
   if (batch > 1 && fb * batch > (128L << 20)) chunks.push_back(128L << 20);
@@ -715,7 +714,7 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   double best_ms = 1e30;
   doe_tune best{};
   for (int n1 : {1, 2, 4}) {
-    if (n1 > 1 && (ny % n1 || ny / n1 < 2 || nx < 256 || long(nx) * n1 > split_max)) continue;
+    if (!split_fits(nx, ny, n1, f32 ? 4 : 8)) continue;
     for (int rp : {2, 1}) {
       for (int cv = 0; cv <= (f32 ? 5 : 1); ++cv) {
         for (long ch : chunks) {

@@ -252,6 +252,14 @@ constexpr bool split_ok() {
   return N1 == 1 || (sizeof(T) == 4 && NX >= 256 && NX * N1 <= 16384) ||
          (sizeof(T) == 8 && FP64 && NX >= 256 && NX * N1 <= 8192);
 }
+// Whether the plain fft2d of an nx-by-ny grid can take column split n1 with
+// compute of `tsize` bytes (the split_ok bounds at run time).
+inline bool split_fits(int nx, int ny, int n1, int tsize) {
+  if (n1 == 1) return true;
+  const long rows_max = tsize == 4 ? 16384 : 8192;  // N1 rows of a row group
+  return (n1 == 2 || n1 == 4) && ny % n1 == 0 && ny / n1 >= 2 && nx >= 256 &&
+         long(nx) * n1 <= rows_max;
+}
 // Column split of the plain fft2d: FP32 as col_split; FP64 splits columns
 // of 1024..8192 points in two where the row pair fits (16 x 2048^2 FP64:
 // 0.855 ms split in two with one-plane rows, 0.858 ms in four, 0.883 ms

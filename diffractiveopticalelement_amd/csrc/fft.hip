@@ -73,8 +73,7 @@ int run_fft2d(doe_ctx* ctx, doe_field* f, int inverse) {
   const doe_tune tu = tune_for(ctx, f->nx, f->ny, f->batch, f->dtype);
   Plan2 pl;
   const int n1 = tu.n1;
-  const long split_max = sizeof(T) == 4 ? 16384 : 8192;  // N1 rows of a row group
-  if (n1 > 1 && (f->ny % n1 || f->ny / n1 < 2 || f->nx < 256 || long(f->nx) * n1 > split_max)) {
+  if (!split_fits(f->nx, f->ny, n1, int(sizeof(T)))) {
     set_error("fft2d: column split " + std::to_string(n1) + " does not fit " +
               std::to_string(f->nx) + "x" + std::to_string(f->ny));
     return DOE_ERR_UNSUPPORTED;
