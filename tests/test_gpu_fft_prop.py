@@ -344,6 +344,24 @@ def test_half_storage_propagators(ctx, dtype):
     ctx.asm(fi, fo, dx, dx, LAM, z)
     re, im = fo.download()
     assert rel((re[0] + 1j * im[0]).ravel(), out["re"] + 1j * out["im"]) < 3 * TOL[dtype]
+    # Fresnel forward and inverse, and the lens transform, in the same storage
+    args = {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "z": z}
+    out, res = oj.run("fresnel", args, {"re": u.real.ravel(), "im": u.imag.ravel()})
+    odx, ody = ctx.fresnel(fi, fo, dx, dx, LAM, z)
+    re, im = fo.download()
+    assert rel((re[0] + 1j * im[0]).ravel(), out["re"] + 1j * out["im"]) < 3 * TOL[dtype]
+    back, _ = oj.run("fresnel_inv", {**args, "in_dx": res["dx"], "in_dy": res["dy"]}, out)
+    fb = ctx.field(n, n, 1, dtype)
+    ctx.fresnel(fo, fb, dx, dx, LAM, z, inverse=True, in_dx=odx[0], in_dy=ody[0])
+    br, bi = fb.download()
+    assert rel((br[0] + 1j * bi[0]).ravel(), back["re"] + 1j * back["im"]) < 3 * TOL[dtype]
+    out, _ = oj.run("lens", {"nx": n, "ny": n, "dx": dx, "dy": dx, "lambda": LAM, "f": 0.05},
+                    {"re": u.real.ravel(), "im": u.imag.ravel()})
+    ctx.lens(fi, fo, dx, dx, LAM, 0.05)
+    re, im = fo.download()
+    assert rel((re[0] + 1j * im[0]).ravel(), out["re"] + 1j * out["im"]) < 3 * TOL[dtype]
+    for x in (fi, fo, fb):
+        x.free()
 
 
 def test_parameter_blocks_in_flight(ctx):
