@@ -702,12 +702,15 @@ def main() -> int:
     dom = kernels[dom_tag]
 
     # rest of the configuration's pipeline: ASM fwd + inv (band-limited), Fresnel fwd
+    # three untimed rounds first: the clock has dropped while the host
+    # prepared the leg, and the first rounds run 5-10 % slower
     ctx.set_flags(asynchronous=True, profile=False)
-    ctx.asm(field, scratch, DX, DX, LAM, zs)
-    ctx.asm(scratch, scratch, DX, DX, LAM, -zs)
-    ctx.fresnel(field, scratch, DX, DX, LAM, zs)
+    for _ in range(3):
+        ctx.asm(field, scratch, DX, DX, LAM, zs)
+        ctx.asm(scratch, scratch, DX, DX, LAM, -zs)
+        ctx.fresnel(field, scratch, DX, DX, LAM, zs)
     ctx.sync()
-    reps = max(3, args.steps // 4)
+    reps = max(5, args.steps // 2)
     ctx.set_flags(asynchronous=True, profile=True)
     ctx.timer_start()
     for _ in range(reps):
