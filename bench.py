@@ -419,7 +419,10 @@ def c3_leg(ctx, count: int = 256, iters: int = 200, seed: int = 7, graph: bool =
         b.set(i, ills[n], t)
     b.random_init(seed)
     setup_s = time.perf_counter() - t0
-    b.run(5, graph=graph, per_class=per_class)  # warm-up (captures a 5-iteration graph)
+    # warm-up: the same run, which captures the iters-iteration graph (each
+    # run starts the designs afresh from their initial phases); the timed run
+    # replays it
+    b.run(iters, graph=graph, per_class=per_class)
     ctx.sync()
     ctx.timer_start()
     b.run(iters, graph=graph, per_class=per_class)
