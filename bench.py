@@ -178,7 +178,7 @@ def storage_variants(ctx, field, steps: int) -> dict:
         ctx.fft2d(g)
         ctx.fft2d(g, inverse=True)
         ctx.sync()
-        reps = 2 * max(2, steps // 4)
+        reps = 2 * max(5, steps // 2)
         ctx.timer_start()
         for i in range(reps):
             ctx.fft2d(g, inverse=bool(i & 1))
