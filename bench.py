@@ -172,7 +172,7 @@ def storage_variants(ctx, field, steps: int) -> dict:
             dst = ctx.wrap(dst_re, dst_im, N, N, 1, dt)
             dst.upload(re, im)
             dst.free()
-        ctx.autotune_fft2d(N, N, B, dt, reps=2)  # the configurations that exist for this storage
+        ctx.autotune_fft2d(N, N, B, dt, reps=3)  # the configurations that exist for this storage
         # forward and inverse alternate (as in the main step), so half-width
         # storage stays finite
         ctx.fft2d(g)
@@ -668,7 +668,7 @@ def main() -> int:
         # pick the fastest configuration for this shape (outside the timed
         # region); the warm-up follows at once, as a GPU left idle for a
         # second runs the next few steps 5-8 % slower while its clock rises
-        rep_t = ctx.autotune_fft2d(N, N, B, "f32", reps=2)
+        rep_t = ctx.autotune_fft2d(N, N, B, "f32", reps=3)
         tuning = ctx.tune_get(N, N, B, "f32")
         tuning["candidates"] = len(rep_t.strip().splitlines()) - 1
     ctx.set_flags(asynchronous=True, profile=False)

@@ -713,6 +713,13 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
   std::string rep;
   double best_ms = 1e30;
   doe_tune best{};
+  // bring the clock up first: after an idle spell the first milliseconds of
+  // work run slower and would rank the first candidates low
+  if (had) ctx->tune.erase(key);
+  for (int k = 0; k < 16; ++k) {
+    if (dispatch_fft2d(ctx, f, 0) || dispatch_fft2d(ctx, f, 1)) break;
+  }
+  (void)hipGetLastError();
   for (int n1 : {1, 2, 4}) {
     if (!split_fits(nx, ny, n1, f32 ? 4 : 8)) continue;
     for (int rp : {2, 1}) {
@@ -731,13 +738,18 @@ int doe_autotune_fft2d(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int r
             (void)hipGetLastError();
             continue;  // a variant that does not exist for this shape
           }
-          double ms = 0;
-          r = doe_timer_start(ctx);
-          for (int k = 0; !r && k < reps; ++k) {
-            r = dispatch_fft2d(ctx, f, 0);
-            if (!r) r = dispatch_fft2d(ctx, f, 1);
+          // the better of two timed windows of `reps` pairs
+          double ms = 1e30;
+          for (int w = 0; !r && w < 2; ++w) {
+            double wms = 0;
+            r = doe_timer_start(ctx);
+            for (int k = 0; !r && k < reps; ++k) {
+              r = dispatch_fft2d(ctx, f, 0);
+              if (!r) r = dispatch_fft2d(ctx, f, 1);
+            }
+            if (!r) r = doe_timer_stop(ctx, &wms);
+            if (!r) ms = std::min(ms, wms);
           }
-          if (!r) r = doe_timer_stop(ctx, &ms);
           if (r) continue;
           ms /= 2.0 * reps;  // per transform
           char line[160];

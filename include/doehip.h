@@ -121,8 +121,9 @@ int doe_fft2d(doe_ctx* ctx, doe_field* f, int inverse);
  * default to n1 = 1 and one-plane rows; FP64 columns of 1024..8192 points
  * split in two (one-plane rows from 1024-point rows).  Variants are built for sides
  * 256..8192.  doe_autotune_fft2d times every configuration that exists for
- * the shape (forward + inverse, `reps` times, on a scratch field of uniform
- * noise), keeps the fastest and writes one line per candidate
+ * the shape (the better of two windows of `reps` forward + inverse pairs, on
+ * a scratch field of uniform noise, after 16 untimed pairs that bring the
+ * clock up), keeps the fastest and writes one line per candidate
  * ("n1=.. colvar=.. rowplanes=.. chunk=.. ms=..", ms per transform) and a
  * final "best ..." line into `report`. */
 int doe_tune_set(doe_ctx* ctx, int nx, int ny, int batch, int dtype, int n1, int colvar,
