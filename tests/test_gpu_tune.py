@@ -140,6 +140,31 @@ def test_autotune_picks_a_working_configuration(ctx):
     f.free()
 
 
+@pytest.mark.parametrize("dt,tol", [("f16", 2e-3), ("f64", 1e-13)])
+def test_autotune_other_storage(ctx, dt, tol):
+    # the tuner over the FP16 / FP64 variants: every candidate line parses,
+    # the pick is the fastest line and computes the transform
+    nx, ny, batch = 1024, 1024, 2
+    rep = ctx.autotune_fft2d(nx, ny, batch, dt, reps=2)
+    lines = rep.strip().splitlines()
+    cand = [dict(kv.split("=") for kv in ln.split()) for ln in lines[:-1]]
+    best = dict(kv.split("=") for kv in lines[-1].split()[1:])
+    assert len(cand) >= (8 if dt == "f64" else 30)
+    fastest = min(cand, key=lambda c: float(c["ms"]))
+    assert float(best["ms"]) == float(fastest["ms"])
+    t = ctx.tune_get(nx, ny, batch, dt)
+    assert int(best["n1"]) == t["n1"] and int(best["colvar"]) == t["colvar"]
+    assert int(best["rowplanes"]) == t["rowplanes"]
+    rng = np.random.default_rng(8)
+    u = (rng.standard_normal((batch, ny, nx)) + 1j * rng.standard_normal((batch, ny, nx))) * 0.01
+    f = ctx.upload(u.real, u.imag, dt)
+    ctx.fft2d(f)
+    re, im = f.download()
+    assert rel(re + 1j * im, np.fft.fft2(u)) < tol
+    f.free()
+    ctx.tune_set(nx, ny, batch, dt)
+
+
 def test_tune_argument_errors(ctx):
     with pytest.raises(nat.DoeError, match="n1"):
         ctx.tune_set(512, 512, 1, "f32", n1=3)
