@@ -59,15 +59,15 @@ __device__ __forceinline__ double js_round(double x) {
 // the two quotients differ by a few ulps, which changes the rounding only
 // when the quotient lies that close to a half-integer (an integer boundary
 // gives the same result from either side), so within 2^-40 of one the IEEE
-// division decides.
+// division decides.  Away from a half-integer, floor(y + 0.5) is the
+// round-half-up of y exactly (the sum's rounding error, at most 2^-53 |y|,
+// is far inside the 2^-40 margin), and the fraction test runs beside it
+// (y - floor(y) is exact, the hardware fraction instruction).
 __device__ __forceinline__ double round_quot(double v, double step, double inv) {
   const double y = v * inv;
-  const double f = floor(y);
-  const double d = (y - f) - 0.5;
-  if (fabs(d) > 0x1p-40 * fmax(1.0, fabs(y))) {
-    const double r = d >= 0 ? f + 1 : f;
-    return copysign(r, y);
-  }
+  const double d = __builtin_amdgcn_fract(y) - 0.5;
+  const double r = floor(y + 0.5);
+  if (fabs(d) > 0x1p-40 * fmax(1.0, fabs(y))) return copysign(r, y);
   return js_round(v / step);
 }
 
