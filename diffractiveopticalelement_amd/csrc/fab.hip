@@ -219,18 +219,23 @@ __global__ __launch_bounds__(kRowThreads) void k_diffuse_rows(const double* __re
   for (int j = 0; j < ny; ++j) {
     const int dir = (j & 1) ? -1 : 1;
     if (tid == 0) {
-      double* cur = row(j % 3);
+      // lane 0's lane index (0) added to the row pointers: the compiler then
+      // keeps the chain's LDS addresses in vector registers instead of
+      // copying a scalar address into one before every store
+      const int lane = int(__builtin_amdgcn_mbcnt_lo(~0u, 0u));
+      double* cur = row(j % 3) + lane;
+      double* err_l = err + lane;
       const bool wrap_e = fabs(step) > 3.0;  // |levels| < 2: the error can reach -pi
       if (dir == 1) {
         if (wrap_e)
-          chain_row<1, true>(cur, err, nx, step, inv);
+          chain_row<1, true>(cur, err_l, nx, step, inv);
         else
-          chain_row<1, false>(cur, err, nx, step, inv);
+          chain_row<1, false>(cur, err_l, nx, step, inv);
       } else {
         if (wrap_e)
-          chain_row<-1, true>(cur + nx - 1, err + nx - 1, nx, step, inv);
+          chain_row<-1, true>(cur + nx - 1, err_l + nx - 1, nx, step, inv);
         else
-          chain_row<-1, false>(cur + nx - 1, err + nx - 1, nx, step, inv);
+          chain_row<-1, false>(cur + nx - 1, err_l + nx - 1, nx, step, inv);
       }
     } else if (tid >= 64) {
       // the buffer of row j-1 (levels) becomes row j+2's
